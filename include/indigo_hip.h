@@ -1,0 +1,165 @@
+/*
+ * indigo_hip.h - C ABI of the MI355X (gfx950) implementation of Indigo2's six
+ * graph kernels: BFS, SSSP, CC, MIS, PageRank and triangle counting over the
+ * ECLgraph CSR (nindex / nlist / eweight, all int32).
+ *
+ * Every function returns an int status (INDIGO_OK == 0, negative on failure);
+ * indigo_last_error() returns a per-thread description of the last failure.
+ * Pointers are plain host or device pointers as stated per argument ("d_" =
+ * device memory, "h_" = host memory); no function keeps a pointer after it
+ * returns.
+ *
+ * Two layers:
+ *   1. Drop-in entries (indigo_bfs_vertex ... indigo_tc_edge): one per entry
+ *      function of the reference harness headers, with the same pointer
+ *      ownership, the same outputs and the same timing convention (wall time
+ *      around the iteration loop, allocation and H2D of the graph excluded).
+ *      The C++ harnesses wrap them in the reference's static signatures.
+ *   2. Resident entries (indigo_ctx_*, indigo_bfs ... indigo_tc): the graph
+ *      and the outputs stay in device memory, work is issued on the context's
+ *      stream with cached workspace, and the kernel time of the last call is
+ *      measured with HIP events on that stream.  Benchmarks, the batched
+ *      engine and the serving loop use this layer.
+ */
+#ifndef INDIGO_HIP_H
+#define INDIGO_HIP_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+#define INDIGO_OK 0
+#define INDIGO_ERR_INVALID_ARG (-1)   /* null pointer, bad size, source out of range */
+#define INDIGO_ERR_OUT_OF_MEMORY (-2) /* device or host allocation failed */
+#define INDIGO_ERR_DEVICE (-3)        /* HIP runtime error (launch, copy, sticky fault) */
+#define INDIGO_ERR_UNSORTED (-4)      /* TC input has unsorted lists or self loops */
+#define INDIGO_ERR_TIMEOUT (-5)       /* a bounded device-side wait expired; results invalid */
+#define INDIGO_ERR_CAPACITY (-6)      /* request exceeds a fixed capacity (batch, admission) */
+#define INDIGO_ERR_NO_DEVICE (-7)     /* no usable GPU */
+
+/* Description of the last error on the calling thread ("" if none). */
+const char* indigo_last_error(void);
+
+/* Library version as MAJOR*10000 + MINOR*100 + PATCH. */
+int indigo_version(void);
+
+/* Device properties.  Replaces GPUinfo (include/bfs_vertex_cuda.h:63-73):
+ * *compute_units x *max_threads_per_cu is the persistent-grid bound the
+ * reference derives from SMs x maxThreadsPerSM.  name may be NULL. */
+int indigo_device_info(int device, int* compute_units, int* max_threads_per_cu, char* name, int name_len);
+
+/* Device memory plumbing for harnesses (cudaMalloc / cudaMemcpy / cudaFree
+ * call sites in every include/{alg}_cuda.h main(), e.g. bfs_vertex_cuda.h:140-144). */
+int indigo_malloc(void** d_ptr, size_t bytes);
+int indigo_free(void* d_ptr);
+int indigo_memcpy_h2d(void* d_dst, const void* h_src, size_t bytes);
+int indigo_memcpy_d2h(void* h_dst, const void* d_src, size_t bytes);
+
+/* ---- drop-in entries ---------------------------------------------------- */
+/* Each writes the elapsed seconds of its iteration loop to *seconds (may be
+ * NULL).  Unreachable vertices get INT_MAX (reference bfs_vertex_cuda.h:46). */
+
+/* GPUbfs_vertex, include/bfs_vertex_cuda.h:61: hop distance from src.
+ * d_nindex/d_nlist device, h_dist host int[nodes]. */
+int indigo_bfs_vertex(int src, int nodes, int edges, const int* d_nindex, const int* d_nlist,
+                      int* h_dist, double* seconds);
+
+/* GPUbfs_edge, include/bfs_edge_cuda.h:61: same output; h_sp is the host
+ * edge-source array sp[e] (bfs_edge_cuda.h:140-146), validated for length only. */
+int indigo_bfs_edge(int src, int nodes, int edges, const int* d_nindex, const int* d_nlist,
+                    const int* h_sp, int* h_dist, double* seconds);
+
+/* GPUsssp_vertex, include/sssp_vertex_cuda.h:60: int32 shortest-path
+ * distances with the device weights d_eweight[edges]. */
+int indigo_sssp_vertex(int src, int nodes, int edges, const int* d_nindex, const int* d_nlist,
+                       const int* d_eweight, int* h_dist, double* seconds);
+
+/* GPUsssp_edge, include/sssp_edge_cuda.h:60 (h_sp host, as in the reference). */
+int indigo_sssp_edge(int src, int nodes, int edges, const int* d_nindex, const int* d_nlist,
+                     const int* d_eweight, const int* h_sp, int* h_dist, double* seconds);
+
+/* GPUcc_vertex, include/cc_vertex_cuda.h:57: label[v] = smallest vertex id
+ * in v's connected component. */
+int indigo_cc_vertex(int nodes, int edges, const int* d_nindex, const int* d_nlist,
+                     int* h_label, double* seconds);
+
+/* GPUcc_edge, include/cc_edge_cuda.h:57 (h_sp host). */
+int indigo_cc_edge(int nodes, int edges, const int* d_nindex, const int* d_nlist,
+                   const int* h_sp, int* h_label, double* seconds);
+
+/* GPUmis_vertex, include/mis_vertex_cuda.h:54: status[v] in {1 included,
+ * 2 excluded} of the greedy maximal independent set in descending
+ * (priority, id) order, priority[v] = hash(v + 712313887)
+ * (mis_vertex_cuda.h:174-179).  signed_priority = 0 compares priorities as
+ * unsigned int (the suite's C++ path, the default), 1 as signed int (the
+ * suite's CUDA path).  h_priority (host unsigned[nodes]) receives the
+ * priorities if non-NULL. */
+int indigo_mis_vertex(int nodes, int edges, const int* d_nindex, const int* d_nlist,
+                      int signed_priority, unsigned* h_priority, int* h_status, double* seconds);
+
+/* GPUmis_edge, include/mis_edge_cuda.h:54 (d_sp device, as in the reference). */
+int indigo_mis_edge(int nodes, int edges, const int* d_nindex, const int* d_nlist, const int* d_sp,
+                    int signed_priority, unsigned* h_priority, int* h_status, double* seconds);
+
+/* PR_GPU, include/pr_cuda.h:48: the graph is on the HOST and the callee
+ * copies it.  h_scores (host float[nodes]) holds the initial scores on entry
+ * (the harness fills 1/nodes) and the result on exit; h_degree host int[nodes].
+ * Damping 0.85; stops when the L1 change of an iteration is <= 1e-4 or after
+ * 100 iterations (pr_cuda.h:42-45).  *iterations (may be NULL) gets the count. */
+int indigo_pr(int nodes, int edges, const int* h_nindex, const int* h_nlist, const int* h_degree,
+              float* h_scores, int* iterations, double* seconds);
+
+/* GPUtc_vertex, include/tc_vertex_cuda.h:42: number of triangles of a graph
+ * with sorted, self-loop-free adjacency lists, counted in 64 bits. */
+int indigo_tc_vertex(long long* count, int nodes, const int* d_nindex, const int* d_nlist, double* seconds);
+
+/* GPUtc_edge, include/tc_edge_cuda.h:42 (d_sp device, sp sorted as built by
+ * the harness, tc_edge_cuda.h:159-165). */
+int indigo_tc_edge(long long* count, int edges, const int* d_nindex, const int* d_nlist, const int* d_sp,
+                   double* seconds);
+
+/* ---- resident layer ---------------------------------------------------- */
+typedef struct indigo_ctx indigo_ctx;
+
+/* A context owns one HIP stream (default priority), a workspace cache and a
+ * pair of timing events.  Contexts are independent; one context must not be
+ * used by two host threads at once. */
+int indigo_ctx_create(int device, indigo_ctx** out);
+int indigo_ctx_destroy(indigo_ctx* ctx);
+/* Milliseconds between the first and last kernel of the last resident call,
+ * measured with HIP events on the context stream. */
+int indigo_ctx_last_ms(indigo_ctx* ctx, float* ms);
+/* Number of kernel launches and iterations (levels, rounds, sweeps) of the
+ * last resident call. */
+int indigo_ctx_last_stats(indigo_ctx* ctx, int* launches, int* iterations);
+/* Release the cached workspace (it regrows on the next call). */
+int indigo_ctx_trim(indigo_ctx* ctx);
+/* Optional tuning knob: block size / variant selection hints (0 = auto). */
+int indigo_ctx_set_hint(indigo_ctx* ctx, int key, int value);
+#define INDIGO_HINT_BFS_DIRECTION 1 /* 0 auto, 1 top-down only, 2 allow bottom-up */
+#define INDIGO_HINT_SSSP_DELTA 2    /* 0 auto, else near-far split increment */
+#define INDIGO_HINT_SMALL_FRONTIER 3 /* 0 auto, 1 off: never use the one-workgroup mode */
+
+/* All resident entries take the device CSR as (nodes, edges, d_nindex,
+ * d_nlist[, d_eweight]) and return after the result is in device memory. */
+int indigo_bfs(indigo_ctx* ctx, int nodes, int edges, const int* d_nindex, const int* d_nlist, int src,
+               int* d_dist);
+int indigo_sssp(indigo_ctx* ctx, int nodes, int edges, const int* d_nindex, const int* d_nlist,
+                const int* d_eweight, int src, int* d_dist);
+int indigo_cc(indigo_ctx* ctx, int nodes, int edges, const int* d_nindex, const int* d_nlist, int* d_label);
+int indigo_mis(indigo_ctx* ctx, int nodes, int edges, const int* d_nindex, const int* d_nlist,
+               int signed_priority, int* d_status);
+/* d_scores: device float[nodes]; initialised to 1/nodes by the call. */
+int indigo_pagerank(indigo_ctx* ctx, int nodes, int edges, const int* d_nindex, const int* d_nlist,
+                    float* d_scores, int* iterations);
+int indigo_tc(indigo_ctx* ctx, int nodes, int edges, const int* d_nindex, const int* d_nlist,
+              long long* count);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* INDIGO_HIP_H */

@@ -1,0 +1,218 @@
+// Device-side building blocks shared by the gfx950 graph kernels:
+// relaxed agent-scope atomics, 64-lane wave scans and aggregated appends,
+// and a bounded grid barrier for cooperative (co-resident) launches.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <climits>
+#include <cstdint>
+
+namespace indigo {
+
+constexpr int kWave = 64;
+constexpr int kInf = INT_MAX;  // unreachable marker (reference bfs_vertex_cuda.h:46)
+
+#define IG_DEV __device__ __forceinline__
+
+// ---- relaxed atomics on plain int arrays (the suite's atomicRead/atomicWrite,
+// include/bfs_vertex_cuda.h:49-59, as agent-scope relaxed HIP atomics) ------
+template <typename T>
+IG_DEV T ld_relaxed(const T* p)
+{
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <typename T>
+IG_DEV void st_relaxed(T* p, T v)
+{
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <typename T>
+IG_DEV T atomic_min_relaxed(T* p, T v)
+{
+  return __hip_atomic_fetch_min(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <typename T>
+IG_DEV T atomic_max_relaxed(T* p, T v)
+{
+  return __hip_atomic_fetch_max(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <typename T>
+IG_DEV T atomic_add_relaxed(T* p, T v)
+{
+  return __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <typename T>
+IG_DEV bool cas_relaxed(T* p, T expected, T desired)
+{
+  return __hip_atomic_compare_exchange_strong(p, &expected, desired, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                              __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// ---- wave helpers ---------------------------------------------------------
+IG_DEV int lane_id() { return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u)); }
+
+// Number of set bits of `mask` in lanes below the calling lane.
+IG_DEV int lanes_below(unsigned long long mask)
+{
+  return __builtin_amdgcn_mbcnt_hi(static_cast<unsigned>(mask >> 32),
+                                   __builtin_amdgcn_mbcnt_lo(static_cast<unsigned>(mask), 0u));
+}
+
+template <typename T>
+IG_DEV T wave_shfl(T v, int src_lane)
+{
+  return __shfl(v, src_lane, kWave);
+}
+
+// Inclusive prefix sum over the 64 lanes (all lanes must be active).
+IG_DEV int wave_inclusive_scan(int x)
+{
+  const int lane = lane_id();
+#pragma unroll
+  for (int d = 1; d < kWave; d <<= 1) {
+    const int y = __shfl_up(x, d, kWave);
+    if (lane >= d) x += y;
+  }
+  return x;
+}
+
+template <typename T>
+IG_DEV T wave_sum(T x)
+{
+#pragma unroll
+  for (int d = kWave / 2; d > 0; d >>= 1) x += __shfl_xor(x, d, kWave);
+  return x;
+}
+
+template <typename T>
+IG_DEV T wave_min(T x)
+{
+#pragma unroll
+  for (int d = kWave / 2; d > 0; d >>= 1) {
+    const T y = __shfl_xor(x, d, kWave);
+    x = y < x ? y : x;
+  }
+  return x;
+}
+
+// Appends `item` for every lane with `pred` to queue[] using one atomic per
+// wave on *counter; returns the slot (or -1).  Lanes that are inactive at the
+// call simply do not take part.
+template <typename C>
+IG_DEV int wave_append(bool pred, int item, int* queue, C* counter)
+{
+  const unsigned long long m = __ballot(pred);
+  if (m == 0) return -1;
+  const int leader = __ffsll(static_cast<long long>(m)) - 1;
+  C base = 0;
+  if (lane_id() == leader) base = atomic_add_relaxed(counter, static_cast<C>(__popcll(m)));
+  base = wave_shfl(base, leader);
+  if (!pred) return -1;
+  const int slot = static_cast<int>(base) + lanes_below(m);
+  queue[slot] = item;
+  return slot;
+}
+
+// Owner search for the wave load balancer: given the non-decreasing inclusive
+// prefix `incl` held by lanes 0..63, returns the number of lanes whose incl <= k,
+// i.e. the lane whose [incl - deg, incl) range contains k.
+IG_DEV int wave_owner(int incl, int k)
+{
+  int lo = 0;
+#pragma unroll
+  for (int step = 32; step > 0; step >>= 1) {
+    const int probe = lo + step - 1;
+    const int v = __shfl(incl, probe < kWave ? probe : kWave - 1, kWave);
+    if (probe < kWave && v <= k) lo += step;
+  }
+  return lo;
+}
+
+// ---- bounded grid barrier -------------------------------------------------
+// Workgroups are dispatched round-robin over the 8 XCDs, so workgroup b
+// arrives on counter b % 8 first; the last arrival of each group bumps the
+// top counter and the very last one advances the generation.  Every wait is
+// bounded by a wall-clock deadline: on expiry the waiter raises `abort` and
+// all barriers from then on return false, so every wave leaves the kernel.
+struct alignas(128) PaddedU32 {
+  unsigned int v;
+  unsigned int pad[31];
+};
+
+struct GridBarrier {
+  PaddedU32 group[8];
+  PaddedU32 top;
+  PaddedU32 gen;
+  PaddedU32 abort;
+};
+
+IG_DEV long long now_ticks() { return wall_clock64(); }
+
+IG_DEV bool grid_barrier(GridBarrier* gb, long long deadline)
+{
+  __shared__ int s_ok;
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int ok = 1;
+    const unsigned nblocks = gridDim.x;
+    const unsigned g = blockIdx.x & 7u;
+    const unsigned groups = nblocks < 8u ? nblocks : 8u;
+    const unsigned gsize = (nblocks - g + 7u) >> 3;
+    const unsigned gen0 = __hip_atomic_load(&gb->gen.v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    bool last = false;
+    const unsigned a = __hip_atomic_fetch_add(&gb->group[g].v, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    if (a == gsize - 1u) {
+      __hip_atomic_store(&gb->group[g].v, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      const unsigned t = __hip_atomic_fetch_add(&gb->top.v, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+      if (t == groups - 1u) {
+        __hip_atomic_store(&gb->top.v, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(&gb->gen.v, gen0 + 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        last = true;
+      }
+    }
+    if (!last) {
+      while (__hip_atomic_load(&gb->gen.v, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == gen0) {
+        if (__hip_atomic_load(&gb->abort.v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u) {
+          ok = 0;
+          break;
+        }
+        if (now_ticks() > deadline) {
+          __hip_atomic_store(&gb->abort.v, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          ok = 0;
+          break;
+        }
+        __builtin_amdgcn_s_sleep(1);
+      }
+    }
+    if (__hip_atomic_load(&gb->abort.v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u) ok = 0;
+    s_ok = ok;
+  }
+  __syncthreads();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  return s_ok != 0;
+}
+
+// Same as grid_barrier's abort test, for loops that run between barriers
+// (one-workgroup phases): true when the kernel must stop.
+IG_DEV bool kernel_aborted(GridBarrier* gb, long long deadline)
+{
+  if (__hip_atomic_load(&gb->abort.v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u) return true;
+  if (now_ticks() > deadline) {
+    __hip_atomic_store(&gb->abort.v, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return true;
+  }
+  return false;
+}
+
+// MIS priority: the suite's integer hash (include/mis_vertex_cuda.h:174-179,
+// xor-shift-multiply by 0x45d9f3b twice) of v + 712313887 (init kernels, e.g.
+// MIS_CUDA_V_Topo_Push_Determ_IntType_NonPersist_Atomic_Thread.cu:51).
+__host__ __device__ inline unsigned int mis_hash(unsigned int val)
+{
+  val = ((val >> 16) ^ val) * 0x45d9f3bu;
+  val = ((val >> 16) ^ val) * 0x45d9f3bu;
+  return (val >> 16) ^ val;
+}
+
+}  // namespace indigo

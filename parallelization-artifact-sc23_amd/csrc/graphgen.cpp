@@ -1,0 +1,395 @@
+// Host-side CSR construction, synthetic generators and .egr I/O.
+// See include/indigo_graph.h for the contract.  Host C++17 only.
+#include "indigo_graph.h"
+#include "indigo_hip.h"
+#include "errors.hpp"
+
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <thread>
+#include <vector>
+
+namespace {
+
+using indigo::set_error;
+
+inline uint64_t mix64(uint64_t x)
+{
+  // splitmix64 finaliser (Steele, Lea, Flood: "Fast splittable pseudorandom
+  // number generators", OOPSLA 2014)
+  x += 0x9E3779B97F4A7C15ull;
+  x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+  x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+  return x ^ (x >> 31);
+}
+
+// Counter-based stream: draw k of stream s is mix64(key(s) + k * golden).
+struct Stream {
+  uint64_t state;
+  explicit Stream(uint64_t seed, uint64_t id) : state(mix64(seed ^ mix64(id + 0x632BE59BD9B4E019ull))) {}
+  uint64_t next() { state += 0x9E3779B97F4A7C15ull; return mix64(state); }
+  double uniform() { return (next() >> 11) * (1.0 / 9007199254740992.0); }  // [0, 1)
+};
+
+inline int edge_weight(uint64_t seed, int u, int v, int wmin, int wmax)
+{
+  const uint64_t lo = static_cast<uint32_t>(std::min(u, v));
+  const uint64_t hi = static_cast<uint32_t>(std::max(u, v));
+  const uint64_t h = mix64(seed ^ mix64((lo << 32) | hi));
+  const uint64_t range = static_cast<uint64_t>(wmax - wmin) + 1;
+  return wmin + static_cast<int>(h % range);
+}
+
+int pick_threads(int threads)
+{
+  if (threads > 0) return std::min(threads, 64);
+  const unsigned hc = std::thread::hardware_concurrency();
+  return static_cast<int>(std::max(1u, std::min(hc, 16u)));
+}
+
+template <typename F>
+void parallel_for(long long n, int threads, F&& fn)
+{
+  if (n <= 0) return;
+  const int t = static_cast<int>(std::min<long long>(threads, std::max<long long>(1, n / 4096 + 1)));
+  if (t <= 1) {
+    fn(0LL, n, 0);
+    return;
+  }
+  std::vector<std::thread> pool;
+  pool.reserve(t);
+  for (int i = 0; i < t; i++) {
+    const long long b = n * i / t, e = n * (i + 1) / t;
+    pool.emplace_back([&fn, b, e, i] { fn(b, e, i); });
+  }
+  for (auto& th : pool) th.join();
+}
+
+bool weights_wanted(int wmin, int wmax) { return wmin >= 1 && wmax >= wmin; }
+
+void clear(indigo_host_graph* g)
+{
+  g->nodes = 0;
+  g->edges = 0;
+  g->nindex = nullptr;
+  g->nlist = nullptr;
+  g->eweight = nullptr;
+}
+
+// Edge-list -> simple undirected CSR.  Counting sort by source, then each
+// list sorted and de-duplicated in parallel, then compacted.
+int build_csr_impl(int nodes, long long npairs, const int* src, const int* dst, bool symmetrize,
+                   uint64_t wseed, int wmin, int wmax, int threads, indigo_host_graph* out)
+{
+  if (out == nullptr || nodes < 1 || npairs < 0 || (npairs > 0 && (src == nullptr || dst == nullptr)))
+    return set_error(INDIGO_ERR_INVALID_ARG, "build_csr: bad arguments");
+  clear(out);
+  const int nt = pick_threads(threads);
+  for (long long i = 0; i < npairs; i++) {
+    if (src[i] < 0 || src[i] >= nodes || dst[i] < 0 || dst[i] >= nodes)
+      return set_error(INDIGO_ERR_INVALID_ARG, "build_csr: vertex id out of range");
+  }
+
+  std::vector<std::atomic<long long>> cnt(static_cast<size_t>(nodes) + 1);
+  for (auto& c : cnt) c.store(0, std::memory_order_relaxed);
+  parallel_for(npairs, nt, [&](long long b, long long e, int) {
+    for (long long i = b; i < e; i++) {
+      const int s = src[i], d = dst[i];
+      if (s == d) continue;
+      cnt[s].fetch_add(1, std::memory_order_relaxed);
+      if (symmetrize) cnt[d].fetch_add(1, std::memory_order_relaxed);
+    }
+  });
+  std::vector<long long> off(static_cast<size_t>(nodes) + 1, 0);
+  for (int v = 0; v < nodes; v++) off[v + 1] = off[v] + cnt[v].load(std::memory_order_relaxed);
+  const long long raw = off[nodes];
+  std::vector<int> buf(static_cast<size_t>(std::max<long long>(raw, 1)));
+  for (int v = 0; v < nodes; v++) cnt[v].store(off[v], std::memory_order_relaxed);
+  parallel_for(npairs, nt, [&](long long b, long long e, int) {
+    for (long long i = b; i < e; i++) {
+      const int s = src[i], d = dst[i];
+      if (s == d) continue;
+      buf[cnt[s].fetch_add(1, std::memory_order_relaxed)] = d;
+      if (symmetrize) buf[cnt[d].fetch_add(1, std::memory_order_relaxed)] = s;
+    }
+  });
+  // sort + unique each list; new degree into deg[]
+  std::vector<int> deg(nodes);
+  parallel_for(nodes, nt, [&](long long b, long long e, int) {
+    for (long long v = b; v < e; v++) {
+      int* first = buf.data() + off[v];
+      int* last = buf.data() + off[v + 1];
+      std::sort(first, last);
+      deg[v] = static_cast<int>(std::unique(first, last) - first);
+    }
+  });
+  long long total = 0;
+  for (int v = 0; v < nodes; v++) total += deg[v];
+  if (total > 0x7fffffffLL) return set_error(INDIGO_ERR_CAPACITY, "build_csr: more than 2^31-1 CSR entries");
+
+  out->nodes = nodes;
+  out->edges = static_cast<int>(total);
+  out->nindex = static_cast<int*>(malloc(sizeof(int) * (static_cast<size_t>(nodes) + 1)));
+  out->nlist = static_cast<int*>(malloc(sizeof(int) * static_cast<size_t>(std::max<long long>(total, 1))));
+  const bool want_w = weights_wanted(wmin, wmax);
+  out->eweight = want_w ? static_cast<int*>(malloc(sizeof(int) * static_cast<size_t>(std::max<long long>(total, 1)))) : nullptr;
+  if (!out->nindex || !out->nlist || (want_w && !out->eweight)) {
+    indigo_graph_release(out);
+    return set_error(INDIGO_ERR_OUT_OF_MEMORY, "build_csr: host allocation failed");
+  }
+  out->nindex[0] = 0;
+  for (int v = 0; v < nodes; v++) out->nindex[v + 1] = out->nindex[v] + deg[v];
+  parallel_for(nodes, nt, [&](long long b, long long e, int) {
+    for (long long v = b; v < e; v++) {
+      const int o = out->nindex[v];
+      std::memcpy(out->nlist + o, buf.data() + off[v], sizeof(int) * deg[v]);
+      if (want_w)
+        for (int j = 0; j < deg[v]; j++) out->eweight[o + j] = edge_weight(wseed, static_cast<int>(v), out->nlist[o + j], wmin, wmax);
+    }
+  });
+  return INDIGO_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int indigo_build_csr(int nodes, long long npairs, const int* src, const int* dst, int symmetrize,
+                     unsigned long long wseed, int wmin, int wmax, int threads, indigo_host_graph* out)
+{
+  return build_csr_impl(nodes, npairs, src, dst, symmetrize != 0, wseed, wmin, wmax, threads, out);
+}
+
+int indigo_gen_grid(int rows, int cols, unsigned long long seed, int wmin, int wmax, indigo_host_graph* out)
+{
+  if (out == nullptr || rows < 1 || cols < 1 || static_cast<long long>(rows) * cols > 0x7fffffffLL / 5)
+    return set_error(INDIGO_ERR_INVALID_ARG, "gen_grid: bad size");
+  clear(out);
+  const int n = rows * cols;
+  long long e = 0;
+  std::vector<int> deg(n);
+  for (int r = 0; r < rows; r++)
+    for (int c = 0; c < cols; c++)
+      deg[r * cols + c] = (r > 0) + (c > 0) + (c + 1 < cols) + (r + 1 < rows);
+  for (int v = 0; v < n; v++) e += deg[v];
+  out->nodes = n;
+  out->edges = static_cast<int>(e);
+  out->nindex = static_cast<int*>(malloc(sizeof(int) * (static_cast<size_t>(n) + 1)));
+  out->nlist = static_cast<int*>(malloc(sizeof(int) * static_cast<size_t>(std::max<long long>(e, 1))));
+  const bool want_w = weights_wanted(wmin, wmax);
+  out->eweight = want_w ? static_cast<int*>(malloc(sizeof(int) * static_cast<size_t>(std::max<long long>(e, 1)))) : nullptr;
+  if (!out->nindex || !out->nlist || (want_w && !out->eweight)) {
+    indigo_graph_release(out);
+    return set_error(INDIGO_ERR_OUT_OF_MEMORY, "gen_grid: host allocation failed");
+  }
+  int k = 0;
+  out->nindex[0] = 0;
+  for (int r = 0; r < rows; r++) {
+    for (int c = 0; c < cols; c++) {
+      const int v = r * cols + c;
+      // ascending ids: up, left, right, down
+      if (r > 0) out->nlist[k++] = v - cols;
+      if (c > 0) out->nlist[k++] = v - 1;
+      if (c + 1 < cols) out->nlist[k++] = v + 1;
+      if (r + 1 < rows) out->nlist[k++] = v + cols;
+      out->nindex[v + 1] = k;
+    }
+  }
+  if (want_w)
+    for (int v = 0; v < n; v++)
+      for (int j = out->nindex[v]; j < out->nindex[v + 1]; j++) out->eweight[j] = edge_weight(seed, v, out->nlist[j], wmin, wmax);
+  return INDIGO_OK;
+}
+
+int indigo_gen_rmat(int scale, int edge_factor, double a, double b, double c, unsigned long long seed,
+                    int wmin, int wmax, int threads, indigo_host_graph* out)
+{
+  if (out == nullptr || scale < 1 || scale > 30 || edge_factor < 1 || a < 0 || b < 0 || c < 0 || a + b + c > 1.0)
+    return set_error(INDIGO_ERR_INVALID_ARG, "gen_rmat: bad parameters");
+  const long long n = 1LL << scale;
+  const long long samples = static_cast<long long>(edge_factor) * n;
+  if (2 * samples > 0x7fffffffLL * 2) return set_error(INDIGO_ERR_CAPACITY, "gen_rmat: too many samples");
+  std::vector<int> s(samples), d(samples);
+  const int nt = pick_threads(threads);
+  const double ab = a + b, abc = a + b + c;
+  parallel_for(samples, nt, [&](long long beg, long long end, int) {
+    for (long long i = beg; i < end; i++) {
+      Stream rng(seed, static_cast<uint64_t>(i));
+      int u = 0, v = 0;
+      for (int lvl = 0; lvl < scale; lvl++) {
+        const double r = rng.uniform();
+        const int ubit = r >= ab;                       // c or d quadrant -> lower half rows
+        const int vbit = (r >= a && r < ab) || r >= abc;  // b or d quadrant -> right half cols
+        u = (u << 1) | ubit;
+        v = (v << 1) | vbit;
+      }
+      s[i] = u;
+      d[i] = v;
+    }
+  });
+  return build_csr_impl(static_cast<int>(n), samples, s.data(), d.data(), true, seed ^ 0x5eedULL, wmin, wmax, threads, out);
+}
+
+int indigo_gen_er(int nodes, long long samples, unsigned long long seed, int wmin, int wmax, int threads,
+                  indigo_host_graph* out)
+{
+  if (out == nullptr || nodes < 1 || samples < 0) return set_error(INDIGO_ERR_INVALID_ARG, "gen_er: bad parameters");
+  std::vector<int> s(samples), d(samples);
+  const int nt = pick_threads(threads);
+  parallel_for(samples, nt, [&](long long beg, long long end, int) {
+    for (long long i = beg; i < end; i++) {
+      Stream rng(seed, static_cast<uint64_t>(i));
+      s[i] = static_cast<int>(rng.next() % static_cast<uint64_t>(nodes));
+      d[i] = static_cast<int>(rng.next() % static_cast<uint64_t>(nodes));
+    }
+  });
+  return build_csr_impl(nodes, samples, s.data(), d.data(), true, seed ^ 0x5eedULL, wmin, wmax, threads, out);
+}
+
+int indigo_gen_road(int rows, int cols, double keep_p, double diag_p, unsigned long long seed, int wmin,
+                    int wmax, int threads, indigo_host_graph* out)
+{
+  if (out == nullptr || rows < 1 || cols < 1 || keep_p < 0 || keep_p > 1 || diag_p < 0 || diag_p > 1 ||
+      static_cast<long long>(rows) * cols > 0x7fffffffLL / 8)
+    return set_error(INDIGO_ERR_INVALID_ARG, "gen_road: bad parameters");
+  const long long n = static_cast<long long>(rows) * cols;
+  // per vertex: right edge, down edge, optional diagonal -> at most 3 pairs
+  std::vector<int> s(3 * n), d(3 * n);
+  std::vector<unsigned char> used(3 * n, 0);
+  const int nt = pick_threads(threads);
+  parallel_for(n, nt, [&](long long beg, long long end, int) {
+    for (long long v = beg; v < end; v++) {
+      Stream rng(seed, static_cast<uint64_t>(v));
+      const int r = static_cast<int>(v / cols), c = static_cast<int>(v % cols);
+      const double pr = rng.uniform(), pd = rng.uniform(), pg = rng.uniform();
+      const uint64_t pick = rng.next();
+      if (c + 1 < cols && pr < keep_p) { s[3 * v] = static_cast<int>(v); d[3 * v] = static_cast<int>(v + 1); used[3 * v] = 1; }
+      if (r + 1 < rows && pd < keep_p) { s[3 * v + 1] = static_cast<int>(v); d[3 * v + 1] = static_cast<int>(v + cols); used[3 * v + 1] = 1; }
+      if (pg < diag_p) {
+        int cand[4], m = 0;
+        const int dr[4] = {-1, -1, 1, 1}, dc[4] = {-1, 1, -1, 1};
+        for (int k = 0; k < 4; k++) {
+          const int rr = r + dr[k], cc = c + dc[k];
+          if (rr >= 0 && rr < rows && cc >= 0 && cc < cols) cand[m++] = rr * cols + cc;
+        }
+        if (m > 0) { s[3 * v + 2] = static_cast<int>(v); d[3 * v + 2] = cand[pick % m]; used[3 * v + 2] = 1; }
+      }
+    }
+  });
+  long long k = 0;
+  for (long long i = 0; i < 3 * n; i++) {
+    if (used[i]) { s[k] = s[i]; d[k] = d[i]; k++; }
+  }
+  return build_csr_impl(static_cast<int>(n), k, s.data(), d.data(), true, seed ^ 0x5eedULL, wmin, wmax, threads, out);
+}
+
+int indigo_graph_copy(const indigo_host_graph* src, indigo_host_graph* out)
+{
+  if (src == nullptr || out == nullptr || src->nodes < 1 || src->edges < 0)
+    return set_error(INDIGO_ERR_INVALID_ARG, "graph_copy: bad arguments");
+  clear(out);
+  out->nodes = src->nodes;
+  out->edges = src->edges;
+  const size_t ne = static_cast<size_t>(std::max(src->edges, 1));
+  out->nindex = static_cast<int*>(malloc(sizeof(int) * (static_cast<size_t>(src->nodes) + 1)));
+  out->nlist = static_cast<int*>(malloc(sizeof(int) * ne));
+  out->eweight = src->eweight ? static_cast<int*>(malloc(sizeof(int) * ne)) : nullptr;
+  if (!out->nindex || !out->nlist || (src->eweight && !out->eweight)) {
+    indigo_graph_release(out);
+    return set_error(INDIGO_ERR_OUT_OF_MEMORY, "graph_copy: host allocation failed");
+  }
+  std::memcpy(out->nindex, src->nindex, sizeof(int) * (static_cast<size_t>(src->nodes) + 1));
+  std::memcpy(out->nlist, src->nlist, sizeof(int) * static_cast<size_t>(src->edges));
+  if (src->eweight) std::memcpy(out->eweight, src->eweight, sizeof(int) * static_cast<size_t>(src->edges));
+  return INDIGO_OK;
+}
+
+int indigo_graph_release(indigo_host_graph* g)
+{
+  if (g == nullptr) return set_error(INDIGO_ERR_INVALID_ARG, "graph_release: null");
+  free(g->nindex);
+  free(g->nlist);
+  free(g->eweight);
+  clear(g);
+  return INDIGO_OK;
+}
+
+int indigo_read_egr(const char* path, indigo_host_graph* out)
+{
+  if (path == nullptr || out == nullptr) return set_error(INDIGO_ERR_INVALID_ARG, "read_egr: null argument");
+  clear(out);
+  FILE* f = fopen(path, "rb");
+  if (f == nullptr) return set_error(INDIGO_ERR_INVALID_ARG, "read_egr: could not open file");
+  int hdr[2];
+  if (fread(hdr, sizeof(int), 2, f) != 2 || hdr[0] < 1 || hdr[1] < 0) {
+    fclose(f);
+    return set_error(INDIGO_ERR_INVALID_ARG, "read_egr: bad header");
+  }
+  out->nodes = hdr[0];
+  out->edges = hdr[1];
+  const size_t nv = static_cast<size_t>(hdr[0]) + 1, ne = static_cast<size_t>(hdr[1]);
+  out->nindex = static_cast<int*>(malloc(sizeof(int) * nv));
+  out->nlist = static_cast<int*>(malloc(sizeof(int) * std::max<size_t>(ne, 1)));
+  out->eweight = static_cast<int*>(malloc(sizeof(int) * std::max<size_t>(ne, 1)));
+  int rc = INDIGO_OK;
+  if (!out->nindex || !out->nlist || !out->eweight) rc = set_error(INDIGO_ERR_OUT_OF_MEMORY, "read_egr: allocation failed");
+  else if (fread(out->nindex, sizeof(int), nv, f) != nv) rc = set_error(INDIGO_ERR_INVALID_ARG, "read_egr: truncated nindex");
+  else if (fread(out->nlist, sizeof(int), ne, f) != ne) rc = set_error(INDIGO_ERR_INVALID_ARG, "read_egr: truncated nlist");
+  else {
+    const size_t got = ne ? fread(out->eweight, sizeof(int), ne, f) : 0;
+    if (got == 0) {
+      free(out->eweight);
+      out->eweight = nullptr;
+    } else if (got != ne) {
+      rc = set_error(INDIGO_ERR_INVALID_ARG, "read_egr: truncated weights");
+    }
+  }
+  fclose(f);
+  if (rc != INDIGO_OK) indigo_graph_release(out);
+  return rc;
+}
+
+int indigo_write_egr(const char* path, const indigo_host_graph* g)
+{
+  if (path == nullptr || g == nullptr || g->nodes < 1 || g->edges < 0 || !g->nindex || (!g->nlist && g->edges))
+    return set_error(INDIGO_ERR_INVALID_ARG, "write_egr: bad arguments");
+  FILE* f = fopen(path, "wb");
+  if (f == nullptr) return set_error(INDIGO_ERR_INVALID_ARG, "write_egr: could not open file");
+  const int hdr[2] = {g->nodes, g->edges};
+  bool ok = fwrite(hdr, sizeof(int), 2, f) == 2;
+  ok = ok && fwrite(g->nindex, sizeof(int), static_cast<size_t>(g->nodes) + 1, f) == static_cast<size_t>(g->nodes) + 1;
+  ok = ok && fwrite(g->nlist, sizeof(int), static_cast<size_t>(g->edges), f) == static_cast<size_t>(g->edges);
+  if (g->eweight) ok = ok && fwrite(g->eweight, sizeof(int), static_cast<size_t>(g->edges), f) == static_cast<size_t>(g->edges);
+  ok = (fclose(f) == 0) && ok;
+  return ok ? INDIGO_OK : set_error(INDIGO_ERR_INVALID_ARG, "write_egr: write failed");
+}
+
+int indigo_graph_check(const indigo_host_graph* g)
+{
+  if (g == nullptr || g->nodes < 1 || g->edges < 0 || !g->nindex) return 1;
+  if (g->nindex[0] != 0 || g->nindex[g->nodes] != g->edges) return 1;
+  for (int v = 0; v < g->nodes; v++)
+    if (g->nindex[v + 1] < g->nindex[v]) return 1;
+  for (int v = 0; v < g->nodes; v++) {
+    for (int j = g->nindex[v]; j < g->nindex[v + 1]; j++) {
+      const int u = g->nlist[j];
+      if (u < 0 || u >= g->nodes) return 2;
+      if (j > g->nindex[v] && g->nlist[j - 1] >= u) return 3;
+      if (u == v) return 4;
+      // reverse edge by binary search in u's (sorted) list
+      const int* b = g->nlist + g->nindex[u];
+      const int* e = g->nlist + g->nindex[u + 1];
+      const int* p = std::lower_bound(b, e, v);
+      if (p == e || *p != v) return 5;
+      if (g->eweight && g->eweight[p - g->nlist] != g->eweight[j]) return 6;
+    }
+  }
+  return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,288 @@
+// Context management, workspace, cooperative launch and device plumbing.
+#include "runtime.hpp"
+
+#include <algorithm>
+#include <cstring>
+#include <mutex>
+
+#include "device_common.hpp"
+
+namespace indigo {
+
+namespace {
+
+__global__ void zero_words_kernel(unsigned int* p, size_t words)
+{
+  for (size_t i = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x; i < words;
+       i += static_cast<size_t>(gridDim.x) * blockDim.x)
+    p[i] = 0u;
+}
+
+}  // namespace
+
+int ensure_workspace(indigo_ctx* ctx, size_t bytes)
+{
+  if (bytes <= ctx->ws_bytes && ctx->ws != nullptr) return INDIGO_OK;
+  if (ctx->ws != nullptr) {
+    IG_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    IG_HIP_TRY(hipFree(ctx->ws));
+    ctx->ws = nullptr;
+    ctx->ws_bytes = 0;
+  }
+  const size_t want = Carver::round(std::max<size_t>(bytes, 1 << 20));
+  if (hipMalloc(&ctx->ws, want) != hipSuccess) {
+    (void)hipGetLastError();
+    ctx->ws = nullptr;
+    return set_error(INDIGO_ERR_OUT_OF_MEMORY, "workspace allocation of %zu bytes failed", want);
+  }
+  ctx->ws_bytes = want;
+  return INDIGO_OK;
+}
+
+int ensure_host_pinned(indigo_ctx* ctx, size_t bytes)
+{
+  if (bytes <= ctx->host_pinned_bytes && ctx->host_pinned != nullptr) return INDIGO_OK;
+  if (ctx->host_pinned != nullptr) {
+    IG_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    IG_HIP_TRY(hipHostFree(ctx->host_pinned));
+    ctx->host_pinned = nullptr;
+    ctx->host_pinned_bytes = 0;
+  }
+  const size_t want = std::max<size_t>(bytes, 4096);
+  if (hipHostMalloc(&ctx->host_pinned, want, hipHostMallocDefault) != hipSuccess) {
+    (void)hipGetLastError();
+    ctx->host_pinned = nullptr;
+    return set_error(INDIGO_ERR_OUT_OF_MEMORY, "pinned host allocation of %zu bytes failed", want);
+  }
+  ctx->host_pinned_bytes = want;
+  return INDIGO_OK;
+}
+
+int coop_launch(indigo_ctx* ctx, const void* kernel, int block, size_t shmem, void** args, int max_blocks,
+                int* grid_out)
+{
+  int per_cu = 0;
+  IG_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, block, shmem));
+  if (per_cu < 1) return set_error(INDIGO_ERR_DEVICE, "kernel cannot be resident with block %d", block);
+  int grid = per_cu * ctx->compute_units;
+  if (max_blocks > 0) grid = std::min(grid, max_blocks);
+  grid = std::max(grid, 1);
+  IG_HIP_TRY(hipLaunchCooperativeKernel(kernel, dim3(grid), dim3(block), args, static_cast<unsigned>(shmem),
+                                        ctx->stream));
+  if (grid_out) *grid_out = grid;
+  return INDIGO_OK;
+}
+
+int zero_async(indigo_ctx* ctx, void* p, size_t bytes)
+{
+  const size_t words = (bytes + 3) / 4;
+  if (words == 0) return INDIGO_OK;
+  const int block = 256;
+  const int grid = static_cast<int>(std::min<size_t>((words + block - 1) / block, 4096));
+  zero_words_kernel<<<grid, block, 0, ctx->stream>>>(static_cast<unsigned int*>(p), words);
+  IG_HIP_TRY(hipGetLastError());
+  return INDIGO_OK;
+}
+
+int check_graph_args(int nodes, int edges, const void* nindex, const void* nlist)
+{
+  if (nodes < 1) return set_error(INDIGO_ERR_INVALID_ARG, "nodes must be >= 1 (got %d)", nodes);
+  if (edges < 0) return set_error(INDIGO_ERR_INVALID_ARG, "edges must be >= 0 (got %d)", edges);
+  if (nindex == nullptr) return set_error(INDIGO_ERR_INVALID_ARG, "nindex is NULL");
+  if (nlist == nullptr && edges > 0) return set_error(INDIGO_ERR_INVALID_ARG, "nlist is NULL");
+  return INDIGO_OK;
+}
+
+int timing_begin(indigo_ctx* ctx)
+{
+  IG_HIP_TRY(hipEventRecord(ctx->ev_start, ctx->stream));
+  return INDIGO_OK;
+}
+
+int timing_end(indigo_ctx* ctx, int launches, int iterations)
+{
+  IG_HIP_TRY(hipEventRecord(ctx->ev_stop, ctx->stream));
+  IG_HIP_TRY(hipEventSynchronize(ctx->ev_stop));
+  float ms = 0.f;
+  IG_HIP_TRY(hipEventElapsedTime(&ms, ctx->ev_start, ctx->ev_stop));
+  ctx->last_ms = ms;
+  ctx->last_launches = launches;
+  ctx->last_iterations = iterations;
+  return INDIGO_OK;
+}
+
+static int ctx_init(int device, indigo_ctx* c)
+{
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || count < 1) {
+    (void)hipGetLastError();
+    return set_error(INDIGO_ERR_NO_DEVICE, "no HIP device available");
+  }
+  if (device < 0 || device >= count) return set_error(INDIGO_ERR_INVALID_ARG, "device %d out of range", device);
+  IG_HIP_TRY(hipSetDevice(device));
+  c->device = device;
+  hipDeviceProp_t prop;
+  IG_HIP_TRY(hipGetDeviceProperties(&prop, device));
+  c->compute_units = prop.multiProcessorCount;
+  c->max_threads_per_cu = prop.maxThreadsPerMultiProcessor;
+  int coop = 0;
+  IG_HIP_TRY(hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, device));
+  if (!coop) return set_error(INDIGO_ERR_NO_DEVICE, "device %d lacks cooperative launch", device);
+  int wall_khz = 0;
+  if (hipDeviceGetAttribute(&wall_khz, hipDeviceAttributeWallClockRate, device) == hipSuccess && wall_khz > 0)
+    c->wall_hz = static_cast<long long>(wall_khz) * 1000;
+  IG_HIP_TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  IG_HIP_TRY(hipEventCreate(&c->ev_start));
+  IG_HIP_TRY(hipEventCreate(&c->ev_stop));
+  return INDIGO_OK;
+}
+
+indigo_ctx* default_ctx(int* rc)
+{
+  static std::mutex mu;
+  static indigo_ctx* ctx = nullptr;
+  std::lock_guard<std::mutex> lock(mu);
+  if (ctx == nullptr) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) {
+      (void)hipGetLastError();
+      dev = 0;
+    }
+    indigo_ctx* c = new indigo_ctx();
+    const int r = ctx_init(dev, c);
+    if (r != INDIGO_OK) {
+      delete c;
+      *rc = r;
+      return nullptr;
+    }
+    ctx = c;
+  }
+  *rc = INDIGO_OK;
+  return ctx;
+}
+
+}  // namespace indigo
+
+using namespace indigo;
+
+extern "C" {
+
+const char* indigo_last_error(void) { return error_buffer(); }
+
+int indigo_version(void) { return 100; }
+
+int indigo_device_info(int device, int* compute_units, int* max_threads_per_cu, char* name, int name_len)
+{
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || count < 1) {
+    (void)hipGetLastError();
+    return set_error(INDIGO_ERR_NO_DEVICE, "no HIP device available");
+  }
+  if (device < 0 || device >= count) return set_error(INDIGO_ERR_INVALID_ARG, "device %d out of range", device);
+  hipDeviceProp_t prop;
+  IG_HIP_TRY(hipGetDeviceProperties(&prop, device));
+  if (compute_units) *compute_units = prop.multiProcessorCount;
+  if (max_threads_per_cu) *max_threads_per_cu = prop.maxThreadsPerMultiProcessor;
+  if (name && name_len > 0) {
+    strncpy(name, prop.name, static_cast<size_t>(name_len) - 1);
+    name[name_len - 1] = '\0';
+  }
+  return INDIGO_OK;
+}
+
+int indigo_malloc(void** d_ptr, size_t bytes)
+{
+  if (d_ptr == nullptr) return set_error(INDIGO_ERR_INVALID_ARG, "indigo_malloc: null out pointer");
+  if (hipMalloc(d_ptr, std::max<size_t>(bytes, 4)) != hipSuccess) {
+    (void)hipGetLastError();
+    *d_ptr = nullptr;
+    return set_error(INDIGO_ERR_OUT_OF_MEMORY, "hipMalloc of %zu bytes failed", bytes);
+  }
+  return INDIGO_OK;
+}
+
+int indigo_free(void* d_ptr)
+{
+  if (d_ptr == nullptr) return INDIGO_OK;
+  IG_HIP_TRY(hipFree(d_ptr));
+  return INDIGO_OK;
+}
+
+int indigo_memcpy_h2d(void* d_dst, const void* h_src, size_t bytes)
+{
+  if (bytes == 0) return INDIGO_OK;
+  if (!d_dst || !h_src) return set_error(INDIGO_ERR_INVALID_ARG, "memcpy_h2d: null pointer");
+  IG_HIP_TRY(hipMemcpy(d_dst, h_src, bytes, hipMemcpyHostToDevice));
+  return INDIGO_OK;
+}
+
+int indigo_memcpy_d2h(void* h_dst, const void* d_src, size_t bytes)
+{
+  if (bytes == 0) return INDIGO_OK;
+  if (!h_dst || !d_src) return set_error(INDIGO_ERR_INVALID_ARG, "memcpy_d2h: null pointer");
+  IG_HIP_TRY(hipMemcpy(h_dst, d_src, bytes, hipMemcpyDeviceToHost));
+  return INDIGO_OK;
+}
+
+int indigo_ctx_create(int device, indigo_ctx** out)
+{
+  if (out == nullptr) return set_error(INDIGO_ERR_INVALID_ARG, "ctx_create: null out pointer");
+  *out = nullptr;
+  indigo_ctx* c = new indigo_ctx();
+  const int rc = ctx_init(device, c);
+  if (rc != INDIGO_OK) {
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return rc;
+  }
+  *out = c;
+  return INDIGO_OK;
+}
+
+int indigo_ctx_destroy(indigo_ctx* ctx)
+{
+  if (ctx == nullptr) return INDIGO_OK;
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
+  if (ctx->ws) (void)hipFree(ctx->ws);
+  if (ctx->host_pinned) (void)hipHostFree(ctx->host_pinned);
+  if (ctx->ev_start) (void)hipEventDestroy(ctx->ev_start);
+  if (ctx->ev_stop) (void)hipEventDestroy(ctx->ev_stop);
+  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+  return INDIGO_OK;
+}
+
+int indigo_ctx_last_ms(indigo_ctx* ctx, float* ms)
+{
+  if (ctx == nullptr || ms == nullptr) return set_error(INDIGO_ERR_INVALID_ARG, "ctx_last_ms: null argument");
+  *ms = ctx->last_ms;
+  return INDIGO_OK;
+}
+
+int indigo_ctx_last_stats(indigo_ctx* ctx, int* launches, int* iterations)
+{
+  if (ctx == nullptr) return set_error(INDIGO_ERR_INVALID_ARG, "ctx_last_stats: null ctx");
+  if (launches) *launches = ctx->last_launches;
+  if (iterations) *iterations = ctx->last_iterations;
+  return INDIGO_OK;
+}
+
+int indigo_ctx_trim(indigo_ctx* ctx)
+{
+  if (ctx == nullptr) return set_error(INDIGO_ERR_INVALID_ARG, "ctx_trim: null ctx");
+  IG_HIP_TRY(hipStreamSynchronize(ctx->stream));
+  if (ctx->ws) IG_HIP_TRY(hipFree(ctx->ws));
+  ctx->ws = nullptr;
+  ctx->ws_bytes = 0;
+  return INDIGO_OK;
+}
+
+int indigo_ctx_set_hint(indigo_ctx* ctx, int key, int value)
+{
+  if (ctx == nullptr || key < 0 || key >= 16) return set_error(INDIGO_ERR_INVALID_ARG, "ctx_set_hint: bad key");
+  ctx->hints[key] = value;
+  return INDIGO_OK;
+}
+
+}  // extern "C"

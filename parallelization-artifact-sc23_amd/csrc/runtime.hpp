@@ -1,0 +1,91 @@
+// Host-side runtime shared by the kernels' launchers: the context object
+// behind indigo_ctx, workspace carving, HIP error mapping and cooperative
+// (persistent) launches.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+
+#include "errors.hpp"
+#include "indigo_hip.h"
+
+struct indigo_ctx {
+  int device = 0;
+  int compute_units = 0;
+  int max_threads_per_cu = 0;
+  long long wall_hz = 100000000;  // wall_clock64() frequency
+  hipStream_t stream = nullptr;
+  hipEvent_t ev_start = nullptr, ev_stop = nullptr;
+  void* ws = nullptr;      // cached device workspace
+  size_t ws_bytes = 0;
+  void* host_pinned = nullptr;  // small pinned staging buffer (readbacks)
+  size_t host_pinned_bytes = 0;
+  float last_ms = 0.f;
+  int last_launches = 0;
+  int last_iterations = 0;
+  int hints[16] = {0};
+  double budget_seconds = 60.0;  // bound on any device-side wait
+};
+
+namespace indigo {
+
+#define IG_HIP_TRY(expr)                                                                       \
+  do {                                                                                         \
+    hipError_t ig_e_ = (expr);                                                                 \
+    if (ig_e_ != hipSuccess)                                                                   \
+      return ::indigo::set_error(INDIGO_ERR_DEVICE, "%s failed: %s (%s:%d)", #expr,           \
+                                 hipGetErrorString(ig_e_), __FILE__, __LINE__);                \
+  } while (0)
+
+#define IG_TRY(expr)                    \
+  do {                                  \
+    int ig_rc_ = (expr);                \
+    if (ig_rc_ != INDIGO_OK) return ig_rc_; \
+  } while (0)
+
+// Grows the context workspace to at least `bytes` (contents not preserved).
+int ensure_workspace(indigo_ctx* ctx, size_t bytes);
+int ensure_host_pinned(indigo_ctx* ctx, size_t bytes);
+
+// Sequentially carves 256-byte-aligned sub-buffers out of a workspace.
+struct Carver {
+  char* base;
+  size_t off = 0;
+  explicit Carver(void* p) : base(static_cast<char*>(p)) {}
+  template <typename T>
+  T* take(size_t count)
+  {
+    off = (off + 255) & ~static_cast<size_t>(255);
+    T* p = reinterpret_cast<T*>(base + off);
+    off += sizeof(T) * (count > 0 ? count : 1);
+    return p;
+  }
+  static size_t round(size_t bytes) { return (bytes + 255) & ~static_cast<size_t>(255); }
+};
+
+// Ticks of wall_clock64() that a persistent kernel may spend in bounded waits.
+inline long long budget_ticks(const indigo_ctx* ctx) { return static_cast<long long>(ctx->budget_seconds * ctx->wall_hz); }
+
+// Launches `kernel` cooperatively with as many workgroups of `block` threads
+// as can be co-resident (occupancy x compute units, capped at max_blocks).
+int coop_launch(indigo_ctx* ctx, const void* kernel, int block, size_t shmem, void** args, int max_blocks,
+                int* grid_out);
+
+// Zeroes `bytes` of device memory with a kernel (no memset node, so the
+// sequence stays safe to capture into a HIP graph).
+int zero_async(indigo_ctx* ctx, void* p, size_t bytes);
+
+// Argument checks shared by the entry points.
+int check_graph_args(int nodes, int edges, const void* nindex, const void* nlist);
+
+// Starts / stops the HIP-event timing of a resident call.
+int timing_begin(indigo_ctx* ctx);
+int timing_end(indigo_ctx* ctx, int launches, int iterations);
+
+// Process-wide default context for the drop-in entries (created lazily on
+// device 0, or on the current device).
+indigo_ctx* default_ctx(int* rc);
+
+}  // namespace indigo

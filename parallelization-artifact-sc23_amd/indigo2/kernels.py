@@ -1,0 +1,214 @@
+"""Python mirror of the six graph kernels over the C ABI.
+
+Two forms, matching include/indigo_hip.h:
+
+* ``Context``: the resident layer.  Graphs are DeviceGraph objects, outputs
+  stay in device memory unless downloaded, and ``last_ms`` is the HIP-event
+  time of the last call on the context's stream.
+* ``GPUbfs_vertex`` ... ``PR_GPU``: the reference's entry-function names and
+  argument meaning (include/*_cuda.h), filling host numpy outputs and
+  returning the elapsed seconds, for harness-style use in tests.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from .graph import DeviceBuffer, DeviceGraph, HostGraph
+
+INF = np.iinfo(np.int32).max
+UNDECIDED, INCLUDED, EXCLUDED = 0, 1, 2  # mis_vertex_cuda.h:50-52
+
+
+def _pi(a: np.ndarray):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+
+
+class Context:
+    def __init__(self, device: int = 0):
+        h = ctypes.c_void_p()
+        _lib.call("indigo_ctx_create", device, ctypes.byref(h))
+        self.h = h.value
+
+    def close(self) -> None:
+        if self.h:
+            _lib.load().indigo_ctx_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_hint(self, key: int, value: int) -> None:
+        _lib.call("indigo_ctx_set_hint", self.h, key, value)
+
+    @property
+    def last_ms(self) -> float:
+        ms = ctypes.c_float()
+        _lib.call("indigo_ctx_last_ms", self.h, ctypes.byref(ms))
+        return ms.value
+
+    @property
+    def last_stats(self) -> tuple[int, int]:
+        launches, iters = ctypes.c_int(), ctypes.c_int()
+        _lib.call("indigo_ctx_last_stats", self.h, ctypes.byref(launches), ctypes.byref(iters))
+        return launches.value, iters.value
+
+    def trim(self) -> None:
+        _lib.call("indigo_ctx_trim", self.h)
+
+    # ---- device-output forms ------------------------------------------------
+    def bfs_into(self, g: DeviceGraph, src: int, out: DeviceBuffer) -> None:
+        _lib.call("indigo_bfs", self.h, g.nodes, g.edges, g.d_nindex.ptr, g.d_nlist.ptr, src, out.ptr)
+
+    def sssp_into(self, g: DeviceGraph, src: int, out: DeviceBuffer) -> None:
+        if g.d_eweight is None:
+            raise ValueError("sssp needs edge weights")
+        _lib.call("indigo_sssp", self.h, g.nodes, g.edges, g.d_nindex.ptr, g.d_nlist.ptr, g.d_eweight.ptr, src,
+                  out.ptr)
+
+    def cc_into(self, g: DeviceGraph, out: DeviceBuffer) -> None:
+        _lib.call("indigo_cc", self.h, g.nodes, g.edges, g.d_nindex.ptr, g.d_nlist.ptr, out.ptr)
+
+    def mis_into(self, g: DeviceGraph, out: DeviceBuffer, signed_priority: bool = False) -> None:
+        _lib.call("indigo_mis", self.h, g.nodes, g.edges, g.d_nindex.ptr, g.d_nlist.ptr, int(signed_priority),
+                  out.ptr)
+
+    def pagerank_into(self, g: DeviceGraph, out: DeviceBuffer) -> int:
+        it = ctypes.c_int()
+        _lib.call("indigo_pagerank", self.h, g.nodes, g.edges, g.d_nindex.ptr, g.d_nlist.ptr, out.ptr,
+                  ctypes.byref(it))
+        return it.value
+
+    def tc(self, g: DeviceGraph) -> int:
+        c = ctypes.c_longlong()
+        _lib.call("indigo_tc", self.h, g.nodes, g.edges, g.d_nindex.ptr, g.d_nlist.ptr, ctypes.byref(c))
+        return int(c.value)
+
+    # ---- host-output conveniences -------------------------------------------
+    def bfs(self, g: DeviceGraph, src: int) -> np.ndarray:
+        out = DeviceBuffer(4 * g.nodes)
+        self.bfs_into(g, src, out)
+        return out.to_array(np.int32, g.nodes)
+
+    def sssp(self, g: DeviceGraph, src: int) -> np.ndarray:
+        out = DeviceBuffer(4 * g.nodes)
+        self.sssp_into(g, src, out)
+        return out.to_array(np.int32, g.nodes)
+
+    def cc(self, g: DeviceGraph) -> np.ndarray:
+        out = DeviceBuffer(4 * g.nodes)
+        self.cc_into(g, out)
+        return out.to_array(np.int32, g.nodes)
+
+    def mis(self, g: DeviceGraph, signed_priority: bool = False) -> np.ndarray:
+        out = DeviceBuffer(4 * g.nodes)
+        self.mis_into(g, out, signed_priority)
+        return out.to_array(np.int32, g.nodes)
+
+    def pagerank(self, g: DeviceGraph) -> tuple[np.ndarray, int]:
+        out = DeviceBuffer(4 * g.nodes)
+        it = self.pagerank_into(g, out)
+        return out.to_array(np.float32, g.nodes), it
+
+
+# ---- the reference's entry functions (include/*_cuda.h) ----------------------
+def GPUbfs_vertex(src: int, g: DeviceGraph, dist: np.ndarray) -> float:
+    """include/bfs_vertex_cuda.h:61."""
+    sec = ctypes.c_double()
+    _lib.call("indigo_bfs_vertex", src, g.nodes, g.edges, g.d_nindex.ptr, g.d_nlist.ptr, _pi(dist),
+              ctypes.byref(sec))
+    return sec.value
+
+
+def GPUbfs_edge(src: int, g: DeviceGraph, dist: np.ndarray, sp: np.ndarray) -> float:
+    """include/bfs_edge_cuda.h:61 (sp on the host)."""
+    sec = ctypes.c_double()
+    _lib.call("indigo_bfs_edge", src, g.nodes, g.edges, g.d_nindex.ptr, g.d_nlist.ptr, _pi(sp), _pi(dist),
+              ctypes.byref(sec))
+    return sec.value
+
+
+def GPUsssp_vertex(src: int, g: DeviceGraph, dist: np.ndarray) -> float:
+    """include/sssp_vertex_cuda.h:60."""
+    sec = ctypes.c_double()
+    _lib.call("indigo_sssp_vertex", src, g.nodes, g.edges, g.d_nindex.ptr, g.d_nlist.ptr, g.d_eweight.ptr,
+              _pi(dist), ctypes.byref(sec))
+    return sec.value
+
+
+def GPUsssp_edge(src: int, g: DeviceGraph, dist: np.ndarray, sp: np.ndarray) -> float:
+    """include/sssp_edge_cuda.h:60 (sp on the host)."""
+    sec = ctypes.c_double()
+    _lib.call("indigo_sssp_edge", src, g.nodes, g.edges, g.d_nindex.ptr, g.d_nlist.ptr, g.d_eweight.ptr, _pi(sp),
+              _pi(dist), ctypes.byref(sec))
+    return sec.value
+
+
+def GPUcc_vertex(g: DeviceGraph, label: np.ndarray) -> float:
+    """include/cc_vertex_cuda.h:57."""
+    sec = ctypes.c_double()
+    _lib.call("indigo_cc_vertex", g.nodes, g.edges, g.d_nindex.ptr, g.d_nlist.ptr, _pi(label), ctypes.byref(sec))
+    return sec.value
+
+
+def GPUcc_edge(g: DeviceGraph, label: np.ndarray, sp: np.ndarray) -> float:
+    """include/cc_edge_cuda.h:57 (sp on the host)."""
+    sec = ctypes.c_double()
+    _lib.call("indigo_cc_edge", g.nodes, g.edges, g.d_nindex.ptr, g.d_nlist.ptr, _pi(sp), _pi(label),
+              ctypes.byref(sec))
+    return sec.value
+
+
+def GPUmis_vertex(g: DeviceGraph, priority: np.ndarray, status: np.ndarray, signed_priority: bool = False) -> float:
+    """include/mis_vertex_cuda.h:54."""
+    sec = ctypes.c_double()
+    _lib.call("indigo_mis_vertex", g.nodes, g.edges, g.d_nindex.ptr, g.d_nlist.ptr, int(signed_priority),
+              priority.ctypes.data_as(ctypes.POINTER(ctypes.c_uint)), _pi(status), ctypes.byref(sec))
+    return sec.value
+
+
+def GPUmis_edge(g: DeviceGraph, d_sp: DeviceBuffer, priority: np.ndarray, status: np.ndarray,
+                signed_priority: bool = False) -> float:
+    """include/mis_edge_cuda.h:54 (sp on the device)."""
+    sec = ctypes.c_double()
+    _lib.call("indigo_mis_edge", g.nodes, g.edges, g.d_nindex.ptr, g.d_nlist.ptr, d_sp.ptr, int(signed_priority),
+              priority.ctypes.data_as(ctypes.POINTER(ctypes.c_uint)), _pi(status), ctypes.byref(sec))
+    return sec.value
+
+
+def PR_GPU(g: HostGraph, scores: np.ndarray, degree: np.ndarray) -> tuple[float, int]:
+    """include/pr_cuda.h:48 (host graph; scores in/out).  Returns (seconds,
+    iterations)."""
+    sec = ctypes.c_double()
+    it = ctypes.c_int()
+    _lib.call("indigo_pr", g.nodes, g.edges, _pi(g.nindex), _pi(g.nlist), _pi(degree),
+              scores.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), ctypes.byref(it), ctypes.byref(sec))
+    return sec.value, it.value
+
+
+def GPUtc_vertex(g: DeviceGraph) -> tuple[int, float]:
+    """include/tc_vertex_cuda.h:42; returns (count, seconds)."""
+    sec = ctypes.c_double()
+    c = ctypes.c_longlong()
+    _lib.call("indigo_tc_vertex", ctypes.byref(c), g.nodes, g.d_nindex.ptr, g.d_nlist.ptr, ctypes.byref(sec))
+    return int(c.value), sec.value
+
+
+def GPUtc_edge(g: DeviceGraph, d_sp: DeviceBuffer) -> tuple[int, float]:
+    """include/tc_edge_cuda.h:42 (sp on the device); returns (count, seconds)."""
+    sec = ctypes.c_double()
+    c = ctypes.c_longlong()
+    _lib.call("indigo_tc_edge", ctypes.byref(c), g.edges, g.d_nindex.ptr, g.d_nlist.ptr, d_sp.ptr,
+              ctypes.byref(sec))
+    return int(c.value), sec.value
