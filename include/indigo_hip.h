@@ -135,6 +135,9 @@ int indigo_ctx_last_ms(indigo_ctx* ctx, float* ms);
 /* Number of kernel launches and iterations (levels, rounds, sweeps) of the
  * last resident call. */
 int indigo_ctx_last_stats(indigo_ctx* ctx, int* launches, int* iterations);
+/* Workgroups of the last cooperative (persistent) launch and the per-CU
+ * residency it was sized for. */
+int indigo_ctx_last_grid(indigo_ctx* ctx, int* grid, int* per_cu);
 /* Release the cached workspace (it regrows on the next call). */
 int indigo_ctx_trim(indigo_ctx* ctx);
 /* Optional tuning knob: block size / variant selection hints (0 = auto). */
@@ -142,6 +145,8 @@ int indigo_ctx_set_hint(indigo_ctx* ctx, int key, int value);
 #define INDIGO_HINT_BFS_DIRECTION 1 /* 0 auto, 1 top-down only, 2 allow bottom-up */
 #define INDIGO_HINT_SSSP_DELTA 2    /* 0 auto, else near-far split increment */
 #define INDIGO_HINT_SMALL_FRONTIER 3 /* 0 auto, 1 off: never use the one-workgroup mode */
+#define INDIGO_HINT_BUDGET_MS 4      /* bound on device-side waits in ms (0: 60000) */
+#define INDIGO_HINT_BLOCKS_PER_CU 5  /* cap on co-resident workgroups per CU of persistent kernels (0: occupancy) */
 
 /* All resident entries take the device CSR as (nodes, edges, d_nindex,
  * d_nlist[, d_eweight]) and return after the result is in device memory. */

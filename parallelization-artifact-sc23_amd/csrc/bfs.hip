@@ -59,6 +59,7 @@ struct BfsArgs {
   FrontierQueues fq[2];
   BfsControl* ctl;
   long long budget;
+  long long start_ticks;
 };
 
 IG_DEV int degree_of(const int* __restrict__ nindex, int u) { return nindex[u + 1] - nindex[u]; }
@@ -232,8 +233,9 @@ IG_DEV int small_phase(const BfsArgs& a, int L, unsigned n, const FrontierQueues
 
 __global__ __launch_bounds__(kBfsBlock) void bfs_kernel(BfsArgs a)
 {
-  const long long deadline = now_ticks() + a.budget;
   BfsControl* ctl = a.ctl;
+  if (!residency_barrier(&ctl->bar, a.start_ticks)) return;
+  const long long deadline = now_ticks() + a.budget;
   const int tid = blockIdx.x * blockDim.x + threadIdx.x;
   const int stride = gridDim.x * blockDim.x;
   for (int i = tid; i < a.nodes; i += stride) a.dist[i] = (i == a.src) ? 0 : kInf;
@@ -372,6 +374,7 @@ int bfs_run(indigo_ctx* ctx, int nodes, int edges, const int* nindex, const int*
   a.nlist = nlist;
   a.dist = dist;
   a.budget = budget_ticks(ctx);
+  a.start_ticks = residency_ticks(ctx);
   IG_TRY(zero_async(ctx, a.ctl, sizeof(BfsControl)));
   void* args[] = {&a};
   int grid = 0;
@@ -390,6 +393,7 @@ int bfs_collect(indigo_ctx* ctx, BfsStats* stats)
   BfsControl* h = static_cast<BfsControl*>(ctx->host_pinned);
   IG_HIP_TRY(hipMemcpyAsync(h, ctl, sizeof(BfsControl), hipMemcpyDeviceToHost, ctx->stream));
   IG_HIP_TRY(hipStreamSynchronize(ctx->stream));
+  IG_TRY(check_abort(ctx, reinterpret_cast<const void*>(bfs_kernel), h->bar.abort.v, "bfs"));
   if (h->status != 0)
     return set_error(INDIGO_ERR_TIMEOUT, "bfs: persistent kernel stopped (status %d) before finishing", h->status);
   if (stats) {

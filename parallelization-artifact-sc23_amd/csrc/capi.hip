@@ -5,7 +5,13 @@
 #include <climits>
 
 #include "bfs.hpp"
+#include "cc.hpp"
+#include "device_common.hpp"
+#include "mis.hpp"
+#include "pr.hpp"
 #include "runtime.hpp"
+#include "sssp.hpp"
+#include "tc.hpp"
 
 using namespace indigo;
 
@@ -75,13 +81,121 @@ int bfs_device(indigo_ctx* ctx, int nodes, int edges, const int* nindex, const i
   IG_TRY(ensure_host_pinned(ctx, 4096));
   IG_HIP_TRY(hipStreamSynchronize(ctx->stream));
   const double t0 = wall_seconds();
-  IG_TRY(timing_begin(ctx));
   BfsStats st;
-  IG_TRY(bfs_run(ctx, nodes, edges, nindex, nlist, src, dist, &st));
-  IG_TRY(timing_end(ctx, st.launches, 0));
-  IG_TRY(bfs_collect(ctx, &st));
+  for (int attempt = 0;; attempt++) {
+    IG_TRY(timing_begin(ctx));
+    IG_TRY(bfs_run(ctx, nodes, edges, nindex, nlist, src, dist, &st));
+    IG_TRY(timing_end(ctx, st.launches, 0));
+    const int rc = bfs_collect(ctx, &st);
+    if (rc != kNotResident) IG_TRY(rc);
+    if (rc == INDIGO_OK) break;
+  }
   if (seconds) *seconds = wall_seconds() - t0;
   ctx->last_iterations = st.levels;
+  return INDIGO_OK;
+}
+
+
+// ---- SSSP / CC / MIS / PR / TC on device buffers ------------------------------
+int sssp_device(indigo_ctx* ctx, int nodes, int edges, const int* nindex, const int* nlist, const int* eweight,
+                int src, int* dist, double* seconds)
+{
+  IG_TRY(ensure_workspace(ctx, sssp_workspace_bytes(nodes)));
+  IG_TRY(ensure_host_pinned(ctx, 4096));
+  IG_HIP_TRY(hipStreamSynchronize(ctx->stream));
+  const double t0 = wall_seconds();
+  int iters = 0, epochs = 0;
+  for (;;) {
+    IG_TRY(timing_begin(ctx));
+    IG_TRY(sssp_run(ctx, nodes, edges, nindex, nlist, eweight, src, dist));
+    IG_TRY(timing_end(ctx, 2, 0));
+    const int rc = sssp_collect(ctx, &iters, &epochs);
+    if (rc != kNotResident) IG_TRY(rc);
+    if (rc == INDIGO_OK) break;
+  }
+  if (seconds) *seconds = wall_seconds() - t0;
+  ctx->last_iterations = iters;
+  return INDIGO_OK;
+}
+
+int cc_device(indigo_ctx* ctx, int nodes, int edges, const int* nindex, const int* nlist, int* label,
+              double* seconds)
+{
+  IG_TRY(ensure_workspace(ctx, cc_workspace_bytes(nodes)));
+  IG_HIP_TRY(hipStreamSynchronize(ctx->stream));
+  const double t0 = wall_seconds();
+  IG_TRY(timing_begin(ctx));
+  int launches = 0;
+  IG_TRY(cc_run(ctx, nodes, edges, nindex, nlist, label, &launches));
+  IG_TRY(timing_end(ctx, launches, 1));
+  if (seconds) *seconds = wall_seconds() - t0;
+  return INDIGO_OK;
+}
+
+int mis_device(indigo_ctx* ctx, int nodes, const int* nindex, const int* nlist, int signed_priority, int* status,
+               double* seconds)
+{
+  IG_TRY(ensure_workspace(ctx, mis_workspace_bytes(nodes)));
+  IG_TRY(ensure_host_pinned(ctx, 4096));
+  IG_HIP_TRY(hipStreamSynchronize(ctx->stream));
+  const double t0 = wall_seconds();
+  int rounds = 0;
+  for (;;) {
+    IG_TRY(timing_begin(ctx));
+    IG_TRY(mis_run(ctx, nodes, nindex, nlist, signed_priority, status));
+    IG_TRY(timing_end(ctx, 2, 0));
+    const int rc = mis_collect(ctx, &rounds);
+    if (rc != kNotResident) IG_TRY(rc);
+    if (rc == INDIGO_OK) break;
+  }
+  if (seconds) *seconds = wall_seconds() - t0;
+  ctx->last_iterations = rounds;
+  return INDIGO_OK;
+}
+
+int pr_device(indigo_ctx* ctx, int nodes, int edges, const int* nindex, const int* nlist, const int* degree, float* scores,
+              int init_uniform, int* iterations, double* seconds)
+{
+  IG_TRY(ensure_workspace(ctx, pr_workspace_bytes(nodes, edges)));
+  IG_TRY(ensure_host_pinned(ctx, 4096));
+  IG_HIP_TRY(hipStreamSynchronize(ctx->stream));
+  const double t0 = wall_seconds();
+  int iters = 0;
+  for (;;) {
+    IG_TRY(timing_begin(ctx));
+    IG_TRY(pr_run(ctx, nodes, edges, nindex, nlist, degree, scores, init_uniform));
+    IG_TRY(timing_end(ctx, 2, 0));
+    const int rc = pr_collect(ctx, &iters, nullptr);
+    if (rc != kNotResident) IG_TRY(rc);
+    if (rc == INDIGO_OK) break;
+  }
+  if (seconds) *seconds = wall_seconds() - t0;
+  ctx->last_iterations = iters;
+  if (iterations) *iterations = iters;
+  return INDIGO_OK;
+}
+
+int tc_device(indigo_ctx* ctx, int nodes, int edges, const int* nindex, const int* nlist, long long* count,
+              double* seconds)
+{
+  IG_TRY(ensure_workspace(ctx, tc_workspace_bytes(nodes, edges)));
+  IG_TRY(ensure_host_pinned(ctx, 4096));
+  IG_HIP_TRY(hipStreamSynchronize(ctx->stream));
+  const double t0 = wall_seconds();
+  IG_TRY(timing_begin(ctx));
+  int launches = 0;
+  IG_TRY(tc_run(ctx, nodes, edges, nindex, nlist, &launches));
+  IG_TRY(timing_end(ctx, launches, 1));
+  IG_TRY(tc_collect(ctx, count));
+  if (seconds) *seconds = wall_seconds() - t0;
+  return INDIGO_OK;
+}
+
+// Reads one int from device memory (used to derive sizes the reference's
+// entry signatures do not pass).
+int read_device_int(const int* d, int* out)
+{
+  IG_HIP_TRY(hipMemcpy(out, d, sizeof(int), hipMemcpyDeviceToHost));
   return INDIGO_OK;
 }
 
@@ -119,6 +233,173 @@ int indigo_bfs_edge(int src, int nodes, int edges, const int* d_nindex, const in
 {
   if (h_sp == nullptr && edges > 0) return set_error(INDIGO_ERR_INVALID_ARG, "bfs_edge: sp is NULL");
   return indigo_bfs_vertex(src, nodes, edges, d_nindex, d_nlist, h_dist, seconds);
+}
+
+int indigo_sssp(indigo_ctx* ctx, int nodes, int edges, const int* d_nindex, const int* d_nlist,
+                const int* d_eweight, int src, int* d_dist)
+{
+  IG_TRY(ctx_prologue(ctx));
+  IG_TRY(check_graph_args(nodes, edges, d_nindex, d_nlist));
+  IG_TRY(check_src(src, nodes));
+  if (d_dist == nullptr) return set_error(INDIGO_ERR_INVALID_ARG, "sssp: dist is NULL");
+  if (d_eweight == nullptr && edges > 0) return set_error(INDIGO_ERR_INVALID_ARG, "sssp: eweight is NULL");
+  return sssp_device(ctx, nodes, edges, d_nindex, d_nlist, d_eweight, src, d_dist, nullptr);
+}
+
+int indigo_sssp_vertex(int src, int nodes, int edges, const int* d_nindex, const int* d_nlist,
+                       const int* d_eweight, int* h_dist, double* seconds)
+{
+  indigo_ctx* ctx = nullptr;
+  IG_TRY(legacy_ctx(&ctx));
+  IG_TRY(check_graph_args(nodes, edges, d_nindex, d_nlist));
+  IG_TRY(check_src(src, nodes));
+  if (h_dist == nullptr) return set_error(INDIGO_ERR_INVALID_ARG, "sssp_vertex: dist is NULL");
+  if (d_eweight == nullptr && edges > 0) return set_error(INDIGO_ERR_INVALID_ARG, "sssp_vertex: eweight is NULL");
+  DeviceBuffer d;
+  IG_TRY(d.alloc(sizeof(int) * static_cast<size_t>(nodes)));
+  IG_TRY(sssp_device(ctx, nodes, edges, d_nindex, d_nlist, d_eweight, src, d.as<int>(), seconds));
+  IG_HIP_TRY(hipMemcpy(h_dist, d.p, sizeof(int) * static_cast<size_t>(nodes), hipMemcpyDeviceToHost));
+  return INDIGO_OK;
+}
+
+int indigo_sssp_edge(int src, int nodes, int edges, const int* d_nindex, const int* d_nlist,
+                     const int* d_eweight, const int* h_sp, int* h_dist, double* seconds)
+{
+  if (h_sp == nullptr && edges > 0) return set_error(INDIGO_ERR_INVALID_ARG, "sssp_edge: sp is NULL");
+  return indigo_sssp_vertex(src, nodes, edges, d_nindex, d_nlist, d_eweight, h_dist, seconds);
+}
+
+int indigo_cc(indigo_ctx* ctx, int nodes, int edges, const int* d_nindex, const int* d_nlist, int* d_label)
+{
+  IG_TRY(ctx_prologue(ctx));
+  IG_TRY(check_graph_args(nodes, edges, d_nindex, d_nlist));
+  if (d_label == nullptr) return set_error(INDIGO_ERR_INVALID_ARG, "cc: label is NULL");
+  return cc_device(ctx, nodes, edges, d_nindex, d_nlist, d_label, nullptr);
+}
+
+int indigo_cc_vertex(int nodes, int edges, const int* d_nindex, const int* d_nlist, int* h_label, double* seconds)
+{
+  indigo_ctx* ctx = nullptr;
+  IG_TRY(legacy_ctx(&ctx));
+  IG_TRY(check_graph_args(nodes, edges, d_nindex, d_nlist));
+  if (h_label == nullptr) return set_error(INDIGO_ERR_INVALID_ARG, "cc_vertex: label is NULL");
+  DeviceBuffer d;
+  IG_TRY(d.alloc(sizeof(int) * static_cast<size_t>(nodes)));
+  IG_TRY(cc_device(ctx, nodes, edges, d_nindex, d_nlist, d.as<int>(), seconds));
+  IG_HIP_TRY(hipMemcpy(h_label, d.p, sizeof(int) * static_cast<size_t>(nodes), hipMemcpyDeviceToHost));
+  return INDIGO_OK;
+}
+
+int indigo_cc_edge(int nodes, int edges, const int* d_nindex, const int* d_nlist, const int* h_sp, int* h_label,
+                   double* seconds)
+{
+  if (h_sp == nullptr && edges > 0) return set_error(INDIGO_ERR_INVALID_ARG, "cc_edge: sp is NULL");
+  return indigo_cc_vertex(nodes, edges, d_nindex, d_nlist, h_label, seconds);
+}
+
+int indigo_mis(indigo_ctx* ctx, int nodes, int edges, const int* d_nindex, const int* d_nlist, int signed_priority,
+               int* d_status)
+{
+  IG_TRY(ctx_prologue(ctx));
+  IG_TRY(check_graph_args(nodes, edges, d_nindex, d_nlist));
+  if (d_status == nullptr) return set_error(INDIGO_ERR_INVALID_ARG, "mis: status is NULL");
+  return mis_device(ctx, nodes, d_nindex, d_nlist, signed_priority, d_status, nullptr);
+}
+
+int indigo_mis_vertex(int nodes, int edges, const int* d_nindex, const int* d_nlist, int signed_priority,
+                      unsigned* h_priority, int* h_status, double* seconds)
+{
+  indigo_ctx* ctx = nullptr;
+  IG_TRY(legacy_ctx(&ctx));
+  IG_TRY(check_graph_args(nodes, edges, d_nindex, d_nlist));
+  if (h_status == nullptr) return set_error(INDIGO_ERR_INVALID_ARG, "mis_vertex: status is NULL");
+  DeviceBuffer d;
+  IG_TRY(d.alloc(sizeof(int) * static_cast<size_t>(nodes)));
+  IG_TRY(mis_device(ctx, nodes, d_nindex, d_nlist, signed_priority, d.as<int>(), seconds));
+  IG_HIP_TRY(hipMemcpy(h_status, d.p, sizeof(int) * static_cast<size_t>(nodes), hipMemcpyDeviceToHost));
+  if (h_priority != nullptr)
+    for (int v = 0; v < nodes; v++) h_priority[v] = mis_hash(static_cast<unsigned>(v) + 712313887u);
+  return INDIGO_OK;
+}
+
+int indigo_mis_edge(int nodes, int edges, const int* d_nindex, const int* d_nlist, const int* d_sp,
+                    int signed_priority, unsigned* h_priority, int* h_status, double* seconds)
+{
+  if (d_sp == nullptr && edges > 0) return set_error(INDIGO_ERR_INVALID_ARG, "mis_edge: sp is NULL");
+  return indigo_mis_vertex(nodes, edges, d_nindex, d_nlist, signed_priority, h_priority, h_status, seconds);
+}
+
+int indigo_pagerank(indigo_ctx* ctx, int nodes, int edges, const int* d_nindex, const int* d_nlist, float* d_scores,
+                    int* iterations)
+{
+  IG_TRY(ctx_prologue(ctx));
+  IG_TRY(check_graph_args(nodes, edges, d_nindex, d_nlist));
+  if (d_scores == nullptr) return set_error(INDIGO_ERR_INVALID_ARG, "pagerank: scores is NULL");
+  return pr_device(ctx, nodes, edges, d_nindex, d_nlist, nullptr, d_scores, 1, iterations, nullptr);
+}
+
+int indigo_pr(int nodes, int edges, const int* h_nindex, const int* h_nlist, const int* h_degree, float* h_scores,
+              int* iterations, double* seconds)
+{
+  indigo_ctx* ctx = nullptr;
+  IG_TRY(legacy_ctx(&ctx));
+  IG_TRY(check_graph_args(nodes, edges, h_nindex, h_nlist));
+  if (h_scores == nullptr || h_degree == nullptr)
+    return set_error(INDIGO_ERR_INVALID_ARG, "pr: scores or degree is NULL");
+  // PR_GPU copies the host graph itself (include/pr_cuda.h:48, variant files)
+  DeviceBuffer dn, dl, dd, ds;
+  IG_TRY(dn.alloc(sizeof(int) * (static_cast<size_t>(nodes) + 1)));
+  IG_TRY(dl.alloc(sizeof(int) * static_cast<size_t>(edges)));
+  IG_TRY(dd.alloc(sizeof(int) * static_cast<size_t>(nodes)));
+  IG_TRY(ds.alloc(sizeof(float) * static_cast<size_t>(nodes)));
+  IG_HIP_TRY(hipMemcpy(dn.p, h_nindex, sizeof(int) * (static_cast<size_t>(nodes) + 1), hipMemcpyHostToDevice));
+  if (edges > 0) IG_HIP_TRY(hipMemcpy(dl.p, h_nlist, sizeof(int) * static_cast<size_t>(edges), hipMemcpyHostToDevice));
+  IG_HIP_TRY(hipMemcpy(dd.p, h_degree, sizeof(int) * static_cast<size_t>(nodes), hipMemcpyHostToDevice));
+  IG_HIP_TRY(hipMemcpy(ds.p, h_scores, sizeof(float) * static_cast<size_t>(nodes), hipMemcpyHostToDevice));
+  IG_TRY(pr_device(ctx, nodes, edges, dn.as<int>(), dl.as<int>(), dd.as<int>(), ds.as<float>(), 0, iterations, seconds));
+  IG_HIP_TRY(hipMemcpy(h_scores, ds.p, sizeof(float) * static_cast<size_t>(nodes), hipMemcpyDeviceToHost));
+  return INDIGO_OK;
+}
+
+int indigo_tc(indigo_ctx* ctx, int nodes, int edges, const int* d_nindex, const int* d_nlist, long long* count)
+{
+  IG_TRY(ctx_prologue(ctx));
+  IG_TRY(check_graph_args(nodes, edges, d_nindex, d_nlist));
+  if (count == nullptr) return set_error(INDIGO_ERR_INVALID_ARG, "tc: count is NULL");
+  return tc_device(ctx, nodes, edges, d_nindex, d_nlist, count, nullptr);
+}
+
+int indigo_tc_vertex(long long* count, int nodes, const int* d_nindex, const int* d_nlist, double* seconds)
+{
+  indigo_ctx* ctx = nullptr;
+  IG_TRY(legacy_ctx(&ctx));
+  if (count == nullptr) return set_error(INDIGO_ERR_INVALID_ARG, "tc_vertex: count is NULL");
+  if (nodes < 1 || d_nindex == nullptr) return set_error(INDIGO_ERR_INVALID_ARG, "tc_vertex: bad graph");
+  int edges = 0;
+  IG_TRY(read_device_int(d_nindex + nodes, &edges));
+  IG_TRY(check_graph_args(nodes, edges, d_nindex, d_nlist));
+  return tc_device(ctx, nodes, edges, d_nindex, d_nlist, count, seconds);
+}
+
+int indigo_tc_edge(long long* count, int edges, const int* d_nindex, const int* d_nlist, const int* d_sp,
+                   double* seconds)
+{
+  indigo_ctx* ctx = nullptr;
+  IG_TRY(legacy_ctx(&ctx));
+  if (count == nullptr) return set_error(INDIGO_ERR_INVALID_ARG, "tc_edge: count is NULL");
+  if (edges < 0 || d_nindex == nullptr || (edges > 0 && (d_nlist == nullptr || d_sp == nullptr)))
+    return set_error(INDIGO_ERR_INVALID_ARG, "tc_edge: bad graph");
+  if (edges == 0) {
+    *count = 0;
+    if (seconds) *seconds = 0.0;
+    return INDIGO_OK;
+  }
+  // sp is sorted (tc_edge_cuda.h:159-165): the last entry's source + 1 bounds
+  // every vertex with a list; trailing isolated vertices hold no triangles
+  int last = 0;
+  IG_TRY(read_device_int(d_sp + edges - 1, &last));
+  const int nodes = last + 1;
+  return tc_device(ctx, nodes, edges, d_nindex, d_nlist, count, seconds);
 }
 
 }  // extern "C"

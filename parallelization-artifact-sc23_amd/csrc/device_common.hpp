@@ -148,7 +148,11 @@ struct GridBarrier {
 
 IG_DEV long long now_ticks() { return wall_clock64(); }
 
-IG_DEV bool grid_barrier(GridBarrier* gb, long long deadline)
+// abort codes stored in GridBarrier::abort
+constexpr unsigned kAbortTimeout = 1u;      // a bounded wait expired mid-run
+constexpr unsigned kAbortNotResident = 2u;  // the grid never assembled: not co-resident
+
+IG_DEV bool grid_barrier(GridBarrier* gb, long long deadline, unsigned abort_code = kAbortTimeout)
 {
   __shared__ int s_ok;
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
@@ -178,7 +182,9 @@ IG_DEV bool grid_barrier(GridBarrier* gb, long long deadline)
           break;
         }
         if (now_ticks() > deadline) {
-          __hip_atomic_store(&gb->abort.v, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          unsigned expected = 0u;
+          __hip_atomic_compare_exchange_strong(&gb->abort.v, &expected, abort_code, __ATOMIC_RELAXED,
+                                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
           ok = 0;
           break;
         }
@@ -191,6 +197,15 @@ IG_DEV bool grid_barrier(GridBarrier* gb, long long deadline)
   __syncthreads();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
   return s_ok != 0;
+}
+
+// First action of every persistent kernel: all workgroups must arrive
+// within `ticks` (they are launched together; the only way to miss is a
+// grid that cannot be co-resident).  On failure the host relaunches with
+// fewer workgroups per CU; nothing has been written yet.
+IG_DEV bool residency_barrier(GridBarrier* gb, long long ticks)
+{
+  return grid_barrier(gb, now_ticks() + ticks, kAbortNotResident);
 }
 
 // Same as grid_barrier's abort test, for loops that run between barriers

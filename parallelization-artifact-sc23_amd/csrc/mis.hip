@@ -51,6 +51,7 @@ struct MisArgs {
   int* wl[2];
   MisControl* ctl;
   long long budget;
+  long long start_ticks;
 };
 
 IG_DEV unsigned long long mis_key(int v, unsigned flip)
@@ -113,8 +114,9 @@ IG_DEV void exclude_neighbours_wave(const MisArgs& a, int v, int beg, int end)
 
 __global__ __launch_bounds__(kMisBlock) void mis_kernel(MisArgs a)
 {
-  const long long deadline = now_ticks() + a.budget;
   MisControl* ctl = a.ctl;
+  if (!residency_barrier(&ctl->bar, a.start_ticks)) return;
+  const long long deadline = now_ticks() + a.budget;
   const int tid = blockIdx.x * blockDim.x + threadIdx.x;
   const int lane = lane_id();
   for (int i = tid; i < a.nodes; i += gridDim.x * blockDim.x) a.status[i] = kUndecided;
@@ -202,6 +204,7 @@ int mis_run(indigo_ctx* ctx, int nodes, const int* nindex, const int* nlist, int
   a.nlist = nlist;
   a.status = status;
   a.budget = budget_ticks(ctx);
+  a.start_ticks = residency_ticks(ctx);
   IG_TRY(zero_async(ctx, a.ctl, sizeof(MisControl)));
   void* args[] = {&a};
   IG_TRY(coop_launch(ctx, reinterpret_cast<const void*>(mis_kernel), kMisBlock, 0, args, 0, nullptr));
@@ -216,6 +219,7 @@ int mis_collect(indigo_ctx* ctx, int* rounds)
   MisControl* h = static_cast<MisControl*>(ctx->host_pinned);
   IG_HIP_TRY(hipMemcpyAsync(h, ctl, sizeof(MisControl), hipMemcpyDeviceToHost, ctx->stream));
   IG_HIP_TRY(hipStreamSynchronize(ctx->stream));
+  IG_TRY(check_abort(ctx, reinterpret_cast<const void*>(mis_kernel), h->bar.abort.v, "mis"));
   if (h->status != 0) return set_error(INDIGO_ERR_TIMEOUT, "mis: persistent kernel stopped (status %d)", h->status);
   if (rounds) *rounds = h->rounds;
   return INDIGO_OK;

@@ -1,0 +1,381 @@
+// PageRank (FP32 scores, FP64 convergence test) on gfx950.
+//
+// Reference semantics: PR_GPU (include/pr_cuda.h:48; constants :42-45) with the
+// Jacobi pull of codes/cuda/pr-cuda/PR_CUDA_Pull_FloatType_NonPersist_Thread_
+// Determ_GlobalAdd.cu:41-69,91-106: contrib[u] = s[u] / deg[u];
+// s'[v] = (1 - 0.85) / N + 0.85 * sum_{u in N(v)} contrib[u]; repeat while the
+// L1 change of the iteration is > 1e-4 and fewer than 100 iterations ran.
+//
+// Design: one cooperative persistent kernel, one grid barrier per iteration.
+//   * contrib is double-buffered and written by the vertex that owns it in
+//     the same pass that computes its score (no separate contrib sweep);
+//   * per-vertex sums are FP32 with Neumaier compensation (the improved
+//     Kahan-Babuska sum: a running error term added once at the end); lists
+//     shorter than kPrWaveDeg are summed by one lane, longer ones by a whole
+//     wave (per-lane compensated partials combined pairwise with the same
+//     error-free step), and lists of kPrHugeDeg or more in kPrChunk-entry
+//     items over all workgroups, whose partials the last-arriving workgroup
+//     combines in chunk order (deterministic);
+//   * |s' - s| is exact in FP64 and summed in FP64, the only FP64 work: each
+//     workgroup stores its partial in a slot (slots double-buffered by
+//     iteration parity); after the barrier every workgroup adds all slots in
+//     the same fixed order, so all of them compute the bit-identical total and
+//     take the same stop decision (no floating-point atomics).
+#include "pr.hpp"
+
+#include <cmath>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "device_common.hpp"
+#include "frontier.hpp"
+
+namespace indigo {
+
+namespace {
+
+constexpr int kPrBlock = 512;
+constexpr int kPrWaveDeg = 32;
+constexpr int kPrHugeDeg = 8192;
+constexpr int kPrChunk = 4096;
+constexpr float kDamp = 0.85f;
+
+constexpr int kPrMaxGrid = 4096;
+
+struct PrControl {
+  GridBarrier bar;
+  unsigned long long hpack;
+  int iterations;
+  int status;
+  double last_diff;
+  int grid;
+  int pad[25];
+};
+
+struct PrArgs {
+  int nodes;
+  int init_uniform;
+  int max_iter;
+  double eps;
+  const int* __restrict__ nindex;
+  const int* __restrict__ nlist;
+  const int* __restrict__ degree;  // optional caller-provided degrees (PR_GPU contract)
+  float* scores;
+  float* contrib[2];
+  int* hq;
+  unsigned* hoff;
+  unsigned* hdone;
+  float2* hpart;
+  double* partial;  // [2][kPrMaxGrid] per-workgroup residuals
+  int* exit_it;     // [kPrMaxGrid] iteration at which each workgroup left
+  double* exit_total;
+  PrControl* ctl;
+  long long budget;
+  long long start_ticks;
+};
+
+// Neumaier-compensated accumulator: value = s + c.
+struct CSum {
+  float s = 0.f, c = 0.f;
+  IG_DEV void add(float x)
+  {
+    const float t = s + x;
+    c += (fabsf(s) >= fabsf(x)) ? ((s - t) + x) : ((x - t) + s);
+    s = t;
+  }
+  IG_DEV void merge(float s2, float c2)
+  {
+    add(s2);
+    c += c2;
+  }
+  IG_DEV float value() const { return s + c; }
+};
+
+IG_DEV CSum wave_merge(CSum a)
+{
+#pragma unroll
+  for (int d = kWave / 2; d > 0; d >>= 1) {
+    const float s2 = __shfl_xor(a.s, d, kWave);
+    const float c2 = __shfl_xor(a.c, d, kWave);
+    a.merge(s2, c2);
+  }
+  return a;
+}
+
+IG_DEV int deg_of(const PrArgs& a, int v)
+{
+  return a.degree ? a.degree[v] : a.nindex[v + 1] - a.nindex[v];
+}
+
+// New score of v from its gathered sum; writes score and next contrib and
+// returns |s' - s| in FP64.
+IG_DEV double finish_vertex(const PrArgs& a, int v, float total, float base, float* cout)
+{
+  const float old = a.scores[v];
+  const float s = base + kDamp * total;
+  a.scores[v] = s;
+  cout[v] = s / static_cast<float>(deg_of(a, v));
+  return fabs(static_cast<double>(s) - static_cast<double>(old));
+}
+
+__global__ __launch_bounds__(kPrBlock) void pr_kernel(PrArgs a)
+{
+  __shared__ double s_red[kPrBlock / kWave];
+  __shared__ float s_ps[kPrBlock / kWave], s_pc[kPrBlock / kWave];
+  __shared__ int s_owner;
+  PrControl* ctl = a.ctl;
+  if (!residency_barrier(&ctl->bar, a.start_ticks)) return;
+  const long long deadline = now_ticks() + a.budget;
+  const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+  const int lane = lane_id();
+  const int wid = threadIdx.x / kWave;
+  const float inv_n = 1.0f / static_cast<float>(a.nodes);
+  const float base = (1.0f - kDamp) / static_cast<float>(a.nodes);
+
+  // init scores / first contrib, collect huge lists
+  const int nwv = (a.nodes + kWave - 1) / kWave;
+  for (int c = blockIdx.x * (kPrBlock / kWave) + wid; c < nwv; c += gridDim.x * (kPrBlock / kWave)) {
+    const int v = c * kWave + lane;
+    const bool valid = v < a.nodes;
+    int deg = 0;
+    if (valid) {
+      deg = deg_of(a, v);
+      const float s = a.init_uniform ? inv_n : a.scores[v];
+      a.scores[v] = s;
+      a.contrib[0][v] = s / static_cast<float>(deg);
+    }
+    const bool huge = valid && a.nindex[v + 1] - a.nindex[v] >= kPrHugeDeg;
+    const unsigned long long hm = __ballot(huge);
+    if (hm != 0) {
+      const int len = huge ? a.nindex[v + 1] - a.nindex[v] : 0;
+      const int chunks = huge ? (len + kPrChunk - 1) / kPrChunk : 0;
+      const int incl = wave_inclusive_scan(chunks);
+      const int total = wave_shfl(incl, kWave - 1);
+      const int leader = __ffsll(static_cast<long long>(hm)) - 1;
+      unsigned long long old = 0;
+      if (lane == leader)
+        old = atomic_add_relaxed(&ctl->hpack, (static_cast<unsigned long long>(total) << 32) | __popcll(hm));
+      old = wave_shfl(old, leader);
+      if (huge) {
+        const unsigned slot = static_cast<unsigned>(old & 0xffffffffull) + lanes_below(hm);
+        a.hq[slot] = v;
+        a.hoff[slot] = static_cast<unsigned>(old >> 32) + static_cast<unsigned>(incl - chunks);
+        a.hdone[slot] = 0;
+      }
+    }
+  }
+  if (!grid_barrier(&ctl->bar, deadline)) {
+    if (tid == 0) ctl->status = 1;
+    if (threadIdx.x == 0) a.exit_it[blockIdx.x] = -1000000;
+    return;
+  }
+  const unsigned long long hp = ld_relaxed(&ctl->hpack);
+  const int nh = static_cast<int>(huge_count(hp));
+  const unsigned nch = huge_chunks(hp);
+
+  int it = 0;
+  for (;;) {
+    it++;
+    const float* cin = a.contrib[(it - 1) & 1];
+    float* cout = a.contrib[it & 1];
+    double d = 0.0;
+
+    // huge lists: one workgroup per chunk; the last one to finish a list
+    // combines its chunk partials in order and finishes the vertex
+    for (unsigned item = blockIdx.x; item < nch; item += gridDim.x) {
+      if (threadIdx.x == 0) s_owner = huge_owner(a.hoff, nh, item);
+      __syncthreads();
+      const int j = s_owner;
+      const int v = a.hq[j];
+      const int vbeg = a.nindex[v], vend = a.nindex[v + 1];
+      const int beg = vbeg + static_cast<int>(item - a.hoff[j]) * kPrChunk;
+      const int end = min(vend, beg + kPrChunk);
+      CSum acc;
+      for (int e = beg + static_cast<int>(threadIdx.x); e < end; e += kPrBlock) acc.add(cin[a.nlist[e]]);
+      acc = wave_merge(acc);
+      if (lane == 0) {
+        s_ps[wid] = acc.s;
+        s_pc[wid] = acc.c;
+      }
+      __syncthreads();
+      if (threadIdx.x == 0) {
+        CSum blk;
+        for (int w = 0; w < kPrBlock / kWave; w++) blk.merge(s_ps[w], s_pc[w]);
+        a.hpart[item] = make_float2(blk.s, blk.c);
+        const unsigned chunks = static_cast<unsigned>((vend - vbeg + kPrChunk - 1) / kPrChunk);
+        const unsigned prev = __hip_atomic_fetch_add(&a.hdone[j], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        if (prev + 1u == chunks * static_cast<unsigned>(it)) {
+          CSum all;
+          // the acquire above orders these loads after every chunk's release
+          for (unsigned k = 0; k < chunks; k++) {
+            const float2 p = a.hpart[a.hoff[j] + k];
+            all.merge(p.x, p.y);
+          }
+          d += finish_vertex(a, v, all.value(), base, cout);
+        }
+      }
+      __syncthreads();
+    }
+
+    // all other vertices: 64 per wave
+    for (int c = blockIdx.x * (kPrBlock / kWave) + wid; c < nwv; c += gridDim.x * (kPrBlock / kWave)) {
+      const int v = c * kWave + lane;
+      const bool valid = v < a.nodes;
+      const int beg = valid ? a.nindex[v] : 0;
+      const int len = valid ? a.nindex[v + 1] - beg : 0;
+      if (valid && len < kPrWaveDeg) {
+        CSum acc;
+        for (int e = beg; e < beg + len; e++) acc.add(cin[a.nlist[e]]);
+        d += finish_vertex(a, v, acc.value(), base, cout);
+      }
+      unsigned long long wide = __ballot(valid && len >= kPrWaveDeg && len < kPrHugeDeg);
+      while (wide != 0) {
+        const int l = __ffsll(static_cast<long long>(wide)) - 1;
+        wide &= wide - 1;
+        const int wv = wave_shfl(v, l);
+        const int wb = wave_shfl(beg, l);
+        const int we = wb + wave_shfl(len, l);
+        CSum acc;
+        for (int e = wb + lane; e < we; e += kWave) acc.add(cin[a.nlist[e]]);
+        acc = wave_merge(acc);
+        if (lane == l) d += finish_vertex(a, wv, acc.value(), base, cout);
+      }
+    }
+
+    // FP64 residual: this workgroup's partial into its slot ...
+    double* slots = a.partial + (it & 1) * kPrMaxGrid;
+    d = wave_sum(d);
+    if (lane == 0) s_red[wid] = d;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double t = 0.0;
+      for (int w = 0; w < kPrBlock / kWave; w++) t += s_red[w];
+      slots[blockIdx.x] = t;
+    }
+    if (!grid_barrier(&ctl->bar, deadline)) {
+      if (tid == 0) ctl->status = 1;
+      if (threadIdx.x == 0) a.exit_it[blockIdx.x] = -it;
+      return;
+    }
+    // ... and every workgroup adds all slots in one fixed order
+    double part = 0.0;
+    for (int b = threadIdx.x; b < static_cast<int>(gridDim.x); b += kPrBlock) part += slots[b];
+    part = wave_sum(part);
+    if (lane == 0) s_red[wid] = part;
+    __syncthreads();
+    double total = 0.0;
+    for (int w = 0; w < kPrBlock / kWave; w++) total += s_red[w];
+    __syncthreads();
+    if (!(total > a.eps && it < a.max_iter)) {
+      if (tid == 0) {
+        ctl->iterations = it;
+        ctl->last_diff = total;
+        ctl->grid = gridDim.x;
+      }
+      if (threadIdx.x == 0) {
+        a.exit_it[blockIdx.x] = it;
+        a.exit_total[blockIdx.x] = total;
+      }
+      break;
+    }
+  }
+}
+
+}  // namespace
+
+// Upper bound on the number of kPrChunk items: every huge list has at least
+// kPrHugeDeg entries, so items <= edges / kPrChunk + edges / kPrHugeDeg.
+static size_t pr_max_chunks(int edges) { return static_cast<size_t>(edges) / kPrChunk * 2 + 2; }
+
+size_t pr_workspace_bytes(int nodes, int edges)
+{
+  const size_t v = static_cast<size_t>(nodes);
+  return Carver::round(sizeof(PrControl)) + 5 * Carver::round(v * 4) + Carver::round(pr_max_chunks(edges) * 8) +
+         Carver::round(2 * kPrMaxGrid * 8) + Carver::round(kPrMaxGrid * 4) + Carver::round(kPrMaxGrid * 8) + 4096;
+}
+
+int pr_run(indigo_ctx* ctx, int nodes, int edges, const int* nindex, const int* nlist, const int* degree,
+           float* scores, int init_uniform)
+{
+  Carver cv(ctx->ws);
+  PrArgs a{};
+  a.ctl = cv.take<PrControl>(1);
+  a.contrib[0] = cv.take<float>(nodes);
+  a.contrib[1] = cv.take<float>(nodes);
+  a.hq = cv.take<int>(nodes);
+  a.hoff = cv.take<unsigned>(nodes);
+  a.hdone = cv.take<unsigned>(nodes);
+  a.hpart = cv.take<float2>(pr_max_chunks(edges));
+  a.partial = cv.take<double>(2 * kPrMaxGrid);
+  a.exit_it = cv.take<int>(kPrMaxGrid);
+  a.exit_total = cv.take<double>(kPrMaxGrid);
+  ctx->diag[0] = a.exit_it;
+  ctx->diag[1] = a.exit_total;
+  a.nodes = nodes;
+  a.init_uniform = init_uniform;
+  a.max_iter = 100;
+  a.eps = static_cast<double>(0.0001f);
+  a.nindex = nindex;
+  a.nlist = nlist;
+  a.degree = degree;
+  a.scores = scores;
+  a.budget = budget_ticks(ctx);
+  a.start_ticks = residency_ticks(ctx);
+  IG_TRY(zero_async(ctx, a.ctl, sizeof(PrControl)));
+  IG_TRY(zero_async(ctx, a.exit_it, sizeof(int) * kPrMaxGrid));
+  void* args[] = {&a};
+  IG_TRY(coop_launch(ctx, reinterpret_cast<const void*>(pr_kernel), kPrBlock, 0, args, kPrMaxGrid, nullptr));
+  return INDIGO_OK;
+}
+
+int pr_collect(indigo_ctx* ctx, int* iterations, double* last_diff)
+{
+  Carver cv(ctx->ws);
+  PrControl* ctl = cv.take<PrControl>(1);
+  IG_TRY(ensure_host_pinned(ctx, sizeof(PrControl)));
+  PrControl* h = static_cast<PrControl*>(ctx->host_pinned);
+  IG_HIP_TRY(hipMemcpyAsync(h, ctl, sizeof(PrControl), hipMemcpyDeviceToHost, ctx->stream));
+  IG_HIP_TRY(hipStreamSynchronize(ctx->stream));
+  IG_TRY(check_abort(ctx, reinterpret_cast<const void*>(pr_kernel), h->bar.abort.v, "pagerank"));
+  if (h->status != 0) {
+    // which iteration each workgroup left at (negative: its barrier gave
+    // up), and the stop total it computed
+    const int g = kPrMaxGrid;
+    std::vector<int> it(g);
+    std::vector<double> tot(g);
+    IG_HIP_TRY(hipMemcpy(it.data(), ctx->diag[0], sizeof(int) * g, hipMemcpyDeviceToHost));
+    IG_HIP_TRY(hipMemcpy(tot.data(), ctx->diag[1], sizeof(double) * g, hipMemcpyDeviceToHost));
+    std::map<int, int> by_it;
+    std::map<double, int> by_total;
+    for (int b = 0; b < g; b++) {
+      if (it[b] == 0) continue;
+      by_it[it[b]]++;
+      if (it[b] > 0) by_total[tot[b]]++;
+    }
+    std::string msg;
+    for (auto& kv : by_it) msg += " it" + std::to_string(kv.first) + "x" + std::to_string(kv.second);
+    msg += " | totals:";
+    int shown = 0;
+    for (auto& kv : by_total) {
+      if (shown++ == 6) break;
+      char buf[64];
+      snprintf(buf, sizeof(buf), " %.17g x%d", kv.first, kv.second);
+      msg += buf;
+    }
+    char bar[160];
+    snprintf(bar, sizeof(bar), " | barrier groups %u %u %u %u %u %u %u %u top %u gen %u",
+             h->bar.group[0].v, h->bar.group[1].v, h->bar.group[2].v, h->bar.group[3].v, h->bar.group[4].v,
+             h->bar.group[5].v, h->bar.group[6].v, h->bar.group[7].v, h->bar.top.v, h->bar.gen.v);
+    msg += bar;
+    msg += " | grid " + std::to_string(ctx->last_grid) + " per_cu " + std::to_string(ctx->last_per_cu) + " cus " +
+           std::to_string(ctx->compute_units);
+    return set_error(INDIGO_ERR_TIMEOUT, "pagerank: persistent kernel stopped (status %d); workgroups:%s",
+                     h->status, msg.c_str());
+  }
+  if (iterations) *iterations = h->iterations;
+  if (last_diff) *last_diff = h->last_diff;
+  return INDIGO_OK;
+}
+
+}  // namespace indigo

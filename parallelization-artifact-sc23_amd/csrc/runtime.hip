@@ -4,6 +4,7 @@
 #include <algorithm>
 #include <cstring>
 #include <mutex>
+#include <unordered_map>
 
 #include "device_common.hpp"
 
@@ -58,19 +59,48 @@ int ensure_host_pinned(indigo_ctx* ctx, size_t bytes)
   return INDIGO_OK;
 }
 
+namespace {
+std::mutex g_cap_mu;
+std::unordered_map<const void*, int> g_cap;  // learned co-residency per kernel
+}  // namespace
+
 int coop_launch(indigo_ctx* ctx, const void* kernel, int block, size_t shmem, void** args, int max_blocks,
                 int* grid_out)
 {
   int per_cu = 0;
   IG_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, block, shmem));
   if (per_cu < 1) return set_error(INDIGO_ERR_DEVICE, "kernel cannot be resident with block %d", block);
+  {
+    std::lock_guard<std::mutex> lock(g_cap_mu);
+    auto it = g_cap.find(kernel);
+    if (it != g_cap.end()) per_cu = std::min(per_cu, it->second);
+  }
+  if (ctx->hints[INDIGO_HINT_BLOCKS_PER_CU] > 0) per_cu = std::min(per_cu, ctx->hints[INDIGO_HINT_BLOCKS_PER_CU]);
   int grid = per_cu * ctx->compute_units;
   if (max_blocks > 0) grid = std::min(grid, max_blocks);
   grid = std::max(grid, 1);
   IG_HIP_TRY(hipLaunchCooperativeKernel(kernel, dim3(grid), dim3(block), args, static_cast<unsigned>(shmem),
                                         ctx->stream));
+  ctx->last_grid = grid;
+  ctx->last_per_cu = per_cu;
   if (grid_out) *grid_out = grid;
   return INDIGO_OK;
+}
+
+int check_abort(indigo_ctx* ctx, const void* kernel, unsigned abort_code, const char* what)
+{
+  if (abort_code == 0u) return INDIGO_OK;
+  if (abort_code == kAbortNotResident) {
+    const int launched = ctx->last_per_cu;
+    if (launched <= 1)
+      return set_error(INDIGO_ERR_DEVICE, "%s: grid of %d workgroups never assembled even at 1 per CU", what,
+                       ctx->last_grid);
+    std::lock_guard<std::mutex> lock(g_cap_mu);
+    g_cap[kernel] = launched - 1;
+    return kNotResident;
+  }
+  return set_error(INDIGO_ERR_TIMEOUT, "%s: persistent kernel stopped at a bounded wait (abort %u)", what,
+                   abort_code);
 }
 
 int zero_async(indigo_ctx* ctx, void* p, size_t bytes)
@@ -265,6 +295,14 @@ int indigo_ctx_last_stats(indigo_ctx* ctx, int* launches, int* iterations)
   if (ctx == nullptr) return set_error(INDIGO_ERR_INVALID_ARG, "ctx_last_stats: null ctx");
   if (launches) *launches = ctx->last_launches;
   if (iterations) *iterations = ctx->last_iterations;
+  return INDIGO_OK;
+}
+
+int indigo_ctx_last_grid(indigo_ctx* ctx, int* grid, int* per_cu)
+{
+  if (ctx == nullptr) return set_error(INDIGO_ERR_INVALID_ARG, "ctx_last_grid: null ctx");
+  if (grid) *grid = ctx->last_grid;
+  if (per_cu) *per_cu = ctx->last_per_cu;
   return INDIGO_OK;
 }
 

@@ -27,6 +27,9 @@ struct indigo_ctx {
   int last_iterations = 0;
   int hints[16] = {0};
   double budget_seconds = 60.0;  // bound on any device-side wait
+  void* diag[4] = {nullptr, nullptr, nullptr, nullptr};  // device diagnostics of the last call
+  int last_grid = 0;    // workgroups of the last cooperative launch
+  int last_per_cu = 0;  // co-resident workgroups per CU it was sized for
 };
 
 namespace indigo {
@@ -66,12 +69,25 @@ struct Carver {
 };
 
 // Ticks of wall_clock64() that a persistent kernel may spend in bounded waits.
-inline long long budget_ticks(const indigo_ctx* ctx) { return static_cast<long long>(ctx->budget_seconds * ctx->wall_hz); }
+inline long long budget_ticks(const indigo_ctx* ctx)
+{
+  const double sec = ctx->hints[INDIGO_HINT_BUDGET_MS] > 0 ? ctx->hints[INDIGO_HINT_BUDGET_MS] * 1e-3 : ctx->budget_seconds;
+  return static_cast<long long>(sec * static_cast<double>(ctx->wall_hz));
+}
 
 // Launches `kernel` cooperatively with as many workgroups of `block` threads
-// as can be co-resident (occupancy x compute units, capped at max_blocks).
+// as can be co-resident: occupancy x compute units, capped at max_blocks and
+// by the per-kernel residency cap learned from earlier launches.
 int coop_launch(indigo_ctx* ctx, const void* kernel, int block, size_t shmem, void** args, int max_blocks,
                 int* grid_out);
+// Internal status: the last cooperative launch did not assemble (see
+// residency_barrier); the cap of `kernel` has been lowered, launch again.
+constexpr int kNotResident = -100;
+// Checks a finished persistent kernel's barrier abort code: 0 ok,
+// kNotResident (cap lowered), or INDIGO_ERR_TIMEOUT.
+int check_abort(indigo_ctx* ctx, const void* kernel, unsigned abort_code, const char* what);
+// Ticks a persistent kernel allows for its grid to assemble.
+inline long long residency_ticks(const indigo_ctx* ctx) { return ctx->wall_hz / 10; }
 
 // Zeroes `bytes` of device memory with a kernel (no memset node, so the
 // sequence stays safe to capture into a HIP graph).

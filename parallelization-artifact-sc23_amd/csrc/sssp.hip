@@ -1,0 +1,345 @@
+// Single-source shortest paths (int32 weights and distances) on gfx950.
+//
+// Reference semantics: GPUsssp_vertex / GPUsssp_edge (include/sssp_vertex_cuda.h:60,
+// include/sssp_edge_cuda.h:60): dist[src] = 0, INT_MAX when unreachable,
+// relaxations d = dist[v] + w in int arithmetic; results equal the serial
+// Dijkstra of sssp_vertex_cuda.h:74-99.
+//
+// Design: one cooperative persistent kernel running near-far
+// label-correcting SSSP (Davidson et al., IPDPS'14):
+//   * the near frontier holds vertices whose distance improved below the
+//     split T; it is expanded with the degree-binned machinery of
+//     frontier.hpp (64-lane in-wave edge balancing, huge lists split over
+//     workgroups); each edge relaxes with atomicMin and, on improvement, the
+//     target joins the next near frontier (deduplicated per iteration with an
+//     exchange on stamp[u]) or, if its distance is >= T, the far pile
+//     (deduplicated per epoch with farstamp[u]) while far_min tracks the
+//     smallest parked distance;
+//   * far piles rotate over three buffers: epoch e files into pile e % 3 and
+//     re-files into pile (e + 1) % 3 while pile (e + 2) % 3, consumed one
+//     barrier earlier, is cleared - no buffer ever has two roles at once;
+//   * when the near frontier empties, the split moves to
+//     max(T, far_min) + delta and one pass re-files the far pile: entries
+//     below the new split join the near frontier, settled entries (distance
+//     below the old split, already expanded at that distance) are dropped,
+//     the rest stay parked;
+//   * vertices are expanded with their current distance (re-read at
+//     expansion), so stale duplicates only cost work, never correctness.
+// delta defaults to a sampled mean weight x 32 / mean degree (at least 1).
+#include "sssp.hpp"
+
+#include "frontier.hpp"
+
+namespace indigo {
+
+namespace {
+
+constexpr int kSsspBlock = 512;
+
+struct SsspControl {
+  GridBarrier bar;
+  FrontierCounters cnt[3];
+  PaddedU32 far_cnt[3];
+  PaddedU32 far_min[3];
+  int iterations;
+  int epochs;
+  int status;
+  int delta;
+  int pad[28];
+};
+
+struct SsspArgs {
+  int nodes;
+  int edges;
+  int src;
+  int delta_hint;
+  const int* __restrict__ nindex;
+  const int* __restrict__ nlist;
+  const int* __restrict__ eweight;
+  int* dist;
+  int* stamp;
+  int* farstamp;
+  int* far[3];
+  FrontierQueues fq[2];
+  SsspControl* ctl;
+  long long budget;
+  long long start_ticks;
+};
+
+IG_DEV int degree_of(const int* __restrict__ nindex, int u) { return nindex[u + 1] - nindex[u]; }
+
+// Relaxation result classes.
+struct Relax {
+  bool near;  // joins the next near frontier
+  int u;
+  int udeg;
+};
+
+// Relaxes edge (dv + w -> u).  Near improvements are deduplicated by stamp;
+// far ones are parked directly (wave-aggregated by the caller).
+IG_DEV Relax relax(const SsspArgs& a, int dv, int e, int T, int it_stamp, int epoch, int* far, unsigned* far_cnt,
+                   unsigned* far_min, bool active)
+{
+  Relax r{false, 0, 0};
+  bool park = false;
+  int u = 0, nd = 0;
+  if (active) {
+    u = a.nlist[e];
+    nd = dv + a.eweight[e];
+    if (nd < a.dist[u]) {
+      const int old = atomic_min_relaxed(&a.dist[u], nd);
+      if (nd < old) {
+        if (nd < T) {
+          if (__hip_atomic_exchange(&a.stamp[u], it_stamp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != it_stamp) {
+            r.near = true;
+            r.u = u;
+            r.udeg = degree_of(a.nindex, u);
+          }
+        } else if (__hip_atomic_exchange(&a.farstamp[u], epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != epoch) {
+          park = true;
+        }
+      }
+    }
+  }
+  const int slot = wave_append(park, u, far, far_cnt);
+  (void)slot;
+  const int m = wave_min(park ? nd : kInf);
+  if (lane_id() == 0 && m != kInf) atomic_min_relaxed(far_min, static_cast<unsigned>(m));
+  return r;
+}
+
+__global__ __launch_bounds__(kSsspBlock) void sssp_kernel(SsspArgs a)
+{
+  __shared__ int s_owner;
+  SsspControl* ctl = a.ctl;
+  if (!residency_barrier(&ctl->bar, a.start_ticks)) return;
+  const long long deadline = now_ticks() + a.budget;
+  const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+  const int stride = gridDim.x * blockDim.x;
+  const int lane = lane_id();
+  for (int i = tid; i < a.nodes; i += stride) {
+    a.dist[i] = (i == a.src) ? 0 : kInf;
+    a.stamp[i] = 0;
+    a.farstamp[i] = 0;
+  }
+  if (blockIdx.x == 0) {
+    // delta: sampled mean weight x 32 / mean degree
+    __shared__ long long s_wsum[kSsspBlock / kWave];
+    __shared__ int s_ns[kSsspBlock / kWave];
+    long long wsum = 0;
+    int ns = 0;
+    if (a.delta_hint <= 0 && a.edges > 0) {
+      const int samples = a.edges < 4096 ? a.edges : 4096;
+      for (int k = threadIdx.x; k < samples; k += blockDim.x) {
+        const long long e = static_cast<long long>(k) * a.edges / samples;
+        wsum += a.eweight[e];
+        ns++;
+      }
+    }
+    wsum = wave_sum(wsum);
+    ns = wave_sum(ns);
+    if (lane == 0) {
+      s_wsum[threadIdx.x / kWave] = wsum;
+      s_ns[threadIdx.x / kWave] = ns;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      int delta = a.delta_hint;
+      if (delta <= 0) {
+        long long ws = 0;
+        int cnt = 0;
+        for (int w = 0; w < kSsspBlock / kWave; w++) {
+          ws += s_wsum[w];
+          cnt += s_ns[w];
+        }
+        const double mean_w = cnt > 0 ? static_cast<double>(ws) / cnt : 1.0;
+        const double mean_d = static_cast<double>(a.edges) / a.nodes;
+        double d = mean_w * 32.0 / (mean_d > 1.0 ? mean_d : 1.0);
+        delta = d < 1.0 ? 1 : (d > 1e9 ? 1000000000 : static_cast<int>(d));
+      }
+      ctl->delta = delta;
+      const int deg = degree_of(a.nindex, a.src);
+      FrontierCounters* c0 = &ctl->cnt[0];
+      if (deg >= kHugeDeg) {
+        a.fq[0].hq[0] = a.src;
+        a.fq[0].hoff[0] = 0;
+        c0->hpack = (static_cast<unsigned long long>((deg + kHugeChunk - 1) / kHugeChunk) << 32) | 1ull;
+      } else {
+        a.fq[0].q[0] = a.src;
+        c0->n = 1;
+      }
+      for (int k = 0; k < 3; k++) ctl->far_min[k].v = static_cast<unsigned>(kInf);
+    }
+  }
+  if (!grid_barrier(&ctl->bar, deadline)) {
+    if (tid == 0) ctl->status = 1;
+    return;
+  }
+  const int delta = ld_relaxed(&ctl->delta);
+  int T = delta;  // split: near holds distances < T
+  int epoch = 1;  // far pile generation (farstamp value); far[epoch % 3] is current
+  int it = 0;     // near iterations (stamp values start at 1)
+  int L = 0;      // counter rotation index
+  const int wpb = kSsspBlock / kWave;
+  const unsigned gwave = blockIdx.x * wpb + threadIdx.x / kWave;
+  const unsigned nwaves = gridDim.x * wpb;
+  for (;;) {
+    FrontierCounters* ic = &ctl->cnt[L % 3];
+    const unsigned n = ld_relaxed(&ic->n);
+    const unsigned long long hp = ld_relaxed(&ic->hpack);
+    const unsigned nh = huge_count(hp), nch = huge_chunks(hp);
+    FrontierCounters* oc = &ctl->cnt[(L + 1) % 3];
+    const FrontierQueues& in = a.fq[L & 1];
+    const FrontierQueues& out = a.fq[(L + 1) & 1];
+    if (tid == 0) {
+      FrontierCounters* z = &ctl->cnt[(L + 2) % 3];
+      z->n = 0;
+      z->hpack = 0;
+      z->m = 0;
+    }
+    int* far = a.far[epoch % 3];
+    unsigned* far_cnt = &ctl->far_cnt[epoch % 3].v;
+    unsigned* far_min = &ctl->far_min[epoch % 3].v;
+    if (static_cast<unsigned long long>(n) + nh == 0) {
+      // near frontier empty: move the split and re-file the far pile
+      const unsigned nf = ld_relaxed(far_cnt);
+      if (nf == 0) break;
+      const int fmin = static_cast<int>(ld_relaxed(far_min));
+      const int T_old = T;
+      const long long Tn = static_cast<long long>(T > fmin ? T : fmin) + delta;
+      T = Tn > kInf ? kInf : static_cast<int>(Tn);
+      it++;
+      const int nepoch = epoch + 1;
+      int* nfar = a.far[nepoch % 3];
+      unsigned* nfar_cnt = &ctl->far_cnt[nepoch % 3].v;
+      unsigned* nfar_min = &ctl->far_min[nepoch % 3].v;
+      if (tid == 0) {
+        // pile (epoch + 2) % 3 was consumed by the previous split, one barrier ago
+        st_relaxed(&ctl->far_cnt[(epoch + 2) % 3].v, 0u);
+        st_relaxed(&ctl->far_min[(epoch + 2) % 3].v, static_cast<unsigned>(kInf));
+      }
+      for (unsigned base = blockIdx.x * blockDim.x; base < nf; base += stride) {
+        const unsigned i = base + threadIdx.x;
+        bool to_near = false, keep = false;
+        int u = 0, du = 0, udeg = 0;
+        if (i < nf) {
+          u = far[i];
+          du = a.dist[u];
+          if (du >= T_old) {
+            if (du < T) {
+              to_near = __hip_atomic_exchange(&a.stamp[u], it, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != it;
+              if (to_near) udeg = degree_of(a.nindex, u);
+            } else {
+              keep = __hip_atomic_exchange(&a.farstamp[u], nepoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != nepoch;
+            }
+          }
+        }
+        frontier_emit(to_near, u, udeg, out, oc);
+        wave_append(keep, u, nfar, nfar_cnt);
+        const int m = wave_min(keep ? du : kInf);
+        if (lane == 0 && m != kInf) atomic_min_relaxed(nfar_min, static_cast<unsigned>(m));
+      }
+      if (tid == 0) ctl->epochs++;
+      epoch = nepoch;
+    } else {
+      it++;
+      // huge lists of the near frontier
+      for (unsigned item = blockIdx.x; item < nch; item += gridDim.x) {
+        if (threadIdx.x == 0) s_owner = huge_owner(in.hoff, static_cast<int>(nh), item);
+        __syncthreads();
+        const int j = s_owner;
+        __syncthreads();
+        const int v = in.hq[j];
+        const int dv = a.dist[v];
+        const int beg = a.nindex[v] + static_cast<int>(item - in.hoff[j]) * kHugeChunk;
+        const int end = min(a.nindex[v + 1], beg + kHugeChunk);
+        for (int base = beg; base < end; base += blockDim.x) {
+          const int e = base + static_cast<int>(threadIdx.x);
+          const Relax r = relax(a, dv, e, T, it, epoch, far, far_cnt, far_min, e < end);
+          frontier_emit(r.near, r.u, r.udeg, out, oc);
+        }
+      }
+      // normal near frontier, 64 vertices per wave
+      for (unsigned c = gwave; c * kWave < n; c += nwaves) {
+        const unsigned idx = c * kWave + lane;
+        const bool valid = idx < n;
+        const int v = valid ? in.q[idx] : 0;
+        const int beg = valid ? a.nindex[v] : 0;
+        const int deg = valid ? a.nindex[v + 1] - beg : 0;
+        const int dv = valid ? a.dist[v] : 0;
+        const int incl = wave_inclusive_scan(deg);
+        const int total = wave_shfl(incl, kWave - 1);
+        for (int base = 0; base < total; base += kWave) {
+          const int k = base + lane;
+          const int o = min(wave_owner(incl, k), kWave - 1);
+          const int obeg = wave_shfl(beg, o);
+          const int oexcl = wave_shfl(incl - deg, o);
+          const int odv = wave_shfl(dv, o);
+          const Relax r = relax(a, odv, obeg + (k - oexcl), T, it, epoch, far, far_cnt, far_min, k < total);
+          frontier_emit(r.near, r.u, r.udeg, out, oc);
+        }
+      }
+    }
+    if (!grid_barrier(&ctl->bar, deadline)) {
+      if (tid == 0) ctl->status = 1;
+      return;
+    }
+    L++;
+  }
+  if (tid == 0) ctl->iterations = it;
+}
+
+}  // namespace
+
+size_t sssp_workspace_bytes(int nodes)
+{
+  const size_t v = static_cast<size_t>(nodes);
+  return Carver::round(sizeof(SsspControl)) + 11 * Carver::round(v * 4) + 4096;
+}
+
+int sssp_run(indigo_ctx* ctx, int nodes, int edges, const int* nindex, const int* nlist, const int* eweight, int src,
+             int* dist)
+{
+  Carver cv(ctx->ws);
+  SsspArgs a{};
+  a.ctl = cv.take<SsspControl>(1);
+  for (int k = 0; k < 2; k++) {
+    a.fq[k].q = cv.take<int>(nodes);
+    a.fq[k].hq = cv.take<int>(nodes);
+    a.fq[k].hoff = cv.take<unsigned>(nodes);
+  }
+  for (int k = 0; k < 3; k++) a.far[k] = cv.take<int>(nodes);
+  a.stamp = cv.take<int>(nodes);
+  a.farstamp = cv.take<int>(nodes);
+  a.nodes = nodes;
+  a.edges = edges;
+  a.src = src;
+  a.delta_hint = ctx->hints[INDIGO_HINT_SSSP_DELTA];
+  a.nindex = nindex;
+  a.nlist = nlist;
+  a.eweight = eweight;
+  a.dist = dist;
+  a.budget = budget_ticks(ctx);
+  a.start_ticks = residency_ticks(ctx);
+  IG_TRY(zero_async(ctx, a.ctl, sizeof(SsspControl)));
+  void* args[] = {&a};
+  IG_TRY(coop_launch(ctx, reinterpret_cast<const void*>(sssp_kernel), kSsspBlock, 0, args, 0, nullptr));
+  return INDIGO_OK;
+}
+
+int sssp_collect(indigo_ctx* ctx, int* iterations, int* epochs)
+{
+  Carver cv(ctx->ws);
+  SsspControl* ctl = cv.take<SsspControl>(1);
+  IG_TRY(ensure_host_pinned(ctx, sizeof(SsspControl)));
+  SsspControl* h = static_cast<SsspControl*>(ctx->host_pinned);
+  IG_HIP_TRY(hipMemcpyAsync(h, ctl, sizeof(SsspControl), hipMemcpyDeviceToHost, ctx->stream));
+  IG_HIP_TRY(hipStreamSynchronize(ctx->stream));
+  IG_TRY(check_abort(ctx, reinterpret_cast<const void*>(sssp_kernel), h->bar.abort.v, "sssp"));
+  if (h->status != 0) return set_error(INDIGO_ERR_TIMEOUT, "sssp: persistent kernel stopped (status %d)", h->status);
+  if (iterations) *iterations = h->iterations;
+  if (epochs) *epochs = h->epochs;
+  return INDIGO_OK;
+}
+
+}  // namespace indigo

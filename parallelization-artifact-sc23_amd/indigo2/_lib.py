@@ -26,6 +26,8 @@ ERR_NO_DEVICE = -7
 HINT_BFS_DIRECTION = 1
 HINT_SSSP_DELTA = 2
 HINT_SMALL_FRONTIER = 3
+HINT_BUDGET_MS = 4
+HINT_BLOCKS_PER_CU = 5
 
 
 class IndigoError(RuntimeError):
@@ -81,6 +83,7 @@ _SIGNATURES = {
     "indigo_ctx_last_ms": [c_void_p, _PF],
     "indigo_ctx_last_stats": [c_void_p, _PI, _PI],
     "indigo_ctx_trim": [c_void_p],
+    "indigo_ctx_last_grid": [c_void_p, _PI, _PI],
     "indigo_ctx_set_hint": [c_void_p, c_int, c_int],
     "indigo_bfs": [c_void_p, c_int, c_int, c_void_p, c_void_p, c_int, c_void_p],
     "indigo_sssp": [c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p, c_int, c_void_p],

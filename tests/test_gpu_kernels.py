@@ -1,0 +1,132 @@
+"""Parity of SSSP, CC, MIS, PageRank and TC on the GPU against the CPU oracle:
+bit-exact for the integer kernels, and for PageRank max-abs <= 1e-6 per vertex
+and L1 <= 1e-4 against the FP64 oracle run for the same iteration count."""
+import numpy as np
+import pytest
+
+from graphs import catalog, clique, weighted
+
+pytestmark = pytest.mark.gpu
+
+PR_MAX_ABS = 1e-6
+PR_L1 = 1e-4
+
+
+def test_sssp_catalog(ig, orc, ctx):
+    for g in catalog(ig, weights=True):
+        dg = ig.DeviceGraph(g)
+        for src in sorted({0, g.nodes // 3, g.nodes - 1}):
+            assert np.array_equal(ctx.sssp(dg, src), orc.sssp(g, src)), (g.name, src)
+
+
+@pytest.mark.parametrize("delta", [0, 1, 7, 100000])
+def test_sssp_delta_hints(ig, orc, ctx, delta):
+    ctx.set_hint(ig._lib.HINT_SSSP_DELTA, delta)
+    try:
+        for g in (ig.rmat(14, 16, seed=3, wmin=1, wmax=255), ig.road(96, 96, seed=2, wmin=1, wmax=1000)):
+            dg = ig.DeviceGraph(g)
+            assert np.array_equal(ctx.sssp(dg, 0), orc.sssp(g, 0)), (g.name, delta)
+    finally:
+        ctx.set_hint(ig._lib.HINT_SSSP_DELTA, 0)
+
+
+def test_sssp_rmat18(ig, orc, ctx):
+    g = ig.rmat(18, 16, seed=1, wmin=1, wmax=255)
+    assert np.array_equal(ctx.sssp(ig.DeviceGraph(g), 0), orc.sssp(g, 0))
+
+
+def test_cc_catalog(ig, orc, ctx):
+    for g in catalog(ig):
+        assert np.array_equal(ctx.cc(ig.DeviceGraph(g)), orc.cc(g)), g.name
+
+
+def test_cc_large(ig, orc, ctx):
+    for g in (ig.rmat(18, 16, seed=2), ig.road(256, 256, keep_p=0.5, seed=3)):
+        assert np.array_equal(ctx.cc(ig.DeviceGraph(g)), orc.cc(g)), g.name
+
+
+@pytest.mark.parametrize("signed", [False, True])
+def test_mis_catalog(ig, orc, ctx, signed):
+    for g in catalog(ig):
+        assert np.array_equal(ctx.mis(ig.DeviceGraph(g), signed), orc.mis(g, signed)), (g.name, signed)
+
+
+def test_mis_large(ig, orc, ctx):
+    g = ig.rmat(18, 16, seed=4)
+    st = ctx.mis(ig.DeviceGraph(g))
+    assert np.array_equal(st, orc.mis(g))
+
+
+def _pr_check(ig, orc, g, scores, iters):
+    ref, _ = orc.pr64(g, iters)
+    err = np.abs(scores.astype(np.float64) - ref)
+    assert err.max() <= PR_MAX_ABS, (g.name, err.max())
+    assert err.sum() <= PR_L1, (g.name, err.sum())
+
+
+def test_pagerank_catalog(ig, orc, ctx):
+    for g in catalog(ig):
+        scores, iters = ctx.pagerank(ig.DeviceGraph(g))
+        assert 1 <= iters <= 100
+        _pr_check(ig, orc, g, scores, iters)
+        # iteration count agrees with the reference recurrence (FP32) to +-1
+        _, it32 = orc.pr_ref32(g)
+        assert abs(it32 - iters) <= 1, (g.name, it32, iters)
+
+
+def test_pagerank_rmat16(ig, orc, ctx):
+    g = ig.rmat(16, 16, seed=5)
+    scores, iters = ctx.pagerank(ig.DeviceGraph(g))
+    _pr_check(ig, orc, g, scores, iters)
+
+
+def test_tc_catalog(ig, orc, ctx):
+    for g in catalog(ig):
+        assert ctx.tc(ig.DeviceGraph(g)) == orc.tc(g), g.name
+    assert ctx.tc(ig.DeviceGraph(clique(ig, 40))) == 40 * 39 * 38 // 6
+
+
+def test_tc_rmat16(ig, orc, ctx):
+    g = ig.rmat(16, 16, seed=6)
+    assert ctx.tc(ig.DeviceGraph(g)) == orc.tc(g)
+
+
+def test_tc_rejects_unsorted(ig, ctx):
+    g = ig.grid(8, 8)
+    nl = g.nlist.copy()
+    nl[g.nindex[9]], nl[g.nindex[9] + 1] = nl[g.nindex[9] + 1], nl[g.nindex[9]]
+    bad = ig.HostGraph(g.nodes, g.edges, g.nindex, nl)
+    with pytest.raises(ig.IndigoError) as e:
+        ctx.tc(ig.DeviceGraph(bad))
+    assert e.value.code == ig._lib.ERR_UNSORTED
+
+
+def test_dropin_entries(ig, orc):
+    g = ig.rmat(12, 16, seed=9, wmin=1, wmax=255)
+    dg = ig.DeviceGraph(g)
+    sp = g.sp()
+    d_sp = ig.DeviceBuffer.from_array(sp)
+    dist = np.empty(g.nodes, np.int32)
+    ig.GPUsssp_vertex(3, dg, dist)
+    assert np.array_equal(dist, orc.sssp(g, 3))
+    ig.GPUsssp_edge(4, dg, dist, sp)
+    assert np.array_equal(dist, orc.sssp(g, 4))
+    lab = np.empty(g.nodes, np.int32)
+    ig.GPUcc_vertex(dg, lab)
+    assert np.array_equal(lab, orc.cc(g))
+    ig.GPUcc_edge(dg, lab, sp)
+    assert np.array_equal(lab, orc.cc(g))
+    pri = np.empty(g.nodes, np.uint32)
+    st = np.empty(g.nodes, np.int32)
+    ig.GPUmis_vertex(dg, pri, st)
+    assert np.array_equal(st, orc.mis(g))
+    assert pri[5] == orc.mis_hash(5 + 712313887)
+    ig.GPUmis_edge(dg, d_sp, pri, st, signed_priority=True)
+    assert np.array_equal(st, orc.mis(g, True))
+    scores = np.full(g.nodes, 1.0 / g.nodes, np.float32)
+    _, iters = ig.PR_GPU(g, scores, g.degree)
+    _pr_check(ig, orc, g, scores, iters)
+    cnt, _ = ig.GPUtc_vertex(dg)
+    assert cnt == orc.tc(g)
+    cnt, _ = ig.GPUtc_edge(dg, d_sp)
+    assert cnt == orc.tc(g)
