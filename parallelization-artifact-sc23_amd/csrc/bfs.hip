@@ -11,17 +11,20 @@
 // e.g. codes/cuda/bfs-cuda/BFS_CUDA_V_Topo_Push_Determ_IntType_ReadModifyWrite_
 // NonPersist_Atomic_Thread_NoBoundsBug2_NoBoundsBug1.cu:98-106).
 // Each level is either
-//   * top-down: the degree-binned frontier (normal queue by 64-lane waves with
-//     an in-wave edge balancer, huge lists split in 2048-edge chunks over all
-//     workgroups); a neighbour is claimed with one CAS INT_MAX -> L+1 so each
-//     vertex enters the next frontier exactly once; or
-//   * bottom-up: every unvisited vertex scans its list for a parent with
-//     dist == L and stops at the first one (Beamer's direction-optimising BFS,
-//     SC'12), which reads far fewer than E entries on low-diameter graphs.
-// The direction switches on Beamer's heuristic (m_f > m_u / 14 -> bottom-up,
+//   * top-down: the degree-binned frontier of frontier.hpp (small lists by
+//     64-lane waves with the in-wave balancer, mid lists one wave each, huge
+//     lists in 2048-entry chunks over all workgroups); a neighbour is claimed
+//     with one CAS INT_MAX -> L+1, so each vertex joins the next frontier
+//     exactly once; or
+//   * bottom-up: every unvisited vertex looks for a parent with dist == L and
+//     stops at the first one (Beamer, Asanovic, Patterson, SC'12): short
+//     lists one lane each in vertex order, long lists (collected once at
+//     start) one wave each with a ballot per 64 entries.
+// The direction follows Beamer's heuristic (m_f > m_u / 14 -> bottom-up,
 // n_f < V / 24 -> top-down), evaluated identically by every workgroup.
 // Small frontiers (high-diameter graphs) are advanced by workgroup 0 alone
-// with LDS-resident queues and workgroup barriers only; see bfs_small_phase.
+// with LDS queues and workgroup barriers only (small_phase).  Vertices without
+// neighbours get their level but never enter a frontier.
 #include "bfs.hpp"
 
 #include "frontier.hpp"
@@ -31,14 +34,15 @@ namespace indigo {
 namespace {
 
 constexpr int kBfsBlock = 512;
-// One-workgroup phase: frontier capacity held in LDS and the frontier size
-// below which the grid hands a level over to workgroup 0.
+// One-workgroup phase: LDS queue capacity; the phase runs while the frontier's
+// degree sum (an upper bound on the next frontier) fits.
 constexpr int kSmallCap = 4096;
-constexpr int kSmallEnter = 2048;
 
 struct BfsControl {
   GridBarrier bar;
+  PaddedU32 small_ack;  // workgroups that have read the counters before a one-workgroup phase
   FrontierCounters cnt[3];
+  FrontierCounters heavy;  // static lists of >= kMidDeg entries (bottom-up)
   int levels;
   int bottom_up_levels;
   int small_levels;
@@ -57,6 +61,7 @@ struct BfsArgs {
   const int* __restrict__ nlist;
   int* dist;
   FrontierQueues fq[2];
+  FrontierQueues heavy;
   BfsControl* ctl;
   long long budget;
   long long start_ticks;
@@ -70,65 +75,31 @@ IG_DEV bool claim(int* dist, int u, int next)
   return dist[u] == kInf && cas_relaxed(&dist[u], kInf, next);
 }
 
-IG_DEV void top_down(const BfsArgs& a, int L, unsigned n, unsigned nh, unsigned nch, const FrontierQueues& in,
+IG_DEV void top_down(const BfsArgs& a, int L, const FrontierCounters& ic, const FrontierQueues& in,
                      const FrontierQueues& out, FrontierCounters* oc)
 {
-  __shared__ int s_owner;
-  const int lane = lane_id();
   const int next = L + 1;
-  // huge adjacency lists: one workgroup per kHugeChunk-edge item
-  for (unsigned item = blockIdx.x; item < nch; item += gridDim.x) {
-    if (threadIdx.x == 0) s_owner = huge_owner(in.hoff, static_cast<int>(nh), item);
-    __syncthreads();
-    const int j = s_owner;
-    __syncthreads();
-    const int v = in.hq[j];
-    const int beg = a.nindex[v] + static_cast<int>(item - in.hoff[j]) * kHugeChunk;
-    const int end = min(a.nindex[v + 1], beg + kHugeChunk);
-    for (int base = beg; base < end; base += blockDim.x) {
-      const int e = base + static_cast<int>(threadIdx.x);
-      bool won = false;
-      int u = 0, udeg = 0;
-      if (e < end) {
-        u = a.nlist[e];
-        won = claim(a.dist, u, next);
-        if (won) udeg = degree_of(a.nindex, u);
-      }
-      frontier_emit(won, u, udeg, out, oc);
+  auto visit = [&](int, int e, bool active) {
+    bool won = false;
+    int u = 0, udeg = 0;
+    if (active) {
+      u = a.nlist[e];
+      won = claim(a.dist, u, next);
+      if (won) udeg = degree_of(a.nindex, u);
     }
-  }
-  // normal queue: one wave per 64 frontier vertices, edges balanced in-wave
-  const int wpb = blockDim.x / kWave;
-  const unsigned gwave = blockIdx.x * wpb + threadIdx.x / kWave;
-  const unsigned nwaves = gridDim.x * wpb;
-  for (unsigned c = gwave; c * kWave < n; c += nwaves) {
-    const unsigned idx = c * kWave + lane;
-    const bool valid = idx < n;
-    const int v = valid ? in.q[idx] : 0;
-    const int beg = valid ? a.nindex[v] : 0;
-    const int deg = valid ? a.nindex[v + 1] - beg : 0;
-    const int incl = wave_inclusive_scan(deg);
-    const int total = wave_shfl(incl, kWave - 1);
-    for (int base = 0; base < total; base += kWave) {
-      const int k = base + lane;
-      const int o = min(wave_owner(incl, k), kWave - 1);
-      const int obeg = wave_shfl(beg, o);
-      const int oexcl = wave_shfl(incl - deg, o);
-      bool won = false;
-      int u = 0, udeg = 0;
-      if (k < total) {
-        u = a.nlist[obeg + (k - oexcl)];
-        won = claim(a.dist, u, next);
-        if (won) udeg = degree_of(a.nindex, u);
-      }
-      frontier_emit(won, u, udeg, out, oc);
-    }
-  }
+    frontier_emit(won && udeg > 0, u, udeg, out, oc);
+  };
+  expand_huge(in, huge_count(ic.hpack), huge_chunks(ic.hpack), a.nindex, visit);
+  expand_mid(in, ic.nm, a.nindex, visit);
+  expand_small(in.q, ic.n, a.nindex, visit);
 }
 
-IG_DEV void bottom_up(const BfsArgs& a, int L, const FrontierQueues& out, FrontierCounters* oc)
+IG_DEV void bottom_up(const BfsArgs& a, int L, unsigned heavy_mid, unsigned heavy_huge, const FrontierQueues& out,
+                      FrontierCounters* oc)
 {
   const int next = L + 1;
+  const int lane = lane_id();
+  // short lists: one lane per vertex, vertex order
   const int stride = gridDim.x * blockDim.x;
   for (int base = blockIdx.x * blockDim.x; base < a.nodes; base += stride) {
     const int v = base + static_cast<int>(threadIdx.x);
@@ -137,52 +108,77 @@ IG_DEV void bottom_up(const BfsArgs& a, int L, const FrontierQueues& out, Fronti
     if (v < a.nodes && a.dist[v] == kInf) {
       const int beg = a.nindex[v], end = a.nindex[v + 1];
       deg = end - beg;
-      for (int e = beg; e < end; e++) {
-        if (a.dist[a.nlist[e]] == L) {
-          found = true;
-          break;
+      if (deg < kMidDeg) {
+        for (int e = beg; e < end; e++) {
+          if (a.dist[a.nlist[e]] == L) {
+            found = true;
+            break;
+          }
         }
+        if (found) a.dist[v] = next;
       }
-      if (found) a.dist[v] = next;
     }
     frontier_emit(found, v, deg, out, oc);
   }
+  // long lists: one wave per vertex, 64 entries per ballot
+  const int wpb = blockDim.x / kWave;
+  const unsigned total = heavy_mid + heavy_huge;
+  for (unsigned i = blockIdx.x * wpb + threadIdx.x / kWave; i < total; i += gridDim.x * wpb) {
+    const int v = i < heavy_mid ? a.heavy.mq[i] : a.heavy.hq[i - heavy_mid];
+    if (a.dist[v] != kInf) continue;
+    const int beg = a.nindex[v], end = a.nindex[v + 1];
+    bool found = false;
+    for (int base = beg; base < end; base += kWave) {
+      const int e = base + lane;
+      if (__ballot(e < end && a.dist[a.nlist[e]] == L) != 0) {
+        found = true;
+        break;
+      }
+    }
+    if (found && lane == 0) a.dist[v] = next;
+    frontier_emit(found && lane == 0, v, end - beg, out, oc);
+  }
 }
 
-// Workgroup 0 advances levels alone while the frontier fits in LDS: each
-// level costs workgroup barriers only.  Entered with the level-L frontier in
-// the normal global queue `in` (n entries, no huge entries); leaves when the
-// frontier is empty or outgrows kSmallCap, writing it back to the global
-// queue of the level it stopped at.  Returns the level reached.
-IG_DEV int small_phase(const BfsArgs& a, int L, unsigned n, const FrontierQueues* fq, GridBarrier* bar,
-                       long long deadline, unsigned* out_n, bool* aborted)
+// Workgroup 0 advances levels alone while the frontier's degree sum fits the
+// LDS queue, paying workgroup barriers only.  Entered at level L with the
+// frontier in the small global queue (n entries, degree sum m <= kSmallCap);
+// on leaving (frontier empty, or its degree sum too large) it re-emits the
+// frontier of the level it reached into that level's global queues and
+// counters, cleared first.  The counters may only be touched once every other
+// workgroup has read level L's counters: each of them bumps small_ack
+// (release) after deciding on this phase, and workgroup 0 waits for all of
+// them (bounded) before the hand-back.  Returns the level reached.
+IG_DEV int small_phase(const BfsArgs& a, int L, unsigned n, BfsControl* ctl, long long deadline, bool* aborted)
 {
   __shared__ int s_q[2][kSmallCap];
   __shared__ int s_cnt[2];
-  __shared__ int s_overflow;
+  __shared__ int s_m[2];
+  __shared__ int s_abort;
   const int lane = lane_id();
-  const int* gin = fq[L & 1].q;
+  const int* gin = a.fq[L & 1].q;
   for (unsigned i = threadIdx.x; i < n; i += blockDim.x) s_q[0][i] = gin[i];
   if (threadIdx.x == 0) {
     s_cnt[0] = static_cast<int>(n);
     s_cnt[1] = 0;
-    s_overflow = 0;
+    s_m[0] = 0;
+    s_m[1] = 0;
+    s_abort = 0;
   }
   __syncthreads();
   int cur = 0;
   *aborted = false;
   for (;;) {
     const int cn = s_cnt[cur];
-    if (cn == 0 || cn > kSmallEnter) break;
-    if ((L & 63) == 0 && threadIdx.x == 0) s_overflow = kernel_aborted(bar, deadline) ? 2 : 0;
+    if (cn == 0 || s_m[cur] > kSmallCap) break;
+    if ((L & 63) == 0 && threadIdx.x == 0) s_abort = kernel_aborted(&ctl->bar, deadline) ? 1 : 0;
     __syncthreads();
-    if (s_overflow == 2) {
+    if (s_abort) {
       *aborted = true;
       return L;
     }
     const int nxt = cur ^ 1;
     const int next = L + 1;
-    // expand: one wave per 64 frontier vertices with the in-wave balancer
     const int wpb = blockDim.x / kWave;
     for (int c = threadIdx.x / kWave; c * kWave < cn; c += wpb) {
       const int idx = c * kWave + lane;
@@ -198,36 +194,65 @@ IG_DEV int small_phase(const BfsArgs& a, int L, unsigned n, const FrontierQueues
         const int obeg = wave_shfl(beg, o);
         const int oexcl = wave_shfl(incl - deg, o);
         bool won = false;
-        int u = 0;
+        int u = 0, udeg = 0;
         if (k < total) {
           u = a.nlist[obeg + (k - oexcl)];
-          won = claim(a.dist, u, next);
+          if (claim(a.dist, u, next)) {
+            udeg = degree_of(a.nindex, u);
+            won = udeg > 0;
+          }
         }
         const unsigned long long m = __ballot(won);
         if (m != 0) {
           const int leader = __ffsll(static_cast<long long>(m)) - 1;
+          const int dsum = wave_sum(won ? udeg : 0);
           int b = 0;
-          if (lane == leader) b = atomicAdd(&s_cnt[nxt], __popcll(m));
-          b = wave_shfl(b, leader);
-          const int slot = b + lanes_below(m);
-          if (won) {
-            if (slot < kSmallCap) s_q[nxt][slot] = u;
-            else fq[next & 1].q[slot] = u;  // spill: written straight to the global queue
+          if (lane == leader) {
+            b = atomicAdd(&s_cnt[nxt], __popcll(m));
+            atomicAdd(&s_m[nxt], dsum);
           }
+          b = wave_shfl(b, leader);
+          // the previous level's degree sum bounds this count by kSmallCap
+          if (won) s_q[nxt][b + lanes_below(m)] = u;
         }
       }
     }
     __syncthreads();
-    if (threadIdx.x == 0) s_cnt[cur] = 0;
+    if (threadIdx.x == 0) {
+      s_cnt[cur] = 0;
+      s_m[cur] = 0;
+    }
     cur = nxt;
     L = next;
     __syncthreads();
   }
-  // write the frontier of level L back to the global queue for that level
+  // hand the level-L frontier back to the grid, once all others have read
+  // the entry level's counters
+  if (threadIdx.x == 0) {
+    const unsigned others = gridDim.x - 1u;
+    while (__hip_atomic_load(&ctl->small_ack.v, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < others) {
+      if (kernel_aborted(&ctl->bar, deadline)) {
+        s_abort = 1;
+        break;
+      }
+      __builtin_amdgcn_s_sleep(1);
+    }
+    __hip_atomic_store(&ctl->small_ack.v, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (!s_abort)
+      for (int k = 0; k < 3; k++) counters_clear(&ctl->cnt[k]);
+  }
+  __syncthreads();
+  if (s_abort) {
+    *aborted = true;
+    return L;
+  }
   const int cn = s_cnt[cur];
-  int* gout = fq[L & 1].q;
-  for (int i = threadIdx.x; i < cn && i < kSmallCap; i += blockDim.x) gout[i] = s_q[cur][i];
-  *out_n = static_cast<unsigned>(cn);
+  for (int base = 0; base < cn; base += blockDim.x) {
+    const int i = base + static_cast<int>(threadIdx.x);
+    const bool valid = i < cn;
+    const int v = valid ? s_q[cur][i] : 0;
+    frontier_emit(valid, v, valid ? degree_of(a.nindex, v) : 0, a.fq[L & 1], &ctl->cnt[L % 3]);
+  }
   return L;
 }
 
@@ -239,65 +264,51 @@ __global__ __launch_bounds__(kBfsBlock) void bfs_kernel(BfsArgs a)
   const int tid = blockIdx.x * blockDim.x + threadIdx.x;
   const int stride = gridDim.x * blockDim.x;
   for (int i = tid; i < a.nodes; i += stride) a.dist[i] = (i == a.src) ? 0 : kInf;
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
+  if (a.allow_bu) collect_heavy(a.nodes, a.nindex, a.heavy, &ctl->heavy);
+  if (blockIdx.x == 0 && threadIdx.x < kWave) {
     const int deg = degree_of(a.nindex, a.src);
-    FrontierCounters* c0 = &ctl->cnt[0];
-    if (deg >= kHugeDeg) {
-      a.fq[0].hq[0] = a.src;
-      a.fq[0].hoff[0] = 0;
-      c0->hpack = (static_cast<unsigned long long>((deg + kHugeChunk - 1) / kHugeChunk) << 32) | 1ull;
-    } else {
-      a.fq[0].q[0] = a.src;
-      c0->n = 1;
-    }
-    c0->m = static_cast<unsigned long long>(deg);
-    ctl->level_base = 0;
+    frontier_emit(threadIdx.x == 0 && deg > 0, a.src, deg, a.fq[0], &ctl->cnt[0]);
   }
   if (!grid_barrier(&ctl->bar, deadline)) {
     if (tid == 0) ctl->status = 1;
     return;
   }
+  const unsigned heavy_mid = ld_relaxed(&ctl->heavy.nm);
+  const unsigned heavy_huge = huge_count(ld_relaxed(&ctl->heavy.hpack));
   bool bu = false;
   unsigned long long seen_edges = 0;
-  int bu_levels = 0, small_levels = 0;
+  int bu_levels = 0;
   int L = 0;
   for (;;) {
-    FrontierCounters* ic = &ctl->cnt[L % 3];
-    const unsigned n = ld_relaxed(&ic->n);
-    const unsigned long long hp = ld_relaxed(&ic->hpack);
-    const unsigned long long mf = ld_relaxed(&ic->m);
-    const unsigned nh = huge_count(hp), nch = huge_chunks(hp);
-    const unsigned long long nf = static_cast<unsigned long long>(n) + nh;
+    FrontierCounters ic;
+    {
+      FrontierCounters* p = &ctl->cnt[L % 3];
+      ic.n = ld_relaxed(&p->n);
+      ic.nm = ld_relaxed(&p->nm);
+      ic.hpack = ld_relaxed(&p->hpack);
+      ic.m = ld_relaxed(&p->m);
+    }
+    const unsigned long long nf = static_cast<unsigned long long>(ic.n) + ic.nm + huge_count(ic.hpack);
     if (nf == 0) break;
-    seen_edges += mf;
+    seen_edges += ic.m;
     if (a.allow_bu) {
       const unsigned long long e = static_cast<unsigned long long>(a.edges);
       const unsigned long long mu = e > seen_edges ? e - seen_edges : 0ull;
       if (!bu) {
-        if (mf * 14ull > mu) bu = true;
+        if (ic.m * 14ull > mu) bu = true;
       } else if (nf * 24ull < static_cast<unsigned long long>(a.nodes)) {
         bu = false;
       }
     }
-    if (!bu && a.allow_small && nh == 0 && n <= static_cast<unsigned>(kSmallEnter)) {
-      // one-workgroup phase; the other workgroups wait at the next barrier
+    if (!bu && a.allow_small && ic.nm == 0 && ic.hpack == 0 && ic.m <= static_cast<unsigned long long>(kSmallCap)) {
+      // one-workgroup phase; the other workgroups acknowledge that they have
+      // read this level's counters and wait at the next barrier
+      if (blockIdx.x != 0 && threadIdx.x == 0)
+        __hip_atomic_fetch_add(&ctl->small_ack.v, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
       if (blockIdx.x == 0) {
-        unsigned out_n = 0;
         bool aborted = false;
-        const int L2 = small_phase(a, L, n, a.fq, &ctl->bar, deadline, &out_n, &aborted);
+        const int L2 = small_phase(a, L, ic.n, ctl, deadline, &aborted);
         if (threadIdx.x == 0) {
-          if (!aborted) {
-            FrontierCounters* oc = &ctl->cnt[L2 % 3];
-            FrontierCounters* z1 = &ctl->cnt[(L2 + 1) % 3];
-            FrontierCounters* z2 = &ctl->cnt[(L2 + 2) % 3];
-            z1->n = 0; z1->hpack = 0; z1->m = 0;
-            z2->n = 0; z2->hpack = 0; z2->m = 0;
-            oc->n = out_n;
-            oc->hpack = 0;
-            // degree sum of the hand-back frontier is not tracked; a zero keeps
-            // the direction heuristic in top-down mode for the next level
-            oc->m = 0;
-          }
           ctl->level_base = L2;
           ctl->small_levels += L2 - L;
         }
@@ -307,29 +318,22 @@ __global__ __launch_bounds__(kBfsBlock) void bfs_kernel(BfsArgs a)
         return;
       }
       const int L2 = ld_relaxed(&ctl->level_base);
-      small_levels += L2 - L;
-      if (L2 == L) {
-        // no progress possible in the small phase (should not happen)
+      if (L2 == L) {  // cannot happen: the phase always advances at least one level
         if (tid == 0) ctl->status = 2;
         return;
       }
       L = L2;
       continue;
     }
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-      FrontierCounters* z = &ctl->cnt[(L + 2) % 3];
-      z->n = 0;
-      z->hpack = 0;
-      z->m = 0;
-    }
+    if (tid == 0) counters_clear(&ctl->cnt[(L + 2) % 3]);
     const FrontierQueues& in = a.fq[L & 1];
     const FrontierQueues& out = a.fq[(L + 1) & 1];
     FrontierCounters* oc = &ctl->cnt[(L + 1) % 3];
     if (bu) {
-      bottom_up(a, L, out, oc);
+      bottom_up(a, L, heavy_mid, heavy_huge, out, oc);
       bu_levels++;
     } else {
-      top_down(a, L, n, nh, nch, in, out, oc);
+      top_down(a, L, ic, in, out, oc);
     }
     if (!grid_barrier(&ctl->bar, deadline)) {
       if (tid == 0) ctl->status = 1;
@@ -341,7 +345,6 @@ __global__ __launch_bounds__(kBfsBlock) void bfs_kernel(BfsArgs a)
     ctl->levels = L;
     ctl->bottom_up_levels = bu_levels;
   }
-  (void)small_levels;
 }
 
 }  // namespace
@@ -349,7 +352,7 @@ __global__ __launch_bounds__(kBfsBlock) void bfs_kernel(BfsArgs a)
 size_t bfs_workspace_bytes(int nodes)
 {
   const size_t v = static_cast<size_t>(nodes);
-  return Carver::round(sizeof(BfsControl)) + 6 * Carver::round(v * 4) + 4096;
+  return Carver::round(sizeof(BfsControl)) + 12 * Carver::round(v * 4) + 4096;
 }
 
 int bfs_run(indigo_ctx* ctx, int nodes, int edges, const int* nindex, const int* nlist, int src, int* dist,
@@ -360,10 +363,12 @@ int bfs_run(indigo_ctx* ctx, int nodes, int edges, const int* nindex, const int*
   Carver cv(ctx->ws);
   BfsArgs a{};
   a.ctl = cv.take<BfsControl>(1);
-  for (int k = 0; k < 2; k++) {
-    a.fq[k].q = cv.take<int>(nodes);
-    a.fq[k].hq = cv.take<int>(nodes);
-    a.fq[k].hoff = cv.take<unsigned>(nodes);
+  for (int k = 0; k < 3; k++) {
+    FrontierQueues& f = k < 2 ? a.fq[k] : a.heavy;
+    f.q = cv.take<int>(nodes);
+    f.mq = cv.take<int>(nodes);
+    f.hq = cv.take<int>(nodes);
+    f.hoff = cv.take<unsigned>(nodes);
   }
   a.nodes = nodes;
   a.edges = edges;

@@ -13,10 +13,11 @@
 // the same labels, bit for bit.
 //   1. init:    parent[v] = min(v, smallest neighbour) (lists are sorted);
 //   2. link:    every undirected edge once (from its larger endpoint); a wave
-//               takes 64 vertices and walks their CSR entries 64 at a time
-//               with the in-wave balancer; lists of >= kHugeDeg entries are
-//               deferred to 3;
-//   3. huge:    those lists in kHugeChunk-entry items, one workgroup each;
+//               takes 64 vertices in id order and walks the CSR entries of
+//               their short lists (< kMidDeg) 64 at a time with the in-wave
+//               balancer; longer lists go to the mid / huge queues;
+//   3. heavy:   mid lists one wave each, huge lists in kHugeChunk-entry
+//               items, one workgroup each;
 //   4. flatten: label[v] = root(v).
 // Finds use path halving with relaxed stores: every store replaces a parent
 // by one of its ancestors, so concurrent finds and links stay valid.
@@ -31,8 +32,7 @@ namespace {
 constexpr int kCcBlock = 256;
 
 struct CcControl {
-  unsigned long long hpack;  // (chunks << 32) | count of deferred huge lists
-  unsigned long long pad[15];
+  FrontierCounters heavy;  // deferred long lists (mid / huge queues)
 };
 
 IG_DEV int find_root(int* parent, int x)
@@ -80,8 +80,8 @@ __global__ __launch_bounds__(kCcBlock) void cc_init(int nodes, const int* __rest
 }
 
 __global__ __launch_bounds__(kCcBlock) void cc_link(int nodes, const int* __restrict__ nindex,
-                                                    const int* __restrict__ nlist, int* parent, int* hq,
-                                                    unsigned* hoff, CcControl* ctl)
+                                                    const int* __restrict__ nlist, int* parent, FrontierQueues heavy,
+                                                    CcControl* ctl)
 {
   const int lane = lane_id();
   const int wpb = blockDim.x / kWave;
@@ -91,25 +91,9 @@ __global__ __launch_bounds__(kCcBlock) void cc_link(int nodes, const int* __rest
     const bool valid = v < nodes;
     const int beg = valid ? nindex[v] : 0;
     int deg = valid ? nindex[v + 1] - beg : 0;
-    const bool huge = deg >= kHugeDeg;
-    // deferred huge lists: packed append (see frontier.hpp)
-    const unsigned long long hm = __ballot(huge);
-    if (hm != 0) {
-      const int chunks = huge ? (deg + kHugeChunk - 1) / kHugeChunk : 0;
-      const int incl = wave_inclusive_scan(chunks);
-      const int total = wave_shfl(incl, kWave - 1);
-      const int leader = __ffsll(static_cast<long long>(hm)) - 1;
-      unsigned long long old = 0;
-      if (lane == leader)
-        old = atomic_add_relaxed(&ctl->hpack, (static_cast<unsigned long long>(total) << 32) | __popcll(hm));
-      old = wave_shfl(old, leader);
-      if (huge) {
-        const unsigned slot = static_cast<unsigned>(old & 0xffffffffull) + lanes_below(hm);
-        hq[slot] = v;
-        hoff[slot] = static_cast<unsigned>(old >> 32) + static_cast<unsigned>(incl - chunks);
-        deg = 0;
-      }
-    }
+    const bool long_list = deg >= kMidDeg;
+    frontier_emit(long_list, v, deg, heavy, &ctl->heavy);
+    if (long_list) deg = 0;
     const int incl = wave_inclusive_scan(deg);
     const int total = wave_shfl(incl, kWave - 1);
     for (int base = 0; base < total; base += kWave) {
@@ -126,26 +110,19 @@ __global__ __launch_bounds__(kCcBlock) void cc_link(int nodes, const int* __rest
   }
 }
 
-__global__ __launch_bounds__(kCcBlock) void cc_link_huge(const int* __restrict__ nindex,
-                                                         const int* __restrict__ nlist, int* parent,
-                                                         const int* hq, const unsigned* hoff, const CcControl* ctl)
+__global__ __launch_bounds__(kCcBlock) void cc_link_heavy(const int* __restrict__ nindex,
+                                                          const int* __restrict__ nlist, int* parent,
+                                                          FrontierQueues heavy, const CcControl* ctl)
 {
-  __shared__ int s_owner;
-  const unsigned long long hp = ctl->hpack;
-  const unsigned nh = huge_count(hp), nch = huge_chunks(hp);
-  for (unsigned item = blockIdx.x; item < nch; item += gridDim.x) {
-    if (threadIdx.x == 0) s_owner = huge_owner(hoff, static_cast<int>(nh), item);
-    __syncthreads();
-    const int j = s_owner;
-    __syncthreads();
-    const int v = hq[j];
-    const int beg = nindex[v] + static_cast<int>(item - hoff[j]) * kHugeChunk;
-    const int end = min(nindex[v + 1], beg + kHugeChunk);
-    for (int e = beg + static_cast<int>(threadIdx.x); e < end; e += blockDim.x) {
+  const unsigned long long hp = ctl->heavy.hpack;
+  auto visit = [&](int v, int e, bool active) {
+    if (active) {
       const int u = nlist[e];
       if (u < v) link(parent, v, u);
     }
-  }
+  };
+  expand_huge(heavy, huge_count(hp), huge_chunks(hp), nindex, visit);
+  expand_mid(heavy, ctl->heavy.nm, nindex, visit);
 }
 
 __global__ __launch_bounds__(kCcBlock) void cc_flatten(int nodes, int* parent, int* label)
@@ -165,7 +142,7 @@ int grid_for(long long items, int per_block, int cap = 1 << 20)
 size_t cc_workspace_bytes(int nodes)
 {
   const size_t v = static_cast<size_t>(nodes);
-  return Carver::round(sizeof(CcControl)) + 3 * Carver::round(v * 4) + 4096;
+  return Carver::round(sizeof(CcControl)) + 4 * Carver::round(v * 4) + 4096;
 }
 
 int cc_run(indigo_ctx* ctx, int nodes, int edges, const int* nindex, const int* nlist, int* label, int* launches)
@@ -174,14 +151,13 @@ int cc_run(indigo_ctx* ctx, int nodes, int edges, const int* nindex, const int* 
   Carver cv(ctx->ws);
   CcControl* ctl = cv.take<CcControl>(1);
   int* parent = cv.take<int>(nodes);
-  int* hq = cv.take<int>(nodes);
-  unsigned* hoff = cv.take<unsigned>(nodes);
+  FrontierQueues heavy{nullptr, cv.take<int>(nodes), cv.take<int>(nodes), cv.take<unsigned>(nodes)};
   IG_TRY(zero_async(ctx, ctl, sizeof(CcControl)));
   const int gv = grid_for(nodes, kCcBlock, 8192);
   cc_init<<<gv, kCcBlock, 0, ctx->stream>>>(nodes, nindex, nlist, parent);
   const int gl = grid_for((nodes + kWave - 1) / kWave, kCcBlock / kWave, 1 << 16);
-  cc_link<<<gl, kCcBlock, 0, ctx->stream>>>(nodes, nindex, nlist, parent, hq, hoff, ctl);
-  cc_link_huge<<<ctx->compute_units * 4, kCcBlock, 0, ctx->stream>>>(nindex, nlist, parent, hq, hoff, ctl);
+  cc_link<<<gl, kCcBlock, 0, ctx->stream>>>(nodes, nindex, nlist, parent, heavy, ctl);
+  cc_link_heavy<<<ctx->compute_units * 8, kCcBlock, 0, ctx->stream>>>(nindex, nlist, parent, heavy, ctl);
   cc_flatten<<<gv, kCcBlock, 0, ctx->stream>>>(nodes, parent, label);
   IG_HIP_TRY(hipGetLastError());
   if (launches) *launches = 5;

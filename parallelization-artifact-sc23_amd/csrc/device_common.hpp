@@ -155,9 +155,13 @@ constexpr unsigned kAbortNotResident = 2u;  // the grid never assembled: not co-
 IG_DEV bool grid_barrier(GridBarrier* gb, long long deadline, unsigned abort_code = kAbortTimeout)
 {
   __shared__ int s_ok;
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+  // The workgroup barrier completes every wave's stores; thread 0's
+  // agent-scope release then publishes them (one L2 write-back per
+  // workgroup instead of one per wave), and its acquire after the wait
+  // invalidates the caches all waves of the workgroup read through.
   __syncthreads();
   if (threadIdx.x == 0) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
     int ok = 1;
     const unsigned nblocks = gridDim.x;
     const unsigned g = blockIdx.x & 7u;
@@ -192,10 +196,10 @@ IG_DEV bool grid_barrier(GridBarrier* gb, long long deadline, unsigned abort_cod
       }
     }
     if (__hip_atomic_load(&gb->abort.v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u) ok = 0;
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     s_ok = ok;
   }
   __syncthreads();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
   return s_ok != 0;
 }
 

@@ -9,34 +9,35 @@
 // suite's C++ path orders priorities as unsigned int, its CUDA path as signed
 // int (data_type), selected here by `signed_priority`.
 //
-// Design: one cooperative persistent kernel; round r processes a worklist of
-// still-undecided vertices (all vertices in round 0) and appends the ones that
-// stay undecided to the next worklist (wave-aggregated).  Decisions are made
-// in place: a decided state (1 included, 2 excluded) never changes, so a
-// vertex that reads a neighbour's final state may act on it in the same round
-// and a stale "undecided" read only delays it.  A vertex is included only when
-// every neighbour with a larger key is excluded, and excluded as soon as a
-// neighbour is included: exactly the greedy order's fixed point.  Lists of
-// fewer than 64 entries are scanned by one lane; longer ones by the whole
-// wave, 64 entries per step with ballots.  Keys are (priority ^ sign flip,
-// id) compared as one 64-bit unsigned value; the priority is recomputed from
-// the hash instead of being stored.  Self loops are ignored.
+// Design: one cooperative persistent kernel; round r processes the vertices
+// still undecided after round r-1, kept in two worklists by list length:
+// lists shorter than kMidDeg (round 0: every such vertex, in id order) are
+// scanned by one lane each, longer ones (round 0: a static list collected at
+// start) by one wave each, 256 entries per step with ballots.  Vertices that
+// stay undecided are appended (wave-aggregated) to the next round's lists.
+// Decisions are made in place: a decided state (1 included, 2 excluded) never
+// changes, so a vertex that reads a neighbour's final state may act on it in
+// the same round and a stale "undecided" read only delays it.  A vertex is
+// included only when every neighbour with a larger key is excluded, and
+// excluded as soon as a neighbour is included: exactly the greedy order's
+// fixed point.  Keys are (priority ^ sign flip, id) compared as one 64-bit
+// unsigned value; the priority is recomputed from the hash instead of being
+// stored.  Self loops are ignored.
 #include "mis.hpp"
 
-#include "device_common.hpp"
+#include "frontier.hpp"
 
 namespace indigo {
 
 namespace {
 
 constexpr int kMisBlock = 512;
-constexpr int kWaveScan = 64;  // lists at least this long are scanned by a whole wave
 
 constexpr int kUndecided = 0, kIncluded = 1, kExcluded = 2;
 
 struct MisControl {
   GridBarrier bar;
-  PaddedU32 cnt[3];
+  FrontierCounters cnt[3];  // n: short-list worklist, nm: long-list worklist
   int rounds;
   int status;
   int pad[30];
@@ -48,7 +49,8 @@ struct MisArgs {
   const int* __restrict__ nindex;
   const int* __restrict__ nlist;
   int* status;
-  int* wl[2];
+  int* wl[2];   // short-list worklists
+  int* hwl[2];  // long-list worklists (round 0: the static list)
   MisControl* ctl;
   long long budget;
   long long start_ticks;
@@ -82,20 +84,24 @@ IG_DEV void exclude_neighbours_lane(const MisArgs& a, int v, int beg, int end)
   }
 }
 
-// Whole wave decides vertex v (long list); all lanes must be converged.
+// Whole wave decides vertex v (long list), 4 x 64 entries per step; all
+// lanes must be converged.
 IG_DEV int decide_wave(const MisArgs& a, int v, int beg, int end)
 {
   const unsigned long long kv = mis_key(v, a.flip);
   const int lane = lane_id();
-  for (int base = beg; base < end; base += kWave) {
-    const int e = base + lane;
+  for (int base = beg; base < end; base += 4 * kWave) {
     bool inc = false, hi = false;
-    if (e < end) {
-      const int u = a.nlist[e];
-      if (u != v) {
-        const int su = ld_relaxed(&a.status[u]);
-        inc = su == kIncluded;
-        hi = su == kUndecided && mis_key(u, a.flip) > kv;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      const int e = base + k * kWave + lane;
+      if (e < end) {
+        const int u = a.nlist[e];
+        if (u != v) {
+          const int su = ld_relaxed(&a.status[u]);
+          inc |= su == kIncluded;
+          hi |= su == kUndecided && mis_key(u, a.flip) > kv;
+        }
       }
     }
     if (__ballot(inc) != 0) return kExcluded;
@@ -118,8 +124,19 @@ __global__ __launch_bounds__(kMisBlock) void mis_kernel(MisArgs a)
   if (!residency_barrier(&ctl->bar, a.start_ticks)) return;
   const long long deadline = now_ticks() + a.budget;
   const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+  const int stride = gridDim.x * blockDim.x;
   const int lane = lane_id();
-  for (int i = tid; i < a.nodes; i += gridDim.x * blockDim.x) a.status[i] = kUndecided;
+  for (int i = tid; i < a.nodes; i += stride) a.status[i] = kUndecided;
+  // round 0 long-list worklist: every vertex with >= kMidDeg entries
+  {
+    const int wpb = blockDim.x / kWave;
+    const int nw = (a.nodes + kWave - 1) / kWave;
+    for (int w = blockIdx.x * wpb + threadIdx.x / kWave; w < nw; w += gridDim.x * wpb) {
+      const int v = w * kWave + lane;
+      const bool heavy = v < a.nodes && a.nindex[v + 1] - a.nindex[v] >= kMidDeg;
+      wave_append(heavy, v, a.hwl[0], &ctl->cnt[0].nm);
+    }
+  }
   if (!grid_barrier(&ctl->bar, deadline)) {
     if (tid == 0) ctl->status = 1;
     return;
@@ -129,52 +146,53 @@ __global__ __launch_bounds__(kMisBlock) void mis_kernel(MisArgs a)
   const unsigned nwaves = gridDim.x * wpb;
   int r = 0;
   for (;; r++) {
-    const unsigned n = (r == 0) ? static_cast<unsigned>(a.nodes) : ld_relaxed(&ctl->cnt[r % 3].v);
-    if (n == 0) break;
-    if (tid == 0) st_relaxed(&ctl->cnt[(r + 2) % 3].v, 0u);
+    FrontierCounters* ic = &ctl->cnt[r % 3];
+    const unsigned n = (r == 0) ? static_cast<unsigned>(a.nodes) : ld_relaxed(&ic->n);
+    const unsigned nh = ld_relaxed(&ic->nm);
+    if (n == 0 && nh == 0) break;
+    if (tid == 0) counters_clear(&ctl->cnt[(r + 2) % 3]);
+    FrontierCounters* oc = &ctl->cnt[(r + 1) % 3];
     const int* in = a.wl[r & 1];
     int* out = a.wl[(r + 1) & 1];
-    unsigned* oc = &ctl->cnt[(r + 1) % 3].v;
-    for (unsigned c = gwave; c * kWave < n; c += nwaves) {
-      const unsigned idx = c * kWave + lane;
-      const bool valid = idx < n;
-      const int v = valid ? (r == 0 ? static_cast<int>(idx) : in[idx]) : 0;
-      int state = kExcluded;  // "nothing to do" for invalid lanes
-      int beg = 0, end = 0;
-      if (valid) {
-        state = ld_relaxed(&a.status[v]);
-        if (state == kUndecided) {
-          beg = a.nindex[v];
-          end = a.nindex[v + 1];
-          if (end - beg < kWaveScan) {
-            state = decide_lane(a, v, beg, end);
-            if (state == kIncluded) {
+    const int* hin = a.hwl[r & 1];
+    int* hout = a.hwl[(r + 1) & 1];
+    // short lists: one lane each
+    for (unsigned base = blockIdx.x * blockDim.x; base < n; base += stride) {
+      const unsigned idx = base + threadIdx.x;
+      bool keep = false;
+      int v = 0;
+      if (idx < n) {
+        v = (r == 0) ? static_cast<int>(idx) : in[idx];
+        if (ld_relaxed(&a.status[v]) == kUndecided) {
+          const int beg = a.nindex[v], end = a.nindex[v + 1];
+          if (end - beg < kMidDeg) {
+            const int st = decide_lane(a, v, beg, end);
+            if (st == kIncluded) {
               st_relaxed(&a.status[v], kIncluded);
               exclude_neighbours_lane(a, v, beg, end);
-            } else if (state == kExcluded) {
+            } else if (st == kExcluded) {
               st_relaxed(&a.status[v], kExcluded);
+            } else {
+              keep = true;
             }
           }
         }
       }
-      // long lists: the whole wave handles them one vertex at a time
-      unsigned long long big = __ballot(valid && state == kUndecided && end - beg >= kWaveScan);
-      while (big != 0) {
-        const int l = __ffsll(static_cast<long long>(big)) - 1;
-        big &= big - 1;
-        const int bv = wave_shfl(v, l);
-        const int bb = wave_shfl(beg, l);
-        const int be = wave_shfl(end, l);
-        const int s = decide_wave(a, bv, bb, be);
-        if (s == kIncluded) {
-          if (lane == 0) st_relaxed(&a.status[bv], kIncluded);
-          exclude_neighbours_wave(a, bv, bb, be);
-        } else if (s == kExcluded) {
-          if (lane == 0) st_relaxed(&a.status[bv], kExcluded);
-        }
-        if (lane == l) state = s;
+      wave_append(keep, v, out, &oc->n);
+    }
+    // long lists: one wave each
+    for (unsigned i = gwave; i < nh; i += nwaves) {
+      const int v = hin[i];
+      if (ld_relaxed(&a.status[v]) != kUndecided) continue;
+      const int beg = a.nindex[v], end = a.nindex[v + 1];
+      const int st = decide_wave(a, v, beg, end);
+      if (st == kIncluded) {
+        if (lane == 0) st_relaxed(&a.status[v], kIncluded);
+        exclude_neighbours_wave(a, v, beg, end);
+      } else if (st == kExcluded) {
+        if (lane == 0) st_relaxed(&a.status[v], kExcluded);
       }
-      wave_append(valid && state == kUndecided, v, out, oc);
+      wave_append(st == kUndecided && lane == 0, v, hout, &oc->nm);
     }
     if (!grid_barrier(&ctl->bar, deadline)) {
       if (tid == 0) ctl->status = 1;
@@ -188,7 +206,7 @@ __global__ __launch_bounds__(kMisBlock) void mis_kernel(MisArgs a)
 
 size_t mis_workspace_bytes(int nodes)
 {
-  return Carver::round(sizeof(MisControl)) + 2 * Carver::round(static_cast<size_t>(nodes) * 4) + 4096;
+  return Carver::round(sizeof(MisControl)) + 4 * Carver::round(static_cast<size_t>(nodes) * 4) + 4096;
 }
 
 int mis_run(indigo_ctx* ctx, int nodes, const int* nindex, const int* nlist, int signed_priority, int* status)
@@ -198,6 +216,8 @@ int mis_run(indigo_ctx* ctx, int nodes, const int* nindex, const int* nlist, int
   a.ctl = cv.take<MisControl>(1);
   a.wl[0] = cv.take<int>(nodes);
   a.wl[1] = cv.take<int>(nodes);
+  a.hwl[0] = cv.take<int>(nodes);
+  a.hwl[1] = cv.take<int>(nodes);
   a.nodes = nodes;
   a.flip = signed_priority ? 0x80000000u : 0u;
   a.nindex = nindex;

@@ -6,21 +6,25 @@
 // s'[v] = (1 - 0.85) / N + 0.85 * sum_{u in N(v)} contrib[u]; repeat while the
 // L1 change of the iteration is > 1e-4 and fewer than 100 iterations ran.
 //
-// Design: one cooperative persistent kernel, one grid barrier per iteration.
-//   * contrib is double-buffered and written by the vertex that owns it in
-//     the same pass that computes its score (no separate contrib sweep);
-//   * per-vertex sums are FP32 with Neumaier compensation (the improved
-//     Kahan-Babuska sum: a running error term added once at the end); lists
-//     shorter than kPrWaveDeg are summed by one lane, longer ones by a whole
-//     wave (per-lane compensated partials combined pairwise with the same
-//     error-free step), and lists of kPrHugeDeg or more in kPrChunk-entry
-//     items over all workgroups, whose partials the last-arriving workgroup
-//     combines in chunk order (deterministic);
-//   * |s' - s| is exact in FP64 and summed in FP64, the only FP64 work: each
-//     workgroup stores its partial in a slot (slots double-buffered by
-//     iteration parity); after the barrier every workgroup adds all slots in
-//     the same fixed order, so all of them compute the bit-identical total and
-//     take the same stop decision (no floating-point atomics).
+// Design: one cooperative persistent kernel; per iteration
+//   phase A  every vertex with a short list (< kPrWaveDeg entries) is summed
+//            by one lane in vertex order (coalesced score / contrib / index
+//            traffic); longer lists, collected once into a static list, one
+//            wave each; lists of kHugeDeg or more in kHugeChunk-entry items,
+//            one workgroup each, whose partial sums go to a per-item slot;
+//   phase B  (only if there are huge lists, after a barrier) one wave per
+//            huge vertex combines its item partials in a fixed tree order;
+// then a barrier and the stop test.  Per-vertex sums are FP32 with Neumaier
+// compensation (the improved Kahan-Babuska sum: a running error term added
+// once at the end); wave and workgroup partials are combined pairwise with the
+// same error-free step.  contrib is double-buffered and written by the vertex
+// that owns it in the pass that computes its score (Jacobi, as the
+// reference's Determ variants), so no separate contrib sweep runs.
+// |s' - s| is exact in FP64 and summed in FP64, the only FP64 work: each
+// workgroup stores its partials in slots (double-buffered by iteration
+// parity); after the barrier every workgroup adds all slots in one fixed
+// order, so all compute the bit-identical total and take the same stop
+// decision (no floating-point atomics).
 #include "pr.hpp"
 
 #include <cmath>
@@ -36,16 +40,14 @@ namespace indigo {
 namespace {
 
 constexpr int kPrBlock = 512;
-constexpr int kPrWaveDeg = 32;
-constexpr int kPrHugeDeg = 8192;
-constexpr int kPrChunk = 4096;
+constexpr int kPrWaveDeg = 16;
 constexpr float kDamp = 0.85f;
 
 constexpr int kPrMaxGrid = 4096;
 
 struct PrControl {
   GridBarrier bar;
-  unsigned long long hpack;
+  FrontierCounters heavy;
   int iterations;
   int status;
   double last_diff;
@@ -63,11 +65,9 @@ struct PrArgs {
   const int* __restrict__ degree;  // optional caller-provided degrees (PR_GPU contract)
   float* scores;
   float* contrib[2];
-  int* hq;
-  unsigned* hoff;
-  unsigned* hdone;
-  float2* hpart;
-  double* partial;  // [2][kPrMaxGrid] per-workgroup residuals
+  FrontierQueues heavy;  // lists of >= kPrWaveDeg entries (mid) and >= kHugeDeg (huge)
+  float2* hpart;         // per huge item: compensated partial (sum, error)
+  double* partial;       // [2][2 * kPrMaxGrid] per-workgroup residuals (phase A, phase B)
   int* exit_it;     // [kPrMaxGrid] iteration at which each workgroup left
   double* exit_total;
   PrControl* ctl;
@@ -119,6 +119,37 @@ IG_DEV double finish_vertex(const PrArgs& a, int v, float total, float base, flo
   return fabs(static_cast<double>(s) - static_cast<double>(old));
 }
 
+// Workgroup-wide compensated sum of one partial per thread; result valid in
+// thread 0.
+IG_DEV CSum block_merge(CSum acc, float* s_ps, float* s_pc)
+{
+  acc = wave_merge(acc);
+  const int wid = threadIdx.x / kWave;
+  if (lane_id() == 0) {
+    s_ps[wid] = acc.s;
+    s_pc[wid] = acc.c;
+  }
+  __syncthreads();
+  CSum blk;
+  if (threadIdx.x == 0)
+    for (int w = 0; w < kPrBlock / kWave; w++) blk.merge(s_ps[w], s_pc[w]);
+  __syncthreads();
+  return blk;
+}
+
+// FP64 sum of one value per thread; result valid in every thread.
+IG_DEV double block_sum_f64(double x, double* s_red)
+{
+  x = wave_sum(x);
+  const int wid = threadIdx.x / kWave;
+  if (lane_id() == 0) s_red[wid] = x;
+  __syncthreads();
+  double t = 0.0;
+  for (int w = 0; w < kPrBlock / kWave; w++) t += s_red[w];
+  __syncthreads();
+  return t;
+}
+
 __global__ __launch_bounds__(kPrBlock) void pr_kernel(PrArgs a)
 {
   __shared__ double s_red[kPrBlock / kWave];
@@ -128,145 +159,102 @@ __global__ __launch_bounds__(kPrBlock) void pr_kernel(PrArgs a)
   if (!residency_barrier(&ctl->bar, a.start_ticks)) return;
   const long long deadline = now_ticks() + a.budget;
   const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+  const int stride = gridDim.x * blockDim.x;
   const int lane = lane_id();
-  const int wid = threadIdx.x / kWave;
+  const int wpb = kPrBlock / kWave;
   const float inv_n = 1.0f / static_cast<float>(a.nodes);
   const float base = (1.0f - kDamp) / static_cast<float>(a.nodes);
 
-  // init scores / first contrib, collect huge lists
-  const int nwv = (a.nodes + kWave - 1) / kWave;
-  for (int c = blockIdx.x * (kPrBlock / kWave) + wid; c < nwv; c += gridDim.x * (kPrBlock / kWave)) {
-    const int v = c * kWave + lane;
-    const bool valid = v < a.nodes;
-    int deg = 0;
-    if (valid) {
-      deg = deg_of(a, v);
-      const float s = a.init_uniform ? inv_n : a.scores[v];
-      a.scores[v] = s;
-      a.contrib[0][v] = s / static_cast<float>(deg);
-    }
-    const bool huge = valid && a.nindex[v + 1] - a.nindex[v] >= kPrHugeDeg;
-    const unsigned long long hm = __ballot(huge);
-    if (hm != 0) {
-      const int len = huge ? a.nindex[v + 1] - a.nindex[v] : 0;
-      const int chunks = huge ? (len + kPrChunk - 1) / kPrChunk : 0;
-      const int incl = wave_inclusive_scan(chunks);
-      const int total = wave_shfl(incl, kWave - 1);
-      const int leader = __ffsll(static_cast<long long>(hm)) - 1;
-      unsigned long long old = 0;
-      if (lane == leader)
-        old = atomic_add_relaxed(&ctl->hpack, (static_cast<unsigned long long>(total) << 32) | __popcll(hm));
-      old = wave_shfl(old, leader);
-      if (huge) {
-        const unsigned slot = static_cast<unsigned>(old & 0xffffffffull) + lanes_below(hm);
-        a.hq[slot] = v;
-        a.hoff[slot] = static_cast<unsigned>(old >> 32) + static_cast<unsigned>(incl - chunks);
-        a.hdone[slot] = 0;
-      }
-    }
+  for (int v = tid; v < a.nodes; v += stride) {
+    const float s = a.init_uniform ? inv_n : a.scores[v];
+    a.scores[v] = s;
+    a.contrib[0][v] = s / static_cast<float>(deg_of(a, v));
   }
+  collect_heavy<kPrWaveDeg>(a.nodes, a.nindex, a.heavy, &ctl->heavy);
   if (!grid_barrier(&ctl->bar, deadline)) {
     if (tid == 0) ctl->status = 1;
     if (threadIdx.x == 0) a.exit_it[blockIdx.x] = -1000000;
     return;
   }
-  const unsigned long long hp = ld_relaxed(&ctl->hpack);
-  const int nh = static_cast<int>(huge_count(hp));
-  const unsigned nch = huge_chunks(hp);
+  const unsigned nm = ld_relaxed(&ctl->heavy.nm);
+  const unsigned long long hp = ld_relaxed(&ctl->heavy.hpack);
+  const unsigned nh = huge_count(hp), nch = huge_chunks(hp);
 
   int it = 0;
   for (;;) {
     it++;
     const float* cin = a.contrib[(it - 1) & 1];
     float* cout = a.contrib[it & 1];
+    double* slots = a.partial + (it & 1) * (2 * kPrMaxGrid);
     double d = 0.0;
 
-    // huge lists: one workgroup per chunk; the last one to finish a list
-    // combines its chunk partials in order and finishes the vertex
+    // phase A: huge items -> partials
     for (unsigned item = blockIdx.x; item < nch; item += gridDim.x) {
-      if (threadIdx.x == 0) s_owner = huge_owner(a.hoff, nh, item);
+      if (threadIdx.x == 0) s_owner = huge_owner(a.heavy.hoff, static_cast<int>(nh), item);
       __syncthreads();
       const int j = s_owner;
-      const int v = a.hq[j];
-      const int vbeg = a.nindex[v], vend = a.nindex[v + 1];
-      const int beg = vbeg + static_cast<int>(item - a.hoff[j]) * kPrChunk;
-      const int end = min(vend, beg + kPrChunk);
+      const int v = a.heavy.hq[j];
+      const int beg = a.nindex[v] + static_cast<int>(item - a.heavy.hoff[j]) * kHugeChunk;
+      const int end = min(a.nindex[v + 1], beg + kHugeChunk);
       CSum acc;
       for (int e = beg + static_cast<int>(threadIdx.x); e < end; e += kPrBlock) acc.add(cin[a.nlist[e]]);
+      const CSum blk = block_merge(acc, s_ps, s_pc);
+      if (threadIdx.x == 0) a.hpart[item] = make_float2(blk.s, blk.c);
+    }
+    // phase A: mid lists, one wave each
+    for (unsigned i = blockIdx.x * wpb + threadIdx.x / kWave; i < nm; i += gridDim.x * wpb) {
+      const int v = a.heavy.mq[i];
+      const int beg = a.nindex[v], end = a.nindex[v + 1];
+      CSum acc;
+      for (int e = beg + lane; e < end; e += kWave) acc.add(cin[a.nlist[e]]);
       acc = wave_merge(acc);
-      if (lane == 0) {
-        s_ps[wid] = acc.s;
-        s_pc[wid] = acc.c;
+      if (lane == 0) d += finish_vertex(a, v, acc.value(), base, cout);
+    }
+    // phase A: short lists, one lane each, vertex order
+    for (int v = tid; v < a.nodes; v += stride) {
+      const int beg = a.nindex[v], end = a.nindex[v + 1];
+      if (end - beg >= kPrWaveDeg) continue;
+      CSum acc;
+      for (int e = beg; e < end; e++) acc.add(cin[a.nlist[e]]);
+      d += finish_vertex(a, v, acc.value(), base, cout);
+    }
+    d = block_sum_f64(d, s_red);
+    if (threadIdx.x == 0) slots[2 * blockIdx.x] = d;
+    d = 0.0;
+
+    if (nh > 0) {
+      if (!grid_barrier(&ctl->bar, deadline)) {
+        if (tid == 0) ctl->status = 1;
+        if (threadIdx.x == 0) a.exit_it[blockIdx.x] = -it;
+        return;
       }
-      __syncthreads();
-      if (threadIdx.x == 0) {
-        CSum blk;
-        for (int w = 0; w < kPrBlock / kWave; w++) blk.merge(s_ps[w], s_pc[w]);
-        a.hpart[item] = make_float2(blk.s, blk.c);
-        const unsigned chunks = static_cast<unsigned>((vend - vbeg + kPrChunk - 1) / kPrChunk);
-        const unsigned prev = __hip_atomic_fetch_add(&a.hdone[j], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        if (prev + 1u == chunks * static_cast<unsigned>(it)) {
-          CSum all;
-          // the acquire above orders these loads after every chunk's release
-          for (unsigned k = 0; k < chunks; k++) {
-            const float2 p = a.hpart[a.hoff[j] + k];
-            all.merge(p.x, p.y);
-          }
-          d += finish_vertex(a, v, all.value(), base, cout);
+      // phase B: one wave per huge vertex combines its item partials
+      for (unsigned j = blockIdx.x * wpb + threadIdx.x / kWave; j < nh; j += gridDim.x * wpb) {
+        const int v = a.heavy.hq[j];
+        const int len = a.nindex[v + 1] - a.nindex[v];
+        const unsigned first = a.heavy.hoff[j];
+        const int items = (len + kHugeChunk - 1) / kHugeChunk;
+        CSum acc;
+        for (int k = lane; k < items; k += kWave) {
+          const float2 p = a.hpart[first + k];
+          acc.merge(p.x, p.y);
         }
-      }
-      __syncthreads();
-    }
-
-    // all other vertices: 64 per wave
-    for (int c = blockIdx.x * (kPrBlock / kWave) + wid; c < nwv; c += gridDim.x * (kPrBlock / kWave)) {
-      const int v = c * kWave + lane;
-      const bool valid = v < a.nodes;
-      const int beg = valid ? a.nindex[v] : 0;
-      const int len = valid ? a.nindex[v + 1] - beg : 0;
-      if (valid && len < kPrWaveDeg) {
-        CSum acc;
-        for (int e = beg; e < beg + len; e++) acc.add(cin[a.nlist[e]]);
-        d += finish_vertex(a, v, acc.value(), base, cout);
-      }
-      unsigned long long wide = __ballot(valid && len >= kPrWaveDeg && len < kPrHugeDeg);
-      while (wide != 0) {
-        const int l = __ffsll(static_cast<long long>(wide)) - 1;
-        wide &= wide - 1;
-        const int wv = wave_shfl(v, l);
-        const int wb = wave_shfl(beg, l);
-        const int we = wb + wave_shfl(len, l);
-        CSum acc;
-        for (int e = wb + lane; e < we; e += kWave) acc.add(cin[a.nlist[e]]);
         acc = wave_merge(acc);
-        if (lane == l) d += finish_vertex(a, wv, acc.value(), base, cout);
+        if (lane == 0) d += finish_vertex(a, v, acc.value(), base, cout);
       }
     }
+    d = block_sum_f64(d, s_red);
+    if (threadIdx.x == 0) slots[2 * blockIdx.x + 1] = d;
 
-    // FP64 residual: this workgroup's partial into its slot ...
-    double* slots = a.partial + (it & 1) * kPrMaxGrid;
-    d = wave_sum(d);
-    if (lane == 0) s_red[wid] = d;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      double t = 0.0;
-      for (int w = 0; w < kPrBlock / kWave; w++) t += s_red[w];
-      slots[blockIdx.x] = t;
-    }
     if (!grid_barrier(&ctl->bar, deadline)) {
       if (tid == 0) ctl->status = 1;
       if (threadIdx.x == 0) a.exit_it[blockIdx.x] = -it;
       return;
     }
-    // ... and every workgroup adds all slots in one fixed order
+    // every workgroup adds all slots in one fixed order
     double part = 0.0;
-    for (int b = threadIdx.x; b < static_cast<int>(gridDim.x); b += kPrBlock) part += slots[b];
-    part = wave_sum(part);
-    if (lane == 0) s_red[wid] = part;
-    __syncthreads();
-    double total = 0.0;
-    for (int w = 0; w < kPrBlock / kWave; w++) total += s_red[w];
-    __syncthreads();
+    for (int b = threadIdx.x; b < 2 * static_cast<int>(gridDim.x); b += kPrBlock) part += slots[b];
+    const double total = block_sum_f64(part, s_red);
     if (!(total > a.eps && it < a.max_iter)) {
       if (tid == 0) {
         ctl->iterations = it;
@@ -284,15 +272,18 @@ __global__ __launch_bounds__(kPrBlock) void pr_kernel(PrArgs a)
 
 }  // namespace
 
-// Upper bound on the number of kPrChunk items: every huge list has at least
-// kPrHugeDeg entries, so items <= edges / kPrChunk + edges / kPrHugeDeg.
-static size_t pr_max_chunks(int edges) { return static_cast<size_t>(edges) / kPrChunk * 2 + 2; }
+// Upper bound on the number of huge items: every huge list has at least
+// kHugeDeg entries, so items <= edges / kHugeChunk + edges / kHugeDeg.
+static size_t pr_max_chunks(int edges)
+{
+  return static_cast<size_t>(edges) / kHugeChunk + static_cast<size_t>(edges) / kHugeDeg + 2;
+}
 
 size_t pr_workspace_bytes(int nodes, int edges)
 {
   const size_t v = static_cast<size_t>(nodes);
-  return Carver::round(sizeof(PrControl)) + 5 * Carver::round(v * 4) + Carver::round(pr_max_chunks(edges) * 8) +
-         Carver::round(2 * kPrMaxGrid * 8) + Carver::round(kPrMaxGrid * 4) + Carver::round(kPrMaxGrid * 8) + 4096;
+  return Carver::round(sizeof(PrControl)) + 6 * Carver::round(v * 4) + Carver::round(pr_max_chunks(edges) * 8) +
+         Carver::round(4 * kPrMaxGrid * 8) + Carver::round(kPrMaxGrid * 4) + Carver::round(kPrMaxGrid * 8) + 4096;
 }
 
 int pr_run(indigo_ctx* ctx, int nodes, int edges, const int* nindex, const int* nlist, const int* degree,
@@ -303,11 +294,12 @@ int pr_run(indigo_ctx* ctx, int nodes, int edges, const int* nindex, const int* 
   a.ctl = cv.take<PrControl>(1);
   a.contrib[0] = cv.take<float>(nodes);
   a.contrib[1] = cv.take<float>(nodes);
-  a.hq = cv.take<int>(nodes);
-  a.hoff = cv.take<unsigned>(nodes);
-  a.hdone = cv.take<unsigned>(nodes);
+  a.heavy.q = nullptr;
+  a.heavy.mq = cv.take<int>(nodes);
+  a.heavy.hq = cv.take<int>(nodes);
+  a.heavy.hoff = cv.take<unsigned>(nodes);
   a.hpart = cv.take<float2>(pr_max_chunks(edges));
-  a.partial = cv.take<double>(2 * kPrMaxGrid);
+  a.partial = cv.take<double>(4 * kPrMaxGrid);
   a.exit_it = cv.take<int>(kPrMaxGrid);
   a.exit_total = cv.take<double>(kPrMaxGrid);
   ctx->diag[0] = a.exit_it;

@@ -68,49 +68,39 @@ struct SsspArgs {
 
 IG_DEV int degree_of(const int* __restrict__ nindex, int u) { return nindex[u + 1] - nindex[u]; }
 
-// Relaxation result classes.
-struct Relax {
-  bool near;  // joins the next near frontier
-  int u;
-  int udeg;
-};
-
-// Relaxes edge (dv + w -> u).  Near improvements are deduplicated by stamp;
-// far ones are parked directly (wave-aggregated by the caller).
-IG_DEV Relax relax(const SsspArgs& a, int dv, int e, int T, int it_stamp, int epoch, int* far, unsigned* far_cnt,
-                   unsigned* far_min, bool active)
+// Relaxes one CSR entry e of v (distance dv) when `active`: an improvement
+// below the split T joins the next near frontier (deduplicated per iteration
+// by stamp), one at or above T is parked in the far pile (deduplicated per
+// epoch by farstamp).  Wave-collective: all lanes converged.
+IG_DEV void relax(const SsspArgs& a, int dv, int e, bool active, int T, int it_stamp, int epoch, int* far,
+                  unsigned* far_cnt, unsigned* far_min, const FrontierQueues& out, FrontierCounters* oc)
 {
-  Relax r{false, 0, 0};
-  bool park = false;
-  int u = 0, nd = 0;
+  bool near = false, park = false;
+  int u = 0, nd = 0, udeg = 0;
   if (active) {
     u = a.nlist[e];
     nd = dv + a.eweight[e];
     if (nd < a.dist[u]) {
       const int old = atomic_min_relaxed(&a.dist[u], nd);
       if (nd < old) {
-        if (nd < T) {
-          if (__hip_atomic_exchange(&a.stamp[u], it_stamp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != it_stamp) {
-            r.near = true;
-            r.u = u;
-            r.udeg = degree_of(a.nindex, u);
-          }
-        } else if (__hip_atomic_exchange(&a.farstamp[u], epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != epoch) {
-          park = true;
+        udeg = degree_of(a.nindex, u);
+        if (udeg > 0) {
+          if (nd < T)
+            near = __hip_atomic_exchange(&a.stamp[u], it_stamp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != it_stamp;
+          else
+            park = __hip_atomic_exchange(&a.farstamp[u], epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != epoch;
         }
       }
     }
   }
-  const int slot = wave_append(park, u, far, far_cnt);
-  (void)slot;
+  frontier_emit(near, u, udeg, out, oc);
+  wave_append(park, u, far, far_cnt);
   const int m = wave_min(park ? nd : kInf);
   if (lane_id() == 0 && m != kInf) atomic_min_relaxed(far_min, static_cast<unsigned>(m));
-  return r;
 }
 
 __global__ __launch_bounds__(kSsspBlock) void sssp_kernel(SsspArgs a)
 {
-  __shared__ int s_owner;
   SsspControl* ctl = a.ctl;
   if (!residency_barrier(&ctl->bar, a.start_ticks)) return;
   const long long deadline = now_ticks() + a.budget;
@@ -158,17 +148,12 @@ __global__ __launch_bounds__(kSsspBlock) void sssp_kernel(SsspArgs a)
         delta = d < 1.0 ? 1 : (d > 1e9 ? 1000000000 : static_cast<int>(d));
       }
       ctl->delta = delta;
-      const int deg = degree_of(a.nindex, a.src);
-      FrontierCounters* c0 = &ctl->cnt[0];
-      if (deg >= kHugeDeg) {
-        a.fq[0].hq[0] = a.src;
-        a.fq[0].hoff[0] = 0;
-        c0->hpack = (static_cast<unsigned long long>((deg + kHugeChunk - 1) / kHugeChunk) << 32) | 1ull;
-      } else {
-        a.fq[0].q[0] = a.src;
-        c0->n = 1;
-      }
       for (int k = 0; k < 3; k++) ctl->far_min[k].v = static_cast<unsigned>(kInf);
+    }
+    __syncthreads();
+    if (threadIdx.x < kWave) {
+      const int deg = degree_of(a.nindex, a.src);
+      frontier_emit(threadIdx.x == 0 && deg > 0, a.src, deg, a.fq[0], &ctl->cnt[0]);
     }
   }
   if (!grid_barrier(&ctl->bar, deadline)) {
@@ -186,21 +171,17 @@ __global__ __launch_bounds__(kSsspBlock) void sssp_kernel(SsspArgs a)
   for (;;) {
     FrontierCounters* ic = &ctl->cnt[L % 3];
     const unsigned n = ld_relaxed(&ic->n);
+    const unsigned nm = ld_relaxed(&ic->nm);
     const unsigned long long hp = ld_relaxed(&ic->hpack);
     const unsigned nh = huge_count(hp), nch = huge_chunks(hp);
     FrontierCounters* oc = &ctl->cnt[(L + 1) % 3];
     const FrontierQueues& in = a.fq[L & 1];
     const FrontierQueues& out = a.fq[(L + 1) & 1];
-    if (tid == 0) {
-      FrontierCounters* z = &ctl->cnt[(L + 2) % 3];
-      z->n = 0;
-      z->hpack = 0;
-      z->m = 0;
-    }
+    if (tid == 0) counters_clear(&ctl->cnt[(L + 2) % 3]);
     int* far = a.far[epoch % 3];
     unsigned* far_cnt = &ctl->far_cnt[epoch % 3].v;
     unsigned* far_min = &ctl->far_min[epoch % 3].v;
-    if (static_cast<unsigned long long>(n) + nh == 0) {
+    if (static_cast<unsigned long long>(n) + nm + nh == 0) {
       // near frontier empty: move the split and re-file the far pile
       const unsigned nf = ld_relaxed(far_cnt);
       if (nf == 0) break;
@@ -243,42 +224,13 @@ __global__ __launch_bounds__(kSsspBlock) void sssp_kernel(SsspArgs a)
       epoch = nepoch;
     } else {
       it++;
-      // huge lists of the near frontier
-      for (unsigned item = blockIdx.x; item < nch; item += gridDim.x) {
-        if (threadIdx.x == 0) s_owner = huge_owner(in.hoff, static_cast<int>(nh), item);
-        __syncthreads();
-        const int j = s_owner;
-        __syncthreads();
-        const int v = in.hq[j];
-        const int dv = a.dist[v];
-        const int beg = a.nindex[v] + static_cast<int>(item - in.hoff[j]) * kHugeChunk;
-        const int end = min(a.nindex[v + 1], beg + kHugeChunk);
-        for (int base = beg; base < end; base += blockDim.x) {
-          const int e = base + static_cast<int>(threadIdx.x);
-          const Relax r = relax(a, dv, e, T, it, epoch, far, far_cnt, far_min, e < end);
-          frontier_emit(r.near, r.u, r.udeg, out, oc);
-        }
-      }
-      // normal near frontier, 64 vertices per wave
-      for (unsigned c = gwave; c * kWave < n; c += nwaves) {
-        const unsigned idx = c * kWave + lane;
-        const bool valid = idx < n;
-        const int v = valid ? in.q[idx] : 0;
-        const int beg = valid ? a.nindex[v] : 0;
-        const int deg = valid ? a.nindex[v + 1] - beg : 0;
-        const int dv = valid ? a.dist[v] : 0;
-        const int incl = wave_inclusive_scan(deg);
-        const int total = wave_shfl(incl, kWave - 1);
-        for (int base = 0; base < total; base += kWave) {
-          const int k = base + lane;
-          const int o = min(wave_owner(incl, k), kWave - 1);
-          const int obeg = wave_shfl(beg, o);
-          const int oexcl = wave_shfl(incl - deg, o);
-          const int odv = wave_shfl(dv, o);
-          const Relax r = relax(a, odv, obeg + (k - oexcl), T, it, epoch, far, far_cnt, far_min, k < total);
-          frontier_emit(r.near, r.u, r.udeg, out, oc);
-        }
-      }
+      auto visit = [&](int v, int e, bool active) {
+        const int dv = active ? a.dist[v] : 0;
+        relax(a, dv, e, active, T, it, epoch, far, far_cnt, far_min, out, oc);
+      };
+      expand_huge(in, nh, nch, a.nindex, visit);
+      expand_mid(in, nm, a.nindex, visit);
+      expand_small(in.q, n, a.nindex, visit);
     }
     if (!grid_barrier(&ctl->bar, deadline)) {
       if (tid == 0) ctl->status = 1;
@@ -294,7 +246,7 @@ __global__ __launch_bounds__(kSsspBlock) void sssp_kernel(SsspArgs a)
 size_t sssp_workspace_bytes(int nodes)
 {
   const size_t v = static_cast<size_t>(nodes);
-  return Carver::round(sizeof(SsspControl)) + 11 * Carver::round(v * 4) + 4096;
+  return Carver::round(sizeof(SsspControl)) + 13 * Carver::round(v * 4) + 4096;
 }
 
 int sssp_run(indigo_ctx* ctx, int nodes, int edges, const int* nindex, const int* nlist, const int* eweight, int src,
@@ -305,6 +257,7 @@ int sssp_run(indigo_ctx* ctx, int nodes, int edges, const int* nindex, const int
   a.ctl = cv.take<SsspControl>(1);
   for (int k = 0; k < 2; k++) {
     a.fq[k].q = cv.take<int>(nodes);
+    a.fq[k].mq = cv.take<int>(nodes);
     a.fq[k].hq = cv.take<int>(nodes);
     a.fq[k].hoff = cv.take<unsigned>(nodes);
   }

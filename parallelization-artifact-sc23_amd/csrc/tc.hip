@@ -10,10 +10,11 @@
 // large inputs).
 //
 // Passes (plain launches on the context stream):
-//   1. count  : out-degree under the orientation per vertex; lists < 64 by one
-//               lane, < kHugeDeg by a wave (ballot), longer ones in
-//               kHugeChunk-entry items (per-chunk counts kept); also flags
-//               unsorted lists and self loops;
+//   1. count  : out-degree under the orientation per vertex; lists < kMidDeg
+//               by one lane in id order, longer ones binned into mid (one
+//               wave each, ballots) and huge (kHugeChunk-entry items, one
+//               workgroup each, per-item counts kept); also flags unsorted
+//               lists and self loops;
 //   2. scan   : exclusive prefix -> oindex;
 //   3. fill   : oriented lists, order-preserving (so still sorted), same
 //               thread / wave / chunk split as 1;
@@ -31,39 +32,38 @@ namespace indigo {
 namespace {
 
 constexpr int kTcBlock = 256;
-constexpr int kTcWaveDeg = 64;
 constexpr int kStage = 1024;  // entries of N+(v) staged in LDS per wave
 constexpr int kGrab = 8;      // vertices per dynamic grab
 
 struct TcControl {
-  unsigned long long hpack;
+  FrontierCounters heavy;  // lists of >= kMidDeg entries
   unsigned long long count;
   unsigned next;
   int error;
-  int pad[26];
+  int pad[28];
 };
 
 IG_DEV bool outranks(int du, int u, int dv, int v) { return du > dv || (du == dv && u > v); }
 
 IG_DEV int deg(const int* __restrict__ nindex, int v) { return nindex[v + 1] - nindex[v]; }
 
-// Pass 1 / 3 for lists shorter than kHugeDeg (kFill selects the pass).
+// Pass 1 / 3 for short lists, one lane per vertex in id order (kFill selects
+// the pass); the count pass also bins the long lists.
 template <bool kFill>
 __global__ __launch_bounds__(kTcBlock) void tc_orient(int nodes, const int* __restrict__ nindex,
                                                       const int* __restrict__ nlist, int* ocount,
-                                                      const int* __restrict__ oindex, int* olist, int* hq,
-                                                      unsigned* hoff, TcControl* ctl)
+                                                      const int* __restrict__ oindex, int* olist, FrontierQueues heavy,
+                                                      TcControl* ctl)
 {
-  const int lane = lane_id();
   const int wpb = kTcBlock / kWave;
   const int nchunks = (nodes + kWave - 1) / kWave;
   for (int c = blockIdx.x * wpb + threadIdx.x / kWave; c < nchunks; c += gridDim.x * wpb) {
-    const int v = c * kWave + lane;
+    const int v = c * kWave + lane_id();
     const bool valid = v < nodes;
     const int beg = valid ? nindex[v] : 0;
     const int end = valid ? nindex[v + 1] : 0;
     const int dv = end - beg;
-    if (valid && dv < kTcWaveDeg) {
+    if (valid && dv < kMidDeg) {
       int k = 0;
       int bad = 0;
       for (int e = beg; e < end; e++) {
@@ -79,84 +79,41 @@ __global__ __launch_bounds__(kTcBlock) void tc_orient(int nodes, const int* __re
         if (bad) st_relaxed(&ctl->error, 1);
       }
     }
-    // huge lists: count pass records them (packed append); fill pass skips
-    const bool huge = valid && dv >= kHugeDeg;
     if (!kFill) {
-      const unsigned long long hm = __ballot(huge);
-      if (hm != 0) {
-        const int chunks = huge ? (dv + kHugeChunk - 1) / kHugeChunk : 0;
-        const int incl = wave_inclusive_scan(chunks);
-        const int total = wave_shfl(incl, kWave - 1);
-        const int leader = __ffsll(static_cast<long long>(hm)) - 1;
-        unsigned long long old = 0;
-        if (lane == leader)
-          old = atomic_add_relaxed(&ctl->hpack, (static_cast<unsigned long long>(total) << 32) | __popcll(hm));
-        old = wave_shfl(old, leader);
-        if (huge) {
-          const unsigned slot = static_cast<unsigned>(old & 0xffffffffull) + lanes_below(hm);
-          hq[slot] = v;
-          hoff[slot] = static_cast<unsigned>(old >> 32) + static_cast<unsigned>(incl - chunks);
-          ocount[v] = 0;
-        }
-      }
-    }
-    // mid-size lists: one wave per vertex, order kept with ballot prefixes
-    unsigned long long mid = __ballot(valid && dv >= kTcWaveDeg && dv < kHugeDeg);
-    while (mid != 0) {
-      const int l = __ffsll(static_cast<long long>(mid)) - 1;
-      mid &= mid - 1;
-      const int mv = wave_shfl(v, l);
-      const int mb = wave_shfl(beg, l);
-      const int me = wave_shfl(end, l);
-      const int md = me - mb;
-      int k = 0;
-      bool bad = false;
-      for (int base = mb; base < me; base += kWave) {
-        const int e = base + lane;
-        bool keep = false;
-        int u = 0;
-        if (e < me) {
-          u = nlist[e];
-          if (!kFill) bad |= (u == mv) || (e > mb && nlist[e - 1] >= u);
-          keep = outranks(deg(nindex, u), u, md, mv);
-        }
-        const unsigned long long m = __ballot(keep);
-        if (kFill && keep) olist[oindex[mv] + k + lanes_below(m)] = u;
-        k += __popcll(m);
-      }
-      if (!kFill) {
-        if (lane == 0) ocount[mv] = k;
-        if (__ballot(bad) != 0 && lane == 0) st_relaxed(&ctl->error, 1);
-      }
+      const bool long_list = valid && dv >= kMidDeg;
+      if (long_list) ocount[v] = 0;
+      frontier_emit(long_list, v, dv, heavy, &ctl->heavy);
     }
   }
 }
 
-// Pass 1 / 3 for huge lists: one workgroup per kHugeChunk-entry item.
+// Pass 1 / 3 for long lists: huge ones in kHugeChunk-entry items (one
+// workgroup each, per-item counts in hcnt), mid ones one wave each; order is
+// kept with ballot prefixes so the oriented lists stay sorted.
 template <bool kFill>
-__global__ __launch_bounds__(kTcBlock) void tc_orient_huge(const int* __restrict__ nindex,
-                                                           const int* __restrict__ nlist, int* ocount,
-                                                           const int* __restrict__ oindex, int* olist, const int* hq,
-                                                           const unsigned* hoff, int* hcnt, TcControl* ctl)
+__global__ __launch_bounds__(kTcBlock) void tc_orient_heavy(const int* __restrict__ nindex,
+                                                            const int* __restrict__ nlist, int* ocount,
+                                                            const int* __restrict__ oindex, int* olist,
+                                                            FrontierQueues heavy, int* hcnt, TcControl* ctl)
 {
   __shared__ int s_owner, s_off;
   __shared__ int s_wcnt[kTcBlock / kWave];
   const int lane = lane_id();
   const int wid = threadIdx.x / kWave;
-  const unsigned long long hp = ctl->hpack;
+  const unsigned long long hp = ctl->heavy.hpack;
   const unsigned nh = huge_count(hp), nch = huge_chunks(hp);
   for (unsigned item = blockIdx.x; item < nch; item += gridDim.x) {
-    if (threadIdx.x == 0) s_owner = huge_owner(hoff, static_cast<int>(nh), item);
+    if (threadIdx.x == 0) s_owner = huge_owner(heavy.hoff, static_cast<int>(nh), item);
     __syncthreads();
     const int j = s_owner;
-    const int v = hq[j];
+    const int v = heavy.hq[j];
     const int vb = nindex[v], ve = nindex[v + 1];
     const int dv = ve - vb;
-    const int beg = vb + static_cast<int>(item - hoff[j]) * kHugeChunk;
+    const int beg = vb + static_cast<int>(item - heavy.hoff[j]) * kHugeChunk;
     const int end = min(ve, beg + kHugeChunk);
     if (kFill && threadIdx.x == 0) {
       int off = oindex[v];
-      for (unsigned k = hoff[j]; k < item; k++) off += hcnt[k];
+      for (unsigned k = heavy.hoff[j]; k < item; k++) off += hcnt[k];
       s_off = off;
     }
     __syncthreads();
@@ -189,6 +146,31 @@ __global__ __launch_bounds__(kTcBlock) void tc_orient_huge(const int* __restrict
     }
     if (!kFill && bad) st_relaxed(&ctl->error, 1);
     __syncthreads();
+  }
+  const unsigned nm = ctl->heavy.nm;
+  for (unsigned i = blockIdx.x * (kTcBlock / kWave) + wid; i < nm; i += gridDim.x * (kTcBlock / kWave)) {
+    const int v = heavy.mq[i];
+    const int vb = nindex[v], ve = nindex[v + 1];
+    const int dv = ve - vb;
+    int k = kFill ? oindex[v] : 0;
+    bool bad = false;
+    for (int base = vb; base < ve; base += kWave) {
+      const int e = base + lane;
+      bool keep = false;
+      int u = 0;
+      if (e < ve) {
+        u = nlist[e];
+        if (!kFill) bad |= (u == v) || (e > vb && nlist[e - 1] >= u);
+        keep = outranks(deg(nindex, u), u, dv, v);
+      }
+      const unsigned long long m = __ballot(keep);
+      if (kFill && keep) olist[k + lanes_below(m)] = u;
+      k += __popcll(m);
+    }
+    if (!kFill) {
+      if (lane == 0) ocount[v] = k;
+      if (__ballot(bad) != 0 && lane == 0) st_relaxed(&ctl->error, 1);
+    }
   }
 }
 
@@ -266,7 +248,7 @@ int grid_for(long long items, int per_block, int cap)
 size_t tc_workspace_bytes(int nodes, int edges)
 {
   const size_t v = static_cast<size_t>(nodes), e = static_cast<size_t>(edges);
-  return Carver::round(sizeof(TcControl)) + 4 * Carver::round((v + 1) * 4) + Carver::round(e * 4 + 64) +
+  return Carver::round(sizeof(TcControl)) + 5 * Carver::round((v + 1) * 4) + Carver::round(e * 4 + 64) +
          Carver::round((e / kHugeChunk + e / kHugeDeg + 2) * 4) + Carver::round(scan_temp_bytes(nodes + 1)) + 4096;
 }
 
@@ -276,20 +258,19 @@ int tc_run(indigo_ctx* ctx, int nodes, int edges, const int* nindex, const int* 
   TcControl* ctl = cv.take<TcControl>(1);
   int* ocount = cv.take<int>(static_cast<size_t>(nodes) + 1);
   int* oindex = cv.take<int>(static_cast<size_t>(nodes) + 1);
-  int* hq = cv.take<int>(static_cast<size_t>(nodes) + 1);
-  unsigned* hoff = cv.take<unsigned>(static_cast<size_t>(nodes) + 1);
+  FrontierQueues heavy{nullptr, cv.take<int>(nodes), cv.take<int>(nodes), cv.take<unsigned>(nodes)};
   int* olist = cv.take<int>(static_cast<size_t>(edges) + 16);
   int* hcnt = cv.take<int>(static_cast<size_t>(edges) / kHugeChunk + static_cast<size_t>(edges) / kHugeDeg + 2);
   int* temp = cv.take<int>(scan_temp_bytes(nodes + 1) / 4 + 1);
   IG_TRY(zero_async(ctx, ctl, sizeof(TcControl)));
   hipStream_t s = ctx->stream;
   const int gw = grid_for((nodes + kWave - 1) / kWave, kTcBlock / kWave, 1 << 16);
-  const int gh = ctx->compute_units * 4;
-  tc_orient<false><<<gw, kTcBlock, 0, s>>>(nodes, nindex, nlist, ocount, oindex, olist, hq, hoff, ctl);
-  tc_orient_huge<false><<<gh, kTcBlock, 0, s>>>(nindex, nlist, ocount, oindex, olist, hq, hoff, hcnt, ctl);
+  const int gh = ctx->compute_units * 8;
+  tc_orient<false><<<gw, kTcBlock, 0, s>>>(nodes, nindex, nlist, ocount, oindex, olist, heavy, ctl);
+  tc_orient_heavy<false><<<gh, kTcBlock, 0, s>>>(nindex, nlist, ocount, oindex, olist, heavy, hcnt, ctl);
   IG_TRY(exclusive_scan(s, ocount, oindex, nodes, temp));
-  tc_orient<true><<<gw, kTcBlock, 0, s>>>(nodes, nindex, nlist, ocount, oindex, olist, hq, hoff, ctl);
-  tc_orient_huge<true><<<gh, kTcBlock, 0, s>>>(nindex, nlist, ocount, oindex, olist, hq, hoff, hcnt, ctl);
+  tc_orient<true><<<gw, kTcBlock, 0, s>>>(nodes, nindex, nlist, ocount, oindex, olist, heavy, ctl);
+  tc_orient_heavy<true><<<gh, kTcBlock, 0, s>>>(nindex, nlist, ocount, oindex, olist, heavy, hcnt, ctl);
   tc_count<<<ctx->compute_units * 8, kTcBlock, 0, s>>>(nodes, oindex, olist, ctl);
   IG_HIP_TRY(hipGetLastError());
   if (launches) *launches = 10;

@@ -1,0 +1,200 @@
+"""CPU tests that pin the oracle (oracle/indigo_oracle.c) to the reference.
+
+Each oracle function is checked against (a) the golden vectors written by the
+reference's own C++ code (tests/golden, see make_golden.py), (b) the compiled
+reference itself on fresh inputs when oracle/_ref is built, and (c) facts the
+mathematics fixes (closed forms, library routines, brute force), chosen so
+that a dropped term, a wrong index or a transposed operand fails one of them.
+"""
+import glob
+import itertools
+import os
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+from scipy.sparse import csgraph
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+GOLDEN_NAMES = sorted(os.path.basename(p)[:-4] for p in glob.glob(os.path.join(GOLDEN, "*.egr")))
+
+
+def load_golden(ig, name):
+    g = ig.read_egr(os.path.join(GOLDEN, name + ".egr"))
+    with np.load(os.path.join(GOLDEN, name + ".npz"), allow_pickle=False) as z:
+        ref = {k: z[k] for k in z.files}
+    return g, ref
+
+
+def csr_matrix(g, weights=False):
+    data = g.eweight.astype(np.float64) if weights else np.ones(g.edges)
+    return sp.csr_matrix((data, g.nlist, g.nindex), shape=(g.nodes, g.nodes))
+
+
+def test_golden_set_present():
+    assert len(GOLDEN_NAMES) >= 10
+
+
+# ---- (a) golden vectors from the reference ----------------------------------
+@pytest.mark.parametrize("name", GOLDEN_NAMES)
+def test_oracle_matches_reference_goldens(ig, orc, name):
+    g, ref = load_golden(ig, name)
+    for key, want in ref.items():
+        if key.startswith("bfs_"):
+            assert np.array_equal(orc.bfs(g, int(key[4:])), want), key
+        elif key.startswith("sssp_"):
+            assert np.array_equal(orc.sssp(g, int(key[5:])), want), key
+    assert np.array_equal(orc.cc(g), ref["cc"])
+    assert np.array_equal(orc.mis(g), ref["mis"])
+    assert orc.tc(g) == int(ref["tc"][0])
+    scores, it = orc.pr_ref32(g)
+    assert it == int(ref["pr_iterations"][0])
+    assert np.array_equal(scores, ref["pr"])  # same float recurrence, bit for bit
+
+
+# ---- (b) the compiled reference on fresh inputs ------------------------------
+ref_built = pytest.mark.skipif(
+    not all(os.path.exists(os.path.join(os.path.dirname(GOLDEN), "..", "oracle", "_ref", f"libref_{a}.so"))
+            for a in ("bfs", "sssp", "cc", "mis", "pr", "tc")),
+    reason="oracle/_ref not built (needs the reference sources)")
+
+
+@ref_built
+@pytest.mark.parametrize("seed", [11, 12])
+def test_oracle_matches_compiled_reference(ig, orc, seed):
+    for g in (ig.rmat(11, 8, seed=seed, wmin=1, wmax=255), ig.road(40, 40, seed=seed, wmin=1, wmax=1000),
+              ig.erdos_renyi(2000, 5000, seed=seed, wmin=1, wmax=255)):
+        for s in (0, g.nodes - 1):
+            assert np.array_equal(orc.bfs(g, s), orc.ref_bfs(g, s, 4)[0]), g.name
+            assert np.array_equal(orc.sssp(g, s), orc.ref_sssp(g, s, 4)[0]), g.name
+        assert np.array_equal(orc.cc(g), orc.ref_cc(g, 4)[0]), g.name
+        st, pri, _ = orc.ref_mis(g, 4)
+        assert np.array_equal(orc.mis(g), st), g.name
+        assert pri[3] == orc.mis_hash(3 + 712313887)
+        sc, it, _ = orc.ref_pr(g, 4)
+        mine, it2 = orc.pr_ref32(g)
+        assert it == it2 and np.array_equal(sc, mine), g.name
+        assert orc.tc(g) == orc.ref_tc(g, 4)[0], g.name
+
+
+# ---- (c) independent facts ----------------------------------------------------
+def test_bfs_grid_closed_form(ig, orc):
+    g = ig.grid(37, 53)
+    r, c = np.divmod(np.arange(g.nodes), 53)
+    assert np.array_equal(orc.bfs(g, 0), r + c)
+    # from the far corner: distance is (36 - r) + (52 - c)
+    assert np.array_equal(orc.bfs(g, g.nodes - 1), (36 - r) + (52 - c))
+
+
+def test_bfs_unreachable_is_int_max(ig, orc):
+    g = ig.from_edges(5, [0, 3], [1, 4])
+    d = orc.bfs(g, 0)
+    assert list(d) == [0, 1, 2**31 - 1, 2**31 - 1, 2**31 - 1]
+
+
+@pytest.mark.parametrize("seed", [1, 2, 3])
+def test_sssp_matches_scipy_dijkstra(ig, orc, seed):
+    g = ig.rmat(10, 8, seed=seed, wmin=1, wmax=255)
+    want = csgraph.dijkstra(csr_matrix(g, True), indices=0)
+    got = orc.sssp(g, 0).astype(np.float64)
+    got[got == 2**31 - 1] = np.inf
+    assert np.array_equal(got, want)
+
+
+@pytest.mark.parametrize("seed", [1, 2])
+def test_cc_matches_scipy_components(ig, orc, seed):
+    g = ig.road(30, 30, keep_p=0.5, seed=seed)
+    ncomp, lab = csgraph.connected_components(csr_matrix(g), directed=False)
+    mins = np.full(ncomp, g.nodes)
+    np.minimum.at(mins, lab, np.arange(g.nodes))
+    assert np.array_equal(orc.cc(g), mins[lab])
+
+
+def _brute_lexicographic_mis(g, key):
+    order = sorted(range(g.nodes), key=key, reverse=True)
+    st = [0] * g.nodes
+    for v in order:
+        if st[v] == 0:
+            st[v] = 1
+            for u in g.nlist[g.nindex[v]:g.nindex[v + 1]]:
+                if u != v:
+                    st[u] = 2
+    return np.array(st)
+
+
+@pytest.mark.parametrize("signed", [False, True])
+def test_mis_greedy_order_and_validity(ig, orc, signed):
+    g = ig.erdos_renyi(300, 900, seed=5)
+    pri = [orc.mis_hash(v + 712313887) for v in range(g.nodes)]
+    if signed:
+        key = lambda v: (pri[v] - (1 << 32) if pri[v] >= 1 << 31 else pri[v], v)  # noqa: E731
+    else:
+        key = lambda v: (pri[v], v)  # noqa: E731
+    st = orc.mis(g, signed)
+    assert np.array_equal(st, _brute_lexicographic_mis(g, key))
+    for v in range(g.nodes):
+        nb = g.nlist[g.nindex[v]:g.nindex[v + 1]]
+        if st[v] == 1:
+            assert not (st[nb] == 1).any()  # independent
+        else:
+            assert (st[nb] == 1).any()  # maximal
+
+
+def test_mis_hash_known_values(orc):
+    # xor-shift-multiply by 0x45d9f3b, restated independently in Python
+    def h(x):
+        x = (((x >> 16) ^ x) * 0x45D9F3B) & 0xFFFFFFFF
+        x = (((x >> 16) ^ x) * 0x45D9F3B) & 0xFFFFFFFF
+        return (x >> 16) ^ x
+    for v in (0, 1, 2, 1000, 2**31 - 712313888):
+        assert orc.mis_hash(v + 712313887) == h((v + 712313887) & 0xFFFFFFFF)
+
+
+def test_mis_signedness_differs_as_reported(ig, orc):
+    """The suite's C++ (unsigned) and CUDA (signed) orders give different sets
+    on a 32x32 grid: 385 vs 377 members."""
+    g = ig.grid(32, 32)
+    assert int((orc.mis(g, False) == 1).sum()) == 385
+    assert int((orc.mis(g, True) == 1).sum()) == 377
+
+
+def test_tc_closed_forms(ig, orc):
+    for n in (3, 4, 7, 12):
+        s, d = np.triu_indices(n, 1)
+        assert orc.tc(ig.from_edges(n, s, d)) == n * (n - 1) * (n - 2) // 6
+    assert orc.tc(ig.grid(20, 30)) == 0
+
+
+def test_tc_brute_force(ig, orc):
+    g = ig.erdos_renyi(80, 600, seed=9)
+    adj = set(zip(np.repeat(np.arange(g.nodes), g.degree), g.nlist))
+    cnt = sum(1 for a, b, c in itertools.combinations(range(g.nodes), 3)
+              if (a, b) in adj and (b, c) in adj and (a, c) in adj)
+    assert orc.tc(g) == cnt
+
+
+def test_pr64_regular_graph_stays_uniform(ig, orc):
+    # on a regular graph 1/N is the fixed point of the recurrence
+    n = 50
+    s = np.arange(n)
+    g = ig.from_edges(n, np.concatenate([s, s]), np.concatenate([(s + 1) % n, (s + 7) % n]))
+    scores, diff = orc.pr64(g, 5)
+    assert np.allclose(scores, 1.0 / n, rtol=0, atol=1e-15) and diff < 1e-15
+
+
+def test_pr64_star_closed_form(ig, orc):
+    # star with k leaves, one step from 1/N: centre = b + d*k/N, leaf = b + d/(N*k)
+    k = 9
+    n = k + 1
+    g = ig.from_edges(n, np.zeros(k, np.int32), np.arange(1, n))
+    scores, _ = orc.pr64(g, 1)
+    b, d = 0.15 / n, 0.85
+    assert scores[0] == pytest.approx(b + d * k / n, abs=1e-15)
+    assert np.allclose(scores[1:], b + d / (n * k), atol=1e-15)
+
+
+def test_pr32_close_to_pr64(ig, orc):
+    g = ig.rmat(10, 16, seed=1)
+    s32, it = orc.pr_ref32(g)
+    s64, _ = orc.pr64(g, it)
+    assert np.abs(s32 - s64).max() < 1e-6
