@@ -138,6 +138,12 @@ int indigo_ctx_last_stats(indigo_ctx* ctx, int* launches, int* iterations);
 /* Workgroups of the last cooperative (persistent) launch and the per-CU
  * residency it was sized for. */
 int indigo_ctx_last_grid(indigo_ctx* ctx, int* grid, int* per_cu);
+/* Per-level trace of the last BFS or SSSP call (first 256 levels): start of
+ * each level in microseconds since the kernel began, frontier size, its
+ * degree sum, and the mode (0 top-down, 1 bottom-up, 2 one-workgroup phase,
+ * 3 SSSP far-pile split).  Any output pointer may be NULL. */
+int indigo_ctx_last_trace(indigo_ctx* ctx, int max, float* us, unsigned* frontier, double* degree_sum, int* mode,
+                          int* n);
 /* Release the cached workspace (it regrows on the next call). */
 int indigo_ctx_trim(indigo_ctx* ctx);
 /* Optional tuning knob: block size / variant selection hints (0 = auto). */

@@ -6,7 +6,9 @@
 // (include/bfs_vertex_cuda.h:75-99).
 //
 // Design: one cooperative persistent kernel runs every level; levels are
-// separated by the bounded grid barrier, so the host never round-trips per
+// separated by the bounded grid barrier (coherent mode: dist, queues and
+// counters are only touched with device-scope atomic accesses), so the host
+// never round-trips per
 // level (the reference copies a goagain flag D2H every level,
 // e.g. codes/cuda/bfs-cuda/BFS_CUDA_V_Topo_Push_Determ_IntType_ReadModifyWrite_
 // NonPersist_Atomic_Thread_NoBoundsBug2_NoBoundsBug1.cu:98-106).
@@ -28,6 +30,7 @@
 #include "bfs.hpp"
 
 #include "frontier.hpp"
+#include "trace.hpp"
 
 namespace indigo {
 
@@ -49,6 +52,7 @@ struct BfsControl {
   int status;
   int level_base;  // level at which the grid resumes after a one-workgroup phase
   int pad[27];
+  LevelTrace trace;
 };
 
 struct BfsArgs {
@@ -76,7 +80,7 @@ IG_DEV bool claim(int* dist, int u, int next)
 }
 
 IG_DEV void top_down(const BfsArgs& a, int L, const FrontierCounters& ic, const FrontierQueues& in,
-                     const FrontierQueues& out, FrontierCounters* oc)
+                     const FrontierQueues& out, FrontierCounters* oc, FrontierEmitter& E)
 {
   const int next = L + 1;
   auto visit = [&](int, int e, bool active) {
@@ -87,7 +91,7 @@ IG_DEV void top_down(const BfsArgs& a, int L, const FrontierCounters& ic, const 
       won = claim(a.dist, u, next);
       if (won) udeg = degree_of(a.nindex, u);
     }
-    frontier_emit(won && udeg > 0, u, udeg, out, oc);
+    frontier_emit_block(E, won && udeg > 0, u, udeg, out, oc);
   };
   expand_huge(in, huge_count(ic.hpack), huge_chunks(ic.hpack), a.nindex, visit);
   expand_mid(in, ic.nm, a.nindex, visit);
@@ -95,7 +99,7 @@ IG_DEV void top_down(const BfsArgs& a, int L, const FrontierCounters& ic, const 
 }
 
 IG_DEV void bottom_up(const BfsArgs& a, int L, unsigned heavy_mid, unsigned heavy_huge, const FrontierQueues& out,
-                      FrontierCounters* oc)
+                      FrontierCounters* oc, FrontierEmitter& E)
 {
   const int next = L + 1;
   const int lane = lane_id();
@@ -105,38 +109,38 @@ IG_DEV void bottom_up(const BfsArgs& a, int L, unsigned heavy_mid, unsigned heav
     const int v = base + static_cast<int>(threadIdx.x);
     bool found = false;
     int deg = 0;
-    if (v < a.nodes && a.dist[v] == kInf) {
+    if (v < a.nodes && ld_relaxed(&a.dist[v]) == kInf) {
       const int beg = a.nindex[v], end = a.nindex[v + 1];
       deg = end - beg;
       if (deg < kMidDeg) {
         for (int e = beg; e < end; e++) {
-          if (a.dist[a.nlist[e]] == L) {
+          if (ld_relaxed(&a.dist[a.nlist[e]]) == L) {
             found = true;
             break;
           }
         }
-        if (found) a.dist[v] = next;
+        if (found) st_relaxed(&a.dist[v], next);
       }
     }
-    frontier_emit(found, v, deg, out, oc);
+    frontier_emit_block(E, found, v, deg, out, oc);
   }
   // long lists: one wave per vertex, 64 entries per ballot
   const int wpb = blockDim.x / kWave;
   const unsigned total = heavy_mid + heavy_huge;
   for (unsigned i = blockIdx.x * wpb + threadIdx.x / kWave; i < total; i += gridDim.x * wpb) {
-    const int v = i < heavy_mid ? a.heavy.mq[i] : a.heavy.hq[i - heavy_mid];
-    if (a.dist[v] != kInf) continue;
+    const int v = i < heavy_mid ? ld_relaxed(&a.heavy.mq[i]) : ld_relaxed(&a.heavy.hq[i - heavy_mid]);
+    if (ld_relaxed(&a.dist[v]) != kInf) continue;
     const int beg = a.nindex[v], end = a.nindex[v + 1];
     bool found = false;
     for (int base = beg; base < end; base += kWave) {
       const int e = base + lane;
-      if (__ballot(e < end && a.dist[a.nlist[e]] == L) != 0) {
+      if (__ballot(e < end && ld_relaxed(&a.dist[a.nlist[e]]) == L) != 0) {
         found = true;
         break;
       }
     }
-    if (found && lane == 0) a.dist[v] = next;
-    frontier_emit(found && lane == 0, v, end - beg, out, oc);
+    if (found && lane == 0) st_relaxed(&a.dist[v], next);
+    frontier_emit_block(E, found && lane == 0, v, end - beg, out, oc);
   }
 }
 
@@ -157,7 +161,7 @@ IG_DEV int small_phase(const BfsArgs& a, int L, unsigned n, BfsControl* ctl, lon
   __shared__ int s_abort;
   const int lane = lane_id();
   const int* gin = a.fq[L & 1].q;
-  for (unsigned i = threadIdx.x; i < n; i += blockDim.x) s_q[0][i] = gin[i];
+  for (unsigned i = threadIdx.x; i < n; i += blockDim.x) s_q[0][i] = ld_relaxed(&gin[i]);
   if (threadIdx.x == 0) {
     s_cnt[0] = static_cast<int>(n);
     s_cnt[1] = 0;
@@ -258,18 +262,20 @@ IG_DEV int small_phase(const BfsArgs& a, int L, unsigned n, BfsControl* ctl, lon
 
 __global__ __launch_bounds__(kBfsBlock) void bfs_kernel(BfsArgs a)
 {
+  __shared__ FrontierEmitter s_emit;
   BfsControl* ctl = a.ctl;
   if (!residency_barrier(&ctl->bar, a.start_ticks)) return;
   const long long deadline = now_ticks() + a.budget;
   const int tid = blockIdx.x * blockDim.x + threadIdx.x;
   const int stride = gridDim.x * blockDim.x;
-  for (int i = tid; i < a.nodes; i += stride) a.dist[i] = (i == a.src) ? 0 : kInf;
-  if (a.allow_bu) collect_heavy(a.nodes, a.nindex, a.heavy, &ctl->heavy);
+  if (tid == 0) ctl->trace.start = now_ticks();
+  for (int i = tid; i < a.nodes; i += stride) st_relaxed(&a.dist[i], (i == a.src) ? 0 : kInf);
+  if (a.allow_bu) collect_heavy(s_emit, a.nodes, a.nindex, a.heavy, &ctl->heavy);
   if (blockIdx.x == 0 && threadIdx.x < kWave) {
     const int deg = degree_of(a.nindex, a.src);
     frontier_emit(threadIdx.x == 0 && deg > 0, a.src, deg, a.fq[0], &ctl->cnt[0]);
   }
-  if (!grid_barrier(&ctl->bar, deadline)) {
+  if (!grid_barrier<true>(&ctl->bar, deadline)) {
     if (tid == 0) ctl->status = 1;
     return;
   }
@@ -280,14 +286,7 @@ __global__ __launch_bounds__(kBfsBlock) void bfs_kernel(BfsArgs a)
   int bu_levels = 0;
   int L = 0;
   for (;;) {
-    FrontierCounters ic;
-    {
-      FrontierCounters* p = &ctl->cnt[L % 3];
-      ic.n = ld_relaxed(&p->n);
-      ic.nm = ld_relaxed(&p->nm);
-      ic.hpack = ld_relaxed(&p->hpack);
-      ic.m = ld_relaxed(&p->m);
-    }
+    const FrontierCounters ic = counters_load(&ctl->cnt[L % 3]);
     const unsigned long long nf = static_cast<unsigned long long>(ic.n) + ic.nm + huge_count(ic.hpack);
     if (nf == 0) break;
     seen_edges += ic.m;
@@ -300,7 +299,10 @@ __global__ __launch_bounds__(kBfsBlock) void bfs_kernel(BfsArgs a)
         bu = false;
       }
     }
-    if (!bu && a.allow_small && ic.nm == 0 && ic.hpack == 0 && ic.m <= static_cast<unsigned long long>(kSmallCap)) {
+    const bool small = !bu && a.allow_small && ic.nm == 0 && ic.hpack == 0 &&
+                       ic.m <= static_cast<unsigned long long>(kSmallCap);
+    trace_level(&ctl->trace, L, static_cast<unsigned>(nf), ic.m, small ? 2 : (bu ? 1 : 0));
+    if (small) {
       // one-workgroup phase; the other workgroups acknowledge that they have
       // read this level's counters and wait at the next barrier
       if (blockIdx.x != 0 && threadIdx.x == 0)
@@ -309,11 +311,11 @@ __global__ __launch_bounds__(kBfsBlock) void bfs_kernel(BfsArgs a)
         bool aborted = false;
         const int L2 = small_phase(a, L, ic.n, ctl, deadline, &aborted);
         if (threadIdx.x == 0) {
-          ctl->level_base = L2;
+          st_relaxed(&ctl->level_base, L2);
           ctl->small_levels += L2 - L;
         }
       }
-      if (!grid_barrier(&ctl->bar, deadline)) {
+      if (!grid_barrier<true>(&ctl->bar, deadline)) {
         if (tid == 0) ctl->status = 1;
         return;
       }
@@ -329,13 +331,15 @@ __global__ __launch_bounds__(kBfsBlock) void bfs_kernel(BfsArgs a)
     const FrontierQueues& in = a.fq[L & 1];
     const FrontierQueues& out = a.fq[(L + 1) & 1];
     FrontierCounters* oc = &ctl->cnt[(L + 1) % 3];
+    fe_reset(s_emit);
     if (bu) {
-      bottom_up(a, L, heavy_mid, heavy_huge, out, oc);
+      bottom_up(a, L, heavy_mid, heavy_huge, out, oc, s_emit);
       bu_levels++;
     } else {
-      top_down(a, L, ic, in, out, oc);
+      top_down(a, L, ic, in, out, oc, s_emit);
     }
-    if (!grid_barrier(&ctl->bar, deadline)) {
+    fe_flush(s_emit, out, oc);
+    if (!grid_barrier<true>(&ctl->bar, deadline)) {
       if (tid == 0) ctl->status = 1;
       return;
     }
@@ -406,6 +410,7 @@ int bfs_collect(indigo_ctx* ctx, BfsStats* stats)
     stats->bottom_up_levels = h->bottom_up_levels;
     stats->small_levels = h->small_levels;
   }
+  store_trace(ctx, h->trace);
   return INDIGO_OK;
 }
 

@@ -1,5 +1,8 @@
 // Degree-binned frontier queues for the persistent traversal kernels.
 //
+// Queue contents and counters are written and read with device-scope atomic
+// accesses only, so kernels built on them may use the coherent grid barrier.
+//
 // A frontier is three queues, binned by adjacency-list length when a vertex
 // is appended:
 //   * small: length < kMidDeg.  Expanded 64 vertices per wave with an in-wave
@@ -22,6 +25,7 @@
 #pragma once
 
 #include "device_common.hpp"
+#include "emit.hpp"
 
 namespace indigo {
 
@@ -38,6 +42,31 @@ struct alignas(128) FrontierCounters {
   unsigned long long pad[12];
 };
 
+// Per-level trace written by workgroup 0 of a traversal kernel: wall-clock
+// tick at the start of the level, frontier size, its degree sum and the mode
+// (0 top-down, 1 bottom-up, 2 one-workgroup phase, 3 far-pile split).
+constexpr int kTraceMax = 256;
+struct LevelTrace {
+  int n;
+  int pad;
+  long long start;
+  long long t[kTraceMax];
+  unsigned nf[kTraceMax];
+  unsigned long long m[kTraceMax];
+  int mode[kTraceMax];
+};
+
+IG_DEV void trace_level(LevelTrace* tr, int level, unsigned nf, unsigned long long m, int mode)
+{
+  if (blockIdx.x == 0 && threadIdx.x == 0 && level < kTraceMax) {
+    tr->t[level] = now_ticks();
+    tr->nf[level] = nf;
+    tr->m[level] = m;
+    tr->mode[level] = mode;
+    tr->n = level + 1;
+  }
+}
+
 struct FrontierQueues {
   int* q;          // small queue
   int* mq;         // mid queue
@@ -50,11 +79,23 @@ IG_DEV unsigned huge_chunks(unsigned long long hpack) { return static_cast<unsig
 
 IG_DEV void counters_clear(FrontierCounters* c)
 {
-  c->n = 0;
-  c->nm = 0;
-  c->hpack = 0;
-  c->m = 0;
-  c->aux = 0;
+  st_relaxed(&c->n, 0u);
+  st_relaxed(&c->nm, 0u);
+  st_relaxed(&c->hpack, 0ull);
+  st_relaxed(&c->m, 0ull);
+  st_relaxed(&c->aux, 0ull);
+}
+
+// Snapshot of a counter set (device-scope loads).
+IG_DEV FrontierCounters counters_load(const FrontierCounters* c)
+{
+  FrontierCounters r;
+  r.n = ld_relaxed(&c->n);
+  r.nm = ld_relaxed(&c->nm);
+  r.hpack = ld_relaxed(&c->hpack);
+  r.m = ld_relaxed(&c->m);
+  r.aux = ld_relaxed(&c->aux);
+  return r;
 }
 
 // Wave-collective: every active lane with `pred` appends vertex u (whose list
@@ -82,10 +123,71 @@ IG_DEV void frontier_emit(bool pred, int u, int deg, const FrontierQueues& fq, F
     old = wave_shfl(old, leader);
     if (huge) {
       const unsigned slot = static_cast<unsigned>(old & 0xffffffffull) + lanes_below(hm);
-      fq.hq[slot] = u;
-      fq.hoff[slot] = static_cast<unsigned>(old >> 32) + static_cast<unsigned>(incl - chunks);
+      st_relaxed(&fq.hq[slot], u);
+      st_relaxed(&fq.hoff[slot], static_cast<unsigned>(old >> 32) + static_cast<unsigned>(incl - chunks));
     }
   }
+}
+
+// Workgroup-aggregated version of frontier_emit (see emit.hpp): small and mid
+// appends and the degree sum are buffered in LDS and flushed with one atomic
+// per queue by fe_flush; huge entries (rare) still go straight to global.
+constexpr int kEmitSmallCap = 4096;
+constexpr int kEmitMidCap = 512;
+
+struct FrontierEmitter {
+  BlockQueue<kEmitSmallCap> small;
+  BlockQueue<kEmitMidCap> mid;
+  BlockSum m;
+};
+
+// Workgroup-collective.
+IG_DEV void fe_reset(FrontierEmitter& E)
+{
+  if (threadIdx.x == 0) {
+    E.small.count = 0;
+    E.mid.count = 0;
+    E.m.v = 0ull;
+  }
+  __syncthreads();
+}
+
+// Wave-collective.
+template <int kMid = kMidDeg>
+IG_DEV void frontier_emit_block(FrontierEmitter& E, bool pred, int u, int deg, const FrontierQueues& fq,
+                                FrontierCounters* c)
+{
+  const bool huge = pred && deg >= kHugeDeg;
+  const bool mid = pred && deg >= kMid && !huge;
+  const bool small = pred && deg < kMid;
+  bs_add(E.m, static_cast<unsigned long long>(pred ? deg : 0));
+  bq_append(E.small, small, u, fq.q, &c->n);
+  bq_append(E.mid, mid, u, fq.mq, &c->nm);
+  const unsigned long long hm = __ballot(huge);
+  if (hm != 0) {
+    const int chunks = huge ? (deg + kHugeChunk - 1) / kHugeChunk : 0;
+    const int incl = wave_inclusive_scan(chunks);
+    const int total = wave_shfl(incl, kWave - 1);
+    const int leader = __ffsll(static_cast<long long>(hm)) - 1;
+    unsigned long long old = 0;
+    if (lane_id() == leader)
+      old = atomic_add_relaxed(&c->hpack, (static_cast<unsigned long long>(total) << 32) | __popcll(hm));
+    old = wave_shfl(old, leader);
+    if (huge) {
+      const unsigned slot = static_cast<unsigned>(old & 0xffffffffull) + lanes_below(hm);
+      st_relaxed(&fq.hq[slot], u);
+      st_relaxed(&fq.hoff[slot], static_cast<unsigned>(old >> 32) + static_cast<unsigned>(incl - chunks));
+    }
+  }
+}
+
+// Workgroup-collective.
+IG_DEV void fe_flush(FrontierEmitter& E, const FrontierQueues& fq, FrontierCounters* c)
+{
+  bq_flush(E.small, fq.q, &c->n);
+  bq_flush(E.mid, fq.mq, &c->nm);
+  if (threadIdx.x == 0 && E.m.v != 0ull) atomic_add_relaxed(&c->m, E.m.v);
+  __syncthreads();
 }
 
 // Index j of the huge entry owning chunk `item`: the last j with hoff[j] <= item.
@@ -94,25 +196,29 @@ IG_DEV int huge_owner(const unsigned* hoff, int nh, unsigned item)
   int lo = 0, hi = nh - 1;
   while (lo < hi) {
     const int mid = (lo + hi + 1) >> 1;
-    if (hoff[mid] <= item) lo = mid;
+    if (ld_relaxed(&hoff[mid]) <= item) lo = mid;
     else hi = mid - 1;
   }
   return lo;
 }
 
-// Grid-stride pass over vertices 0..nodes-1 (64 per wave) that appends every
-// vertex with a list of >= kMid entries to the mid / huge queues.
+// Workgroup-collective grid-stride pass over vertices 0..nodes-1 (64 per
+// wave) that appends every vertex with a list of >= kMid entries to the mid /
+// huge queues, aggregated per workgroup.
 template <int kMid = kMidDeg>
-IG_DEV void collect_heavy(int nodes, const int* __restrict__ nindex, const FrontierQueues& fq, FrontierCounters* c)
+IG_DEV void collect_heavy(FrontierEmitter& E, int nodes, const int* __restrict__ nindex, const FrontierQueues& fq,
+                          FrontierCounters* c)
 {
   const int lane = lane_id();
   const int wpb = blockDim.x / kWave;
   const int nw = (nodes + kWave - 1) / kWave;
+  fe_reset(E);
   for (int w = blockIdx.x * wpb + threadIdx.x / kWave; w < nw; w += gridDim.x * wpb) {
     const int v = w * kWave + lane;
     const int deg = v < nodes ? nindex[v + 1] - nindex[v] : 0;
-    frontier_emit<kMid>(deg >= kMid, v, deg, fq, c);
+    frontier_emit_block<kMid>(E, deg >= kMid, v, deg, fq, c);
   }
+  fe_flush(E, fq, c);
 }
 
 // ---- expansion helpers ------------------------------------------------------
@@ -130,8 +236,8 @@ IG_DEV void expand_huge(const FrontierQueues& in, unsigned nh, unsigned nch, con
     __syncthreads();
     const int j = s_owner;
     __syncthreads();
-    const int v = in.hq[j];
-    const int beg = nindex[v] + static_cast<int>(item - in.hoff[j]) * kHugeChunk;
+    const int v = ld_relaxed(&in.hq[j]);
+    const int beg = nindex[v] + static_cast<int>(item - ld_relaxed(&in.hoff[j])) * kHugeChunk;
     const int end = min(nindex[v + 1], beg + kHugeChunk);
     for (int base = beg; base < end; base += blockDim.x) {
       const int e = base + static_cast<int>(threadIdx.x);
@@ -147,7 +253,7 @@ IG_DEV void expand_mid(const FrontierQueues& in, unsigned nm, const int* __restr
   const int lane = lane_id();
   const int wpb = blockDim.x / kWave;
   for (unsigned i = blockIdx.x * wpb + threadIdx.x / kWave; i < nm; i += gridDim.x * wpb) {
-    const int v = in.mq[i];
+    const int v = ld_relaxed(&in.mq[i]);
     const int beg = nindex[v], end = nindex[v + 1];
     for (int base = beg; base < end; base += kWave) {
       const int e = base + lane;
@@ -165,7 +271,7 @@ IG_DEV void expand_small(const int* __restrict__ q, unsigned n, const int* __res
   for (unsigned c = blockIdx.x * wpb + threadIdx.x / kWave; c * kWave < n; c += gridDim.x * wpb) {
     const unsigned idx = c * kWave + lane;
     const bool valid = idx < n;
-    const int v = valid ? q[idx] : 0;
+    const int v = valid ? ld_relaxed(&q[idx]) : 0;
     const int beg = valid ? nindex[v] : 0;
     const int deg = valid ? nindex[v + 1] - beg : 0;
     const int incl = wave_inclusive_scan(deg);

@@ -9,7 +9,9 @@
 // suite's C++ path orders priorities as unsigned int, its CUDA path as signed
 // int (data_type), selected here by `signed_priority`.
 //
-// Design: one cooperative persistent kernel; round r processes the vertices
+// Design: one cooperative persistent kernel (coherent-mode barriers: status,
+// worklists and counters only see device-scope atomic accesses); round r
+// processes the vertices
 // still undecided after round r-1, kept in two worklists by list length:
 // lists shorter than kMidDeg (round 0: every such vertex, in id order) are
 // scanned by one lane each, longer ones (round 0: a static list collected at
@@ -120,24 +122,28 @@ IG_DEV void exclude_neighbours_wave(const MisArgs& a, int v, int beg, int end)
 
 __global__ __launch_bounds__(kMisBlock) void mis_kernel(MisArgs a)
 {
+  __shared__ BlockQueue<4096> s_wl;
+  __shared__ BlockQueue<1024> s_hwl;
   MisControl* ctl = a.ctl;
   if (!residency_barrier(&ctl->bar, a.start_ticks)) return;
   const long long deadline = now_ticks() + a.budget;
   const int tid = blockIdx.x * blockDim.x + threadIdx.x;
   const int stride = gridDim.x * blockDim.x;
   const int lane = lane_id();
-  for (int i = tid; i < a.nodes; i += stride) a.status[i] = kUndecided;
+  for (int i = tid; i < a.nodes; i += stride) st_relaxed(&a.status[i], kUndecided);
   // round 0 long-list worklist: every vertex with >= kMidDeg entries
   {
     const int wpb = blockDim.x / kWave;
     const int nw = (a.nodes + kWave - 1) / kWave;
+    bq_reset(s_hwl);
     for (int w = blockIdx.x * wpb + threadIdx.x / kWave; w < nw; w += gridDim.x * wpb) {
       const int v = w * kWave + lane;
       const bool heavy = v < a.nodes && a.nindex[v + 1] - a.nindex[v] >= kMidDeg;
-      wave_append(heavy, v, a.hwl[0], &ctl->cnt[0].nm);
+      bq_append(s_hwl, heavy, v, a.hwl[0], &ctl->cnt[0].nm);
     }
+    bq_flush(s_hwl, a.hwl[0], &ctl->cnt[0].nm);
   }
-  if (!grid_barrier(&ctl->bar, deadline)) {
+  if (!grid_barrier<true>(&ctl->bar, deadline)) {
     if (tid == 0) ctl->status = 1;
     return;
   }
@@ -156,13 +162,15 @@ __global__ __launch_bounds__(kMisBlock) void mis_kernel(MisArgs a)
     int* out = a.wl[(r + 1) & 1];
     const int* hin = a.hwl[r & 1];
     int* hout = a.hwl[(r + 1) & 1];
+    bq_reset(s_wl);
+    bq_reset(s_hwl);
     // short lists: one lane each
     for (unsigned base = blockIdx.x * blockDim.x; base < n; base += stride) {
       const unsigned idx = base + threadIdx.x;
       bool keep = false;
       int v = 0;
       if (idx < n) {
-        v = (r == 0) ? static_cast<int>(idx) : in[idx];
+        v = (r == 0) ? static_cast<int>(idx) : ld_relaxed(&in[idx]);
         if (ld_relaxed(&a.status[v]) == kUndecided) {
           const int beg = a.nindex[v], end = a.nindex[v + 1];
           if (end - beg < kMidDeg) {
@@ -178,11 +186,11 @@ __global__ __launch_bounds__(kMisBlock) void mis_kernel(MisArgs a)
           }
         }
       }
-      wave_append(keep, v, out, &oc->n);
+      bq_append(s_wl, keep, v, out, &oc->n);
     }
     // long lists: one wave each
     for (unsigned i = gwave; i < nh; i += nwaves) {
-      const int v = hin[i];
+      const int v = ld_relaxed(&hin[i]);
       if (ld_relaxed(&a.status[v]) != kUndecided) continue;
       const int beg = a.nindex[v], end = a.nindex[v + 1];
       const int st = decide_wave(a, v, beg, end);
@@ -192,9 +200,11 @@ __global__ __launch_bounds__(kMisBlock) void mis_kernel(MisArgs a)
       } else if (st == kExcluded) {
         if (lane == 0) st_relaxed(&a.status[v], kExcluded);
       }
-      wave_append(st == kUndecided && lane == 0, v, hout, &oc->nm);
+      bq_append(s_hwl, st == kUndecided && lane == 0, v, hout, &oc->nm);
     }
-    if (!grid_barrier(&ctl->bar, deadline)) {
+    bq_flush(s_wl, out, &oc->n);
+    bq_flush(s_hwl, hout, &oc->nm);
+    if (!grid_barrier<true>(&ctl->bar, deadline)) {
       if (tid == 0) ctl->status = 1;
       return;
     }

@@ -155,6 +155,7 @@ __global__ __launch_bounds__(kPrBlock) void pr_kernel(PrArgs a)
   __shared__ double s_red[kPrBlock / kWave];
   __shared__ float s_ps[kPrBlock / kWave], s_pc[kPrBlock / kWave];
   __shared__ int s_owner;
+  __shared__ FrontierEmitter s_emit;
   PrControl* ctl = a.ctl;
   if (!residency_barrier(&ctl->bar, a.start_ticks)) return;
   const long long deadline = now_ticks() + a.budget;
@@ -170,7 +171,7 @@ __global__ __launch_bounds__(kPrBlock) void pr_kernel(PrArgs a)
     a.scores[v] = s;
     a.contrib[0][v] = s / static_cast<float>(deg_of(a, v));
   }
-  collect_heavy<kPrWaveDeg>(a.nodes, a.nindex, a.heavy, &ctl->heavy);
+  collect_heavy<kPrWaveDeg>(s_emit, a.nodes, a.nindex, a.heavy, &ctl->heavy);
   if (!grid_barrier(&ctl->bar, deadline)) {
     if (tid == 0) ctl->status = 1;
     if (threadIdx.x == 0) a.exit_it[blockIdx.x] = -1000000;

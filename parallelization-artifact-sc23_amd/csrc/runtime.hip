@@ -306,6 +306,21 @@ int indigo_ctx_last_grid(indigo_ctx* ctx, int* grid, int* per_cu)
   return INDIGO_OK;
 }
 
+int indigo_ctx_last_trace(indigo_ctx* ctx, int max, float* us, unsigned* frontier, double* degree_sum, int* mode,
+                          int* n)
+{
+  if (ctx == nullptr || n == nullptr) return set_error(INDIGO_ERR_INVALID_ARG, "ctx_last_trace: null argument");
+  const int k = ctx->trace_n < max ? ctx->trace_n : max;
+  for (int i = 0; i < k; i++) {
+    if (us) us[i] = ctx->trace_us[i];
+    if (frontier) frontier[i] = ctx->trace_nf[i];
+    if (degree_sum) degree_sum[i] = ctx->trace_m[i];
+    if (mode) mode[i] = ctx->trace_mode[i];
+  }
+  *n = k;
+  return INDIGO_OK;
+}
+
 int indigo_ctx_trim(indigo_ctx* ctx)
 {
   if (ctx == nullptr) return set_error(INDIGO_ERR_INVALID_ARG, "ctx_trim: null ctx");

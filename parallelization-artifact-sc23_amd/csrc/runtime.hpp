@@ -28,6 +28,12 @@ struct indigo_ctx {
   int hints[16] = {0};
   double budget_seconds = 60.0;  // bound on any device-side wait
   void* diag[4] = {nullptr, nullptr, nullptr, nullptr};  // device diagnostics of the last call
+  // per-level trace of the last traversal call (see indigo_ctx_last_trace)
+  int trace_n = 0;
+  float trace_us[256] = {0};
+  unsigned trace_nf[256] = {0};
+  double trace_m[256] = {0};
+  int trace_mode[256] = {0};
   int last_grid = 0;    // workgroups of the last cooperative launch
   int last_per_cu = 0;  // co-resident workgroups per CU it was sized for
 };

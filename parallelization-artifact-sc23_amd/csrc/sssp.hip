@@ -5,7 +5,9 @@
 // relaxations d = dist[v] + w in int arithmetic; results equal the serial
 // Dijkstra of sssp_vertex_cuda.h:74-99.
 //
-// Design: one cooperative persistent kernel running near-far
+// Design: one cooperative persistent kernel (coherent-mode barriers: dist,
+// stamps, queues, piles and counters only see device-scope atomic accesses)
+// running near-far
 // label-correcting SSSP (Davidson et al., IPDPS'14):
 //   * the near frontier holds vertices whose distance improved below the
 //     split T; it is expanded with the degree-binned machinery of
@@ -29,6 +31,7 @@
 #include "sssp.hpp"
 
 #include "frontier.hpp"
+#include "trace.hpp"
 
 namespace indigo {
 
@@ -46,6 +49,7 @@ struct SsspControl {
   int status;
   int delta;
   int pad[28];
+  LevelTrace trace;
 };
 
 struct SsspArgs {
@@ -72,8 +76,38 @@ IG_DEV int degree_of(const int* __restrict__ nindex, int u) { return nindex[u + 
 // below the split T joins the next near frontier (deduplicated per iteration
 // by stamp), one at or above T is parked in the far pile (deduplicated per
 // epoch by farstamp).  Wave-collective: all lanes converged.
+// Workgroup-aggregated emission state of one SSSP phase.
+struct SsspEmit {
+  FrontierEmitter near;
+  BlockQueue<2048> far;
+  unsigned far_min;
+};
+
+IG_DEV void se_reset(SsspEmit& S)
+{
+  if (threadIdx.x == 0) S.far_min = static_cast<unsigned>(kInf);
+  fe_reset(S.near);
+  bq_reset(S.far);
+}
+
+IG_DEV void se_park(SsspEmit& S, bool park, int u, int d, int* far, unsigned* far_cnt)
+{
+  bq_append(S.far, park, u, far, far_cnt);
+  const int m = wave_min(park ? d : kInf);
+  if (lane_id() == 0 && m != kInf) atomicMin(&S.far_min, static_cast<unsigned>(m));
+}
+
+IG_DEV void se_flush(SsspEmit& S, const FrontierQueues& out, FrontierCounters* oc, int* far, unsigned* far_cnt,
+                     unsigned* far_min)
+{
+  fe_flush(S.near, out, oc);
+  bq_flush(S.far, far, far_cnt);
+  if (threadIdx.x == 0 && S.far_min != static_cast<unsigned>(kInf)) atomic_min_relaxed(far_min, S.far_min);
+  __syncthreads();
+}
+
 IG_DEV void relax(const SsspArgs& a, int dv, int e, bool active, int T, int it_stamp, int epoch, int* far,
-                  unsigned* far_cnt, unsigned* far_min, const FrontierQueues& out, FrontierCounters* oc)
+                  unsigned* far_cnt, SsspEmit& S, const FrontierQueues& out, FrontierCounters* oc)
 {
   bool near = false, park = false;
   int u = 0, nd = 0, udeg = 0;
@@ -93,24 +127,24 @@ IG_DEV void relax(const SsspArgs& a, int dv, int e, bool active, int T, int it_s
       }
     }
   }
-  frontier_emit(near, u, udeg, out, oc);
-  wave_append(park, u, far, far_cnt);
-  const int m = wave_min(park ? nd : kInf);
-  if (lane_id() == 0 && m != kInf) atomic_min_relaxed(far_min, static_cast<unsigned>(m));
+  frontier_emit_block(S.near, near, u, udeg, out, oc);
+  se_park(S, park, u, nd, far, far_cnt);
 }
 
 __global__ __launch_bounds__(kSsspBlock) void sssp_kernel(SsspArgs a)
 {
+  __shared__ SsspEmit s_emit;
   SsspControl* ctl = a.ctl;
   if (!residency_barrier(&ctl->bar, a.start_ticks)) return;
   const long long deadline = now_ticks() + a.budget;
   const int tid = blockIdx.x * blockDim.x + threadIdx.x;
   const int stride = gridDim.x * blockDim.x;
   const int lane = lane_id();
+  if (tid == 0) ctl->trace.start = now_ticks();
   for (int i = tid; i < a.nodes; i += stride) {
-    a.dist[i] = (i == a.src) ? 0 : kInf;
-    a.stamp[i] = 0;
-    a.farstamp[i] = 0;
+    st_relaxed(&a.dist[i], (i == a.src) ? 0 : kInf);
+    st_relaxed(&a.stamp[i], 0);
+    st_relaxed(&a.farstamp[i], 0);
   }
   if (blockIdx.x == 0) {
     // delta: sampled mean weight x 32 / mean degree
@@ -147,8 +181,8 @@ __global__ __launch_bounds__(kSsspBlock) void sssp_kernel(SsspArgs a)
         double d = mean_w * 32.0 / (mean_d > 1.0 ? mean_d : 1.0);
         delta = d < 1.0 ? 1 : (d > 1e9 ? 1000000000 : static_cast<int>(d));
       }
-      ctl->delta = delta;
-      for (int k = 0; k < 3; k++) ctl->far_min[k].v = static_cast<unsigned>(kInf);
+      st_relaxed(&ctl->delta, delta);
+      for (int k = 0; k < 3; k++) st_relaxed(&ctl->far_min[k].v, static_cast<unsigned>(kInf));
     }
     __syncthreads();
     if (threadIdx.x < kWave) {
@@ -156,7 +190,7 @@ __global__ __launch_bounds__(kSsspBlock) void sssp_kernel(SsspArgs a)
       frontier_emit(threadIdx.x == 0 && deg > 0, a.src, deg, a.fq[0], &ctl->cnt[0]);
     }
   }
-  if (!grid_barrier(&ctl->bar, deadline)) {
+  if (!grid_barrier<true>(&ctl->bar, deadline)) {
     if (tid == 0) ctl->status = 1;
     return;
   }
@@ -181,7 +215,10 @@ __global__ __launch_bounds__(kSsspBlock) void sssp_kernel(SsspArgs a)
     int* far = a.far[epoch % 3];
     unsigned* far_cnt = &ctl->far_cnt[epoch % 3].v;
     unsigned* far_min = &ctl->far_min[epoch % 3].v;
-    if (static_cast<unsigned long long>(n) + nm + nh == 0) {
+    const bool split = static_cast<unsigned long long>(n) + nm + nh == 0;
+    trace_level(&ctl->trace, L, split ? ld_relaxed(far_cnt) : n + nm + nh, split ? 0ull : ld_relaxed(&ic->m),
+                split ? 3 : 0);
+    if (split) {
       // near frontier empty: move the split and re-file the far pile
       const unsigned nf = ld_relaxed(far_cnt);
       if (nf == 0) break;
@@ -199,13 +236,14 @@ __global__ __launch_bounds__(kSsspBlock) void sssp_kernel(SsspArgs a)
         st_relaxed(&ctl->far_cnt[(epoch + 2) % 3].v, 0u);
         st_relaxed(&ctl->far_min[(epoch + 2) % 3].v, static_cast<unsigned>(kInf));
       }
+      se_reset(s_emit);
       for (unsigned base = blockIdx.x * blockDim.x; base < nf; base += stride) {
         const unsigned i = base + threadIdx.x;
         bool to_near = false, keep = false;
         int u = 0, du = 0, udeg = 0;
         if (i < nf) {
-          u = far[i];
-          du = a.dist[u];
+          u = ld_relaxed(&far[i]);
+          du = ld_relaxed(&a.dist[u]);
           if (du >= T_old) {
             if (du < T) {
               to_near = __hip_atomic_exchange(&a.stamp[u], it, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != it;
@@ -215,24 +253,25 @@ __global__ __launch_bounds__(kSsspBlock) void sssp_kernel(SsspArgs a)
             }
           }
         }
-        frontier_emit(to_near, u, udeg, out, oc);
-        wave_append(keep, u, nfar, nfar_cnt);
-        const int m = wave_min(keep ? du : kInf);
-        if (lane == 0 && m != kInf) atomic_min_relaxed(nfar_min, static_cast<unsigned>(m));
+        frontier_emit_block(s_emit.near, to_near, u, udeg, out, oc);
+        se_park(s_emit, keep, u, du, nfar, nfar_cnt);
       }
+      se_flush(s_emit, out, oc, nfar, nfar_cnt, nfar_min);
       if (tid == 0) ctl->epochs++;
       epoch = nepoch;
     } else {
       it++;
+      se_reset(s_emit);
       auto visit = [&](int v, int e, bool active) {
-        const int dv = active ? a.dist[v] : 0;
-        relax(a, dv, e, active, T, it, epoch, far, far_cnt, far_min, out, oc);
+        const int dv = active ? ld_relaxed(&a.dist[v]) : 0;
+        relax(a, dv, e, active, T, it, epoch, far, far_cnt, s_emit, out, oc);
       };
       expand_huge(in, nh, nch, a.nindex, visit);
       expand_mid(in, nm, a.nindex, visit);
       expand_small(in.q, n, a.nindex, visit);
+      se_flush(s_emit, out, oc, far, far_cnt, far_min);
     }
-    if (!grid_barrier(&ctl->bar, deadline)) {
+    if (!grid_barrier<true>(&ctl->bar, deadline)) {
       if (tid == 0) ctl->status = 1;
       return;
     }
@@ -292,6 +331,7 @@ int sssp_collect(indigo_ctx* ctx, int* iterations, int* epochs)
   if (h->status != 0) return set_error(INDIGO_ERR_TIMEOUT, "sssp: persistent kernel stopped (status %d)", h->status);
   if (iterations) *iterations = h->iterations;
   if (epochs) *epochs = h->epochs;
+  store_trace(ctx, h->trace);
   return INDIGO_OK;
 }
 

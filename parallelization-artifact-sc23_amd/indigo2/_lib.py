@@ -84,6 +84,7 @@ _SIGNATURES = {
     "indigo_ctx_last_stats": [c_void_p, _PI, _PI],
     "indigo_ctx_trim": [c_void_p],
     "indigo_ctx_last_grid": [c_void_p, _PI, _PI],
+    "indigo_ctx_last_trace": [c_void_p, c_int, _PF, _PU, _PD, _PI, _PI],
     "indigo_ctx_set_hint": [c_void_p, c_int, c_int],
     "indigo_bfs": [c_void_p, c_int, c_int, c_void_p, c_void_p, c_int, c_void_p],
     "indigo_sssp": [c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p, c_int, c_void_p],

@@ -64,6 +64,18 @@ class Context:
         _lib.call("indigo_ctx_last_stats", self.h, ctypes.byref(launches), ctypes.byref(iters))
         return launches.value, iters.value
 
+    def last_trace(self) -> list[tuple[float, int, float, int]]:
+        """Per-level (start_us, frontier, degree_sum, mode) of the last BFS /
+        SSSP call; mode 0 top-down, 1 bottom-up, 2 one-workgroup, 3 split."""
+        m = 256
+        us = (ctypes.c_float * m)()
+        nf = (ctypes.c_uint * m)()
+        ds = (ctypes.c_double * m)()
+        mo = (ctypes.c_int * m)()
+        n = ctypes.c_int()
+        _lib.call("indigo_ctx_last_trace", self.h, m, us, nf, ds, mo, ctypes.byref(n))
+        return [(us[i], nf[i], ds[i], mo[i]) for i in range(n.value)]
+
     def trim(self) -> None:
         _lib.call("indigo_ctx_trim", self.h)
 
