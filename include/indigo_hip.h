@@ -153,6 +153,7 @@ int indigo_ctx_set_hint(indigo_ctx* ctx, int key, int value);
 #define INDIGO_HINT_SMALL_FRONTIER 3 /* 0 auto, 1 off: never use the one-workgroup mode */
 #define INDIGO_HINT_BUDGET_MS 4      /* bound on device-side waits in ms (0: 60000) */
 #define INDIGO_HINT_BLOCKS_PER_CU 5  /* cap on co-resident workgroups per CU of persistent kernels (0: occupancy) */
+#define INDIGO_HINT_TC_LDS_LIST 6    /* longest oriented list TC hashes in LDS (0: 16384); longer ones use binary search */
 
 /* All resident entries take the device CSR as (nodes, edges, d_nindex,
  * d_nlist[, d_eweight]) and return after the result is in device memory. */

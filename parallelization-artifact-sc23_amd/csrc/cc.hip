@@ -15,7 +15,8 @@
 //   2. link:    every undirected edge once (from its larger endpoint); a wave
 //               takes 64 vertices in id order and walks the CSR entries of
 //               their short lists (< kMidDeg) 64 at a time with the in-wave
-//               balancer; longer lists go to the mid / huge queues;
+//               balancer; longer lists go to the mid / huge queues
+//               (appends aggregated per workgroup in LDS);
 //   3. heavy:   mid lists one wave each, huge lists in kHugeChunk-entry
 //               items, one workgroup each;
 //   4. flatten: label[v] = root(v).
@@ -83,16 +84,18 @@ __global__ __launch_bounds__(kCcBlock) void cc_link(int nodes, const int* __rest
                                                     const int* __restrict__ nlist, int* parent, FrontierQueues heavy,
                                                     CcControl* ctl)
 {
+  __shared__ FrontierEmitter s_emit;
   const int lane = lane_id();
   const int wpb = blockDim.x / kWave;
   const int nchunks = (nodes + kWave - 1) / kWave;
+  fe_reset(s_emit);
   for (int c = blockIdx.x * wpb + threadIdx.x / kWave; c < nchunks; c += gridDim.x * wpb) {
     const int v = c * kWave + lane;
     const bool valid = v < nodes;
     const int beg = valid ? nindex[v] : 0;
     int deg = valid ? nindex[v + 1] - beg : 0;
     const bool long_list = deg >= kMidDeg;
-    frontier_emit(long_list, v, deg, heavy, &ctl->heavy);
+    frontier_emit_block(s_emit, long_list, v, deg, heavy, &ctl->heavy);
     if (long_list) deg = 0;
     const int incl = wave_inclusive_scan(deg);
     const int total = wave_shfl(incl, kWave - 1);
@@ -108,6 +111,7 @@ __global__ __launch_bounds__(kCcBlock) void cc_link(int nodes, const int* __rest
       }
     }
   }
+  fe_flush(s_emit, heavy, &ctl->heavy);
 }
 
 __global__ __launch_bounds__(kCcBlock) void cc_link_heavy(const int* __restrict__ nindex,

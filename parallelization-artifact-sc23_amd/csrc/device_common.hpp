@@ -64,15 +64,24 @@ IG_DEV T wave_shfl(T v, int src_lane)
   return __shfl(v, src_lane, kWave);
 }
 
-// Inclusive prefix sum over the 64 lanes (all lanes must be active).
+// Value of v in lane `lane`, which must be the same in all lanes (readlane:
+// no LDS traffic).
+IG_DEV int wave_bcast(int v, int lane) { return __builtin_amdgcn_readlane(v, lane); }
+
+// Inclusive prefix sum over the 64 lanes (all lanes must be active), with DPP
+// row operations instead of LDS permutes: Hillis-Steele within each row of 16
+// lanes (row_shr 1, 2, 4, 8), then row 0's total into row 1 and row 2's into
+// row 3 (row_bcast:15, rows 1 and 3), then lane 31's total into rows 2 and 3
+// (row_bcast:31).  A lane whose DPP source is out of its row, or whose row is
+// masked off, adds the `old` operand, 0.
 IG_DEV int wave_inclusive_scan(int x)
 {
-  const int lane = lane_id();
-#pragma unroll
-  for (int d = 1; d < kWave; d <<= 1) {
-    const int y = __shfl_up(x, d, kWave);
-    if (lane >= d) x += y;
-  }
+  x += __builtin_amdgcn_update_dpp(0, x, 0x111, 0xf, 0xf, false);  // row_shr:1
+  x += __builtin_amdgcn_update_dpp(0, x, 0x112, 0xf, 0xf, false);  // row_shr:2
+  x += __builtin_amdgcn_update_dpp(0, x, 0x114, 0xf, 0xf, false);  // row_shr:4
+  x += __builtin_amdgcn_update_dpp(0, x, 0x118, 0xf, 0xf, false);  // row_shr:8
+  x += __builtin_amdgcn_update_dpp(0, x, 0x142, 0xa, 0xf, false);  // row_bcast:15
+  x += __builtin_amdgcn_update_dpp(0, x, 0x143, 0xc, 0xf, false);  // row_bcast:31
   return x;
 }
 

@@ -28,6 +28,7 @@ HINT_SSSP_DELTA = 2
 HINT_SMALL_FRONTIER = 3
 HINT_BUDGET_MS = 4
 HINT_BLOCKS_PER_CU = 5
+HINT_TC_LDS_LIST = 6
 
 
 class IndigoError(RuntimeError):

@@ -91,6 +91,22 @@ def test_tc_rmat16(ig, orc, ctx):
     assert ctx.tc(ig.DeviceGraph(g)) == orc.tc(g)
 
 
+@pytest.mark.parametrize("lds_list", [0, 600])
+def test_tc_long_oriented_lists(ig, orc, ctx, lds_list):
+    # oriented lists of 257..2048 entries take the 512-thread workgroup pass,
+    # longer ones the 1024-thread pass; with the hint at 600 the lists above
+    # 600 entries are searched by binary search instead of the LDS table
+    ctx.set_hint(ig._lib.HINT_TC_LDS_LIST, lds_list)
+    try:
+        assert ctx.tc(ig.DeviceGraph(clique(ig, 1100))) == 1100 * 1099 * 1098 // 6
+        g = ig.erdos_renyi(3000, 900000, seed=11)
+        assert ctx.tc(ig.DeviceGraph(g)) == orc.tc(g)
+        if lds_list == 0:
+            assert ctx.tc(ig.DeviceGraph(clique(ig, 2100))) == 2100 * 2099 * 2098 // 6
+    finally:
+        ctx.set_hint(ig._lib.HINT_TC_LDS_LIST, 0)
+
+
 def test_tc_rejects_unsorted(ig, ctx):
     g = ig.grid(8, 8)
     nl = g.nlist.copy()
