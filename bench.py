@@ -35,8 +35,44 @@ def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+def tc_count_model(g) -> dict:
+    """Work of tc_count_group<512, 8192>, the kernel that does most of TC.
+
+    Restates the kernel's task split in numpy: oriented out-lists N+(v) by
+    (degree, id); each oriented edge v -> u is taken by the endpoint with the
+    longer out-list, iterating the shorter one (ties: v); the task of x is
+    its out-list plus its in-list.  Tasks with 2 <= |N+(x)| <= 256 and
+    (|N+| + |N-|) * |N+| <= 32768 run in the wave pass, the rest with
+    |N+(x)| <= 2048 in this kernel, in parts of 1024 items.  Algorithmic
+    bytes per launch: 4 per probe (the element of the iterated list), 12 per
+    item (its id and offset pair), 4 per table entry (N+(x) per part) and 16
+    per part (task header)."""
+    v = g.nodes
+    deg = g.degree.astype(np.int64)
+    src = np.repeat(np.arange(v, dtype=np.int64), deg)
+    dst = g.nlist.astype(np.int64)
+    rank = deg * v + np.arange(v, dtype=np.int64)
+    keep = rank[dst] > rank[src]
+    s, d = src[keep], dst[keep]
+    del src, dst, keep, rank
+    oc = np.bincount(s, minlength=v).astype(np.int64)
+    to_d = oc[s] < oc[d]
+    inl = np.bincount(d[to_d], minlength=v).astype(np.int64)
+    owner = np.where(to_d, d, s)
+    probes = np.bincount(owner, weights=np.where(to_d, oc[s], oc[d]), minlength=v)
+    items = oc + inl
+    wave = (oc >= 2) & (oc <= 256) & (items * oc <= 32768)
+    mid = (oc >= 2) & ~wave & (oc <= 2048)
+    parts = (items + 1023) // 1024
+    m = {"probes": float(probes[mid].sum()), "items": int(items[mid].sum()), "parts": int(parts[mid].sum()),
+         "table_entries": int((oc * parts)[mid].sum()), "probes_all_passes": float(probes.sum())}
+    m["bytes"] = 4 * m["probes"] + 12 * m["items"] + 4 * m["table_entries"] + 16 * m["parts"]
+    return m
+
+
 def tc_merge_bytes(g) -> float:
-    """Algorithmic bytes of TC under the reference's merge model:
+    """Algorithmic bytes of a whole TC call under the merge model of the
+    reference's vertex form (id order, include/tc_vertex_cuda.h:44-63):
     4E + 4V + 4 * sum_v sum_{u in N+(v)} (|N+(v) cap (u, inf)| + |N+(u)|),
     N+(x) = neighbours with a larger id."""
     deg = g.degree.astype(np.int64)
@@ -48,7 +84,23 @@ def tc_merge_bytes(g) -> float:
     return 4.0 * g.edges + 4.0 * g.nodes + 4.0 * float(pairs + fan)
 
 
-def alg_bytes(alg: str, g, pr_iters: int, tc_bytes: float) -> float:
+def copy_peak_gbs(ig, ctx) -> float:
+    """Device-to-device copy bandwidth (read + write bytes / time) of 1 GiB,
+    best of 5, timed with HIP events on the context stream."""
+    n = 1 << 30
+    a, b = ig.DeviceBuffer(n), ig.DeviceBuffer(n)
+    ctx.copy_ms(b, a, n)
+    best = max(2.0 * n / (ctx.copy_ms(b, a, n) * 1e-3) / 1e9 for _ in range(5))
+    del a, b
+    return best
+
+
+def kernel_bytes(alg: str, g, pr_iters: int, tc_model: dict) -> float:
+    """Algorithmic HBM bytes of each algorithm's main kernel per launch: every
+    CSR entry read once (4 B) plus the per-entry lookup it drives (4 B: the
+    neighbour's distance / parent / status / contribution; SSSP also the
+    weight), and the per-vertex index pair and result (8 B; PR reads the
+    index pair, scores and writes scores and contributions, 16 B, per sweep)."""
     e, v = float(g.edges), float(g.nodes)
     if alg in ("bfs", "cc", "mis"):
         return 8 * e + 8 * v
@@ -56,7 +108,59 @@ def alg_bytes(alg: str, g, pr_iters: int, tc_bytes: float) -> float:
         return 12 * e + 8 * v
     if alg == "pr":
         return pr_iters * (8 * e + 16 * v)
-    return tc_bytes
+    return tc_model["bytes"]
+
+
+def read_traffic(path: str, workload: str, kernel: str) -> dict | None:
+    """HBM bytes per launch of `kernel` measured with rocprofv3 --pmc
+    (FETCH_SIZE, WRITE_SIZE) on this workload, scaled by the same counters on
+    a copy of known size; recorded by the profiling commands in the file."""
+    try:
+        with open(path) as f:
+            rec = json.load(f)
+    except (OSError, ValueError):
+        return None
+    if rec.get("workload") != workload or rec.get("kernel") != kernel:
+        return None
+    return rec
+
+
+def _pmc_values(path: str, counter: str, match) -> list[float]:
+    import csv
+
+    with open(path) as f:
+        return [float(r["Counter_Value"]) for r in csv.DictReader(f)
+                if r["Counter_Name"] == counter and match(r["Kernel_Name"])]
+
+
+def record_traffic(paths: list[str], workload: str, kernel: str, out: str) -> int:
+    """Writes the traffic record from four rocprofv3 --pmc CSVs: FETCH_SIZE and
+    WRITE_SIZE (KiB per dispatch) of `kernel` on the workload, then of the
+    1 GiB copy of --copy-calibration.  Each counter is scaled by what it reads
+    for the copy, which moves exactly 1 GiB each way."""
+    kf, kw, cf, cw = paths
+    short = kernel.split("<")[0]
+    is_kernel = lambda name: short in name and (kernel == short or kernel.split("<")[1].rstrip(">") in name)
+    is_copy = lambda name: "copy" in name.lower() or "elementwise" in name.lower()
+    fetch, write = _pmc_values(kf, "FETCH_SIZE", is_kernel), _pmc_values(kw, "WRITE_SIZE", is_kernel)
+    cfetch = [x for x in _pmc_values(cf, "FETCH_SIZE", is_copy) if x > 1e5]
+    cwrite = [x for x in _pmc_values(cw, "WRITE_SIZE", is_copy) if x > 1e5]
+    if not (fetch and write and cfetch and cwrite):
+        print("record-traffic: missing dispatches in the CSVs", file=sys.stderr)
+        return 1
+    gib_kib = float(1 << 20)
+    ffac, wfac = gib_kib / statistics.median(cfetch), gib_kib / statistics.median(cwrite)
+    rb = 1024.0 * statistics.mean(fetch) * ffac
+    wb = 1024.0 * statistics.mean(write) * wfac
+    rec = {"workload": workload, "kernel": kernel, "bytes_per_launch": rb + wb, "read_bytes": rb, "write_bytes": wb,
+           "fetch_kib_per_dispatch": statistics.mean(fetch), "write_kib_per_dispatch": statistics.mean(write),
+           "copy_scale": {"fetch": ffac, "write": wfac}, "dispatches": [len(fetch), len(write)],
+           "source": "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, python3 bench.py --algs tc --steps 2 --warmup 0 "
+                     "--no-cpu-baseline, scaled by a 1 GiB copy (bench.py --copy-calibration)"}
+    with open(out, "w") as f:
+        json.dump(rec, f, indent=1)
+    print(json.dumps(rec))
+    return 0
 
 
 def cpu_baseline(scale: int, threads: int) -> dict | None:
@@ -102,12 +206,38 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--scale", type=int, default=22)
     ap.add_argument("--edge-factor", type=int, default=16)
-    ap.add_argument("--cpu-scale", type=int, default=18)
+    ap.add_argument("--cpu-scale", type=int, default=20)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--algs", default=",".join(ALGS), help="subset to run (profiling); the JSON line needs all six")
+    ap.add_argument("--profile-only", action="store_true",
+                    help="run the timed steps and stop (no TC model, no JSON): the command profiled by rocprofv3")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a 1 GiB device-to-device copy 5 times (the PMC scale reference) and exit")
+    ap.add_argument("--record-traffic", nargs=4, metavar=("KFETCH", "KWRITE", "CFETCH", "CWRITE"),
+                    help="write profiles/traffic_<workload>.json for the TC main kernel from four PMC CSVs")
     args = ap.parse_args()
+    if args.record_traffic:
+        workload = f"rmat{args.scale} ef{args.edge_factor}"
+        return record_traffic(args.record_traffic, workload, "tc_count_group<512, 8192>",
+                              os.path.join(REPO, "profiles", f"traffic_{workload.replace(' ', '_')}.json"))
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: every kernel here runs on a single MI355X "
               "(multi-GPU sharding is design-only); run with --gpus 1", file=sys.stderr)
+        return 2
+
+    if args.copy_calibration:
+        import indigo2 as ig
+
+        ctx = ig.Context(0)
+        n = 1 << 30
+        a, b = ig.DeviceBuffer(n), ig.DeviceBuffer(n)
+        for _ in range(5):
+            ctx.copy_ms(b, a, n)
+        log("copied 5 x 1 GiB device to device")
+        return 0
+    algs = tuple(a for a in args.algs.split(",") if a)
+    if not set(algs) <= set(ALGS):
+        print(f"unknown algorithm in --algs {args.algs}", file=sys.stderr)
         return 2
 
     import indigo2 as ig
@@ -137,39 +267,53 @@ def main() -> int:
         return None
 
     for w in range(args.warmup):
-        for a in ALGS:
+        for a in algs:
             run(a)
         log(f"warmup step {w + 1}/{args.warmup} done")
 
-    ms = {a: [] for a in ALGS}
+    ms = {a: [] for a in algs}
+    kms = {a: [] for a in algs}
+    kname = {}
     pr_iters = 0
     tri = 0
     t_start = time.perf_counter()
     for s in range(args.steps):
-        for a in ALGS:
+        for a in algs:
             r = run(a)
             ms[a].append(ctx.last_ms)
+            kname[a], k = ctx.last_kernel
+            kms[a].append(k)
             if a == "pr":
                 pr_iters = r
             elif a == "tc":
                 tri = r
-        log(f"step {s + 1}/{args.steps}: " + " ".join(f"{a}={ms[a][-1]:.3f}ms" for a in ALGS))
+        log(f"step {s + 1}/{args.steps}: " + " ".join(f"{a}={ms[a][-1]:.3f}ms" for a in algs))
     wall = time.perf_counter() - t_start
+    if algs != ALGS or args.profile_only:
+        log("profiling run: no JSON line")
+        return 0
     ms_per_step = 1e3 * wall / args.steps
     value = 6 * 1e-9 * g.edges / (ms_per_step * 1e-3)
 
     med = {a: statistics.median(ms[a]) for a in ALGS}
+    kmed = {a: statistics.mean(kms[a]) for a in ALGS}
     per_gteps = {a: round(1e-9 * g.edges / (med[a] * 1e-3), 4) for a in ALGS}
-    tc_bytes = tc_merge_bytes(g)
+    log("modelling TC work")
+    tcm = tc_count_model(g)
+    copy_gbs = copy_peak_gbs(ig, ctx)
     roof = {}
     for a in ALGS:
-        b = alg_bytes(a, g, pr_iters, tc_bytes)
-        gbs = b / (med[a] * 1e-3) / 1e9
-        roof[a] = {"bytes": b, "ms": round(med[a], 4), "achieved_gbs": round(gbs, 1),
-                   "frac": round(gbs / HBM_PEAK_GBS, 4)}
+        b = kernel_bytes(a, g, pr_iters, tcm)
+        gbs = b / (kmed[a] * 1e-3) / 1e9
+        # the survey's per-algorithm model over the whole call (all launches)
+        b_call = tc_merge_bytes(g) if a == "tc" else b
+        roof[a] = {"kernel": kname[a], "kernel_ms": round(kmed[a], 4), "bytes": b, "achieved_gbs": round(gbs, 1),
+                   "frac": round(gbs / HBM_PEAK_GBS, 4), "call_ms": round(med[a], 4), "call_model_bytes": b_call,
+                   "call_frac": round(b_call / (med[a] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
     dom = max(ALGS, key=lambda a: med[a])
-    kernel_of = {"bfs": "bfs_kernel", "sssp": "sssp_kernel", "cc": "cc_link", "mis": "mis_kernel",
-                 "pr": "pr_kernel", "tc": "tc_count"}
+    workload = f"rmat{args.scale} ef{args.edge_factor}"
+    traffic = read_traffic(os.path.join(REPO, "profiles", f"traffic_{workload.replace(' ', '_')}.json"), workload,
+                           kname[dom])
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -205,13 +349,19 @@ def main() -> int:
         "triangles": tri,
         "roofline": {
             "bound": "hbm",
-            "kernel": kernel_of[dom],
+            "kernel": kname[dom],
             "algorithm": dom,
             "achieved": roof[dom]["achieved_gbs"],
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": roof[dom]["frac"],
-            "traffic": None,
+            "traffic": traffic["bytes_per_launch"] if traffic else None,
+            "algorithmic_bytes": roof[dom]["bytes"],
+            "kernel_ms": roof[dom]["kernel_ms"],
+            "traffic_source": traffic.get("source") if traffic else None,
+            "copy_peak_gbs": round(copy_gbs, 1),
+            "frac_of_copy_peak": round(roof[dom]["achieved_gbs"] / copy_gbs, 4),
+            "tc_model": tcm,
             "per_algorithm": roof,
         },
         "cpu_baseline": cpu,

@@ -135,7 +135,17 @@ int indigo_ctx_last_ms(indigo_ctx* ctx, float* ms);
 /* Number of kernel launches and iterations (levels, rounds, sweeps) of the
  * last resident call. */
 int indigo_ctx_last_stats(indigo_ctx* ctx, int* launches, int* iterations);
-/* Workgroups of the last cooperative (persistent) launch and the per-CU
+/* Duration in milliseconds (HIP events on the context stream, around that one
+ * launch) and name of the main kernel of the last resident call: bfs_kernel,
+ * sssp_kernel, cc_link, mis_kernel, pr_kernel, or tc_count_group<512, 8192>.
+ * name (name_len bytes, may be NULL) receives "" and ms the whole call when
+ * the call marked no kernel. */
+int indigo_ctx_last_kernel(indigo_ctx* ctx, float* ms, char* name, int name_len);
+/* Device-to-device copy of `bytes` on the context stream (hipMemcpyAsync),
+ * timed with the context's events: the copy-bandwidth reference beside the
+ * kernels' roofline figures.  No counterpart in the reference. */
+int indigo_ctx_copy_ms(indigo_ctx* ctx, void* d_dst, const void* d_src, size_t bytes, float* ms);
+/* Workgroups of the last persistent launch and the per-CU
  * residency it was sized for. */
 int indigo_ctx_last_grid(indigo_ctx* ctx, int* grid, int* per_cu);
 /* Per-level trace of the last BFS or SSSP call (first 256 levels): start of

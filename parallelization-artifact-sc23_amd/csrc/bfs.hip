@@ -5,7 +5,7 @@
 // unreachable; the result equals CPUserialDijkstra with unit weights
 // (include/bfs_vertex_cuda.h:75-99).
 //
-// Design: one cooperative persistent kernel runs every level; levels are
+// Design: one persistent kernel runs every level; levels are
 // separated by the bounded grid barrier (coherent mode: dist, queues and
 // counters are only touched with device-scope atomic accesses), so the host
 // never round-trips per
@@ -387,7 +387,7 @@ int bfs_run(indigo_ctx* ctx, int nodes, int edges, const int* nindex, const int*
   IG_TRY(zero_async(ctx, a.ctl, sizeof(BfsControl)));
   void* args[] = {&a};
   int grid = 0;
-  IG_TRY(coop_launch(ctx, reinterpret_cast<const void*>(bfs_kernel), kBfsBlock, 0, args, 0, &grid));
+  IG_TRY(coop_launch(ctx, reinterpret_cast<const void*>(bfs_kernel), kBfsBlock, 0, args, 0, &grid, "bfs_kernel"));
   if (stats) {
     stats->launches = 2;
     stats->grid = grid;

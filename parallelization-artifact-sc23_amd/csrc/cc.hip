@@ -160,7 +160,9 @@ int cc_run(indigo_ctx* ctx, int nodes, int edges, const int* nindex, const int* 
   const int gv = grid_for(nodes, kCcBlock, 8192);
   cc_init<<<gv, kCcBlock, 0, ctx->stream>>>(nodes, nindex, nlist, parent);
   const int gl = grid_for((nodes + kWave - 1) / kWave, kCcBlock / kWave, 1 << 16);
+  IG_TRY(kernel_begin(ctx, "cc_link"));
   cc_link<<<gl, kCcBlock, 0, ctx->stream>>>(nodes, nindex, nlist, parent, heavy, ctl);
+  IG_TRY(kernel_end(ctx));
   cc_link_heavy<<<ctx->compute_units * 8, kCcBlock, 0, ctx->stream>>>(nindex, nlist, parent, heavy, ctl);
   cc_flatten<<<gv, kCcBlock, 0, ctx->stream>>>(nodes, parent, label);
   IG_HIP_TRY(hipGetLastError());

@@ -1,6 +1,6 @@
 // Device-side building blocks shared by the gfx950 graph kernels:
 // relaxed agent-scope atomics, 64-lane wave scans and aggregated appends,
-// and a bounded grid barrier for cooperative (co-resident) launches.
+// and a bounded grid barrier for persistent (co-resident) launches.
 #pragma once
 
 #include <hip/hip_runtime.h>

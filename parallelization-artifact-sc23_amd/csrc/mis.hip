@@ -9,7 +9,7 @@
 // suite's C++ path orders priorities as unsigned int, its CUDA path as signed
 // int (data_type), selected here by `signed_priority`.
 //
-// Design: one cooperative persistent kernel (coherent-mode barriers: status,
+// Design: one persistent kernel (coherent-mode barriers: status,
 // worklists and counters only see device-scope atomic accesses); round r
 // processes the vertices
 // still undecided after round r-1, kept in two worklists by list length:
@@ -237,7 +237,7 @@ int mis_run(indigo_ctx* ctx, int nodes, const int* nindex, const int* nlist, int
   a.start_ticks = residency_ticks(ctx);
   IG_TRY(zero_async(ctx, a.ctl, sizeof(MisControl)));
   void* args[] = {&a};
-  IG_TRY(coop_launch(ctx, reinterpret_cast<const void*>(mis_kernel), kMisBlock, 0, args, 0, nullptr));
+  IG_TRY(coop_launch(ctx, reinterpret_cast<const void*>(mis_kernel), kMisBlock, 0, args, 0, nullptr, "mis_kernel"));
   return INDIGO_OK;
 }
 

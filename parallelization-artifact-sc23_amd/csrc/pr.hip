@@ -6,7 +6,7 @@
 // s'[v] = (1 - 0.85) / N + 0.85 * sum_{u in N(v)} contrib[u]; repeat while the
 // L1 change of the iteration is > 1e-4 and fewer than 100 iterations ran.
 //
-// Design: one cooperative persistent kernel; per iteration
+// Design: one persistent kernel; per iteration
 //   phase A  every vertex with a short list (< kPrWaveDeg entries) is summed
 //            by one lane in vertex order (coalesced score / contrib / index
 //            traffic); longer lists, collected once into a static list, one
@@ -318,7 +318,7 @@ int pr_run(indigo_ctx* ctx, int nodes, int edges, const int* nindex, const int* 
   IG_TRY(zero_async(ctx, a.ctl, sizeof(PrControl)));
   IG_TRY(zero_async(ctx, a.exit_it, sizeof(int) * kPrMaxGrid));
   void* args[] = {&a};
-  IG_TRY(coop_launch(ctx, reinterpret_cast<const void*>(pr_kernel), kPrBlock, 0, args, kPrMaxGrid, nullptr));
+  IG_TRY(coop_launch(ctx, reinterpret_cast<const void*>(pr_kernel), kPrBlock, 0, args, kPrMaxGrid, nullptr, "pr_kernel"));
   return INDIGO_OK;
 }
 

@@ -1,5 +1,7 @@
-// Context management, workspace, cooperative launch and device plumbing.
+// Context management, workspace, persistent launches and device plumbing.
 #include "runtime.hpp"
+
+#include <cstdio>
 
 #include <algorithm>
 #include <cstring>
@@ -65,7 +67,7 @@ std::unordered_map<const void*, int> g_cap;  // learned co-residency per kernel
 }  // namespace
 
 int coop_launch(indigo_ctx* ctx, const void* kernel, int block, size_t shmem, void** args, int max_blocks,
-                int* grid_out)
+                int* grid_out, const char* name)
 {
   int per_cu = 0;
   IG_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, block, shmem));
@@ -79,8 +81,9 @@ int coop_launch(indigo_ctx* ctx, const void* kernel, int block, size_t shmem, vo
   int grid = per_cu * ctx->compute_units;
   if (max_blocks > 0) grid = std::min(grid, max_blocks);
   grid = std::max(grid, 1);
-  IG_HIP_TRY(hipLaunchCooperativeKernel(kernel, dim3(grid), dim3(block), args, static_cast<unsigned>(shmem),
-                                        ctx->stream));
+  if (name != nullptr) IG_TRY(kernel_begin(ctx, name));
+  IG_HIP_TRY(hipLaunchKernel(kernel, dim3(grid), dim3(block), args, static_cast<unsigned>(shmem), ctx->stream));
+  if (name != nullptr) IG_TRY(kernel_end(ctx));
   ctx->last_grid = grid;
   ctx->last_per_cu = per_cu;
   if (grid_out) *grid_out = grid;
@@ -125,7 +128,21 @@ int check_graph_args(int nodes, int edges, const void* nindex, const void* nlist
 
 int timing_begin(indigo_ctx* ctx)
 {
+  ctx->kernel_name = nullptr;
   IG_HIP_TRY(hipEventRecord(ctx->ev_start, ctx->stream));
+  return INDIGO_OK;
+}
+
+int kernel_begin(indigo_ctx* ctx, const char* name)
+{
+  IG_HIP_TRY(hipEventRecord(ctx->ev_k0, ctx->stream));
+  ctx->kernel_name = name;
+  return INDIGO_OK;
+}
+
+int kernel_end(indigo_ctx* ctx)
+{
+  IG_HIP_TRY(hipEventRecord(ctx->ev_k1, ctx->stream));
   return INDIGO_OK;
 }
 
@@ -136,6 +153,15 @@ int timing_end(indigo_ctx* ctx, int launches, int iterations)
   float ms = 0.f;
   IG_HIP_TRY(hipEventElapsedTime(&ms, ctx->ev_start, ctx->ev_stop));
   ctx->last_ms = ms;
+  if (ctx->kernel_name != nullptr) {
+    float kms = 0.f;
+    IG_HIP_TRY(hipEventElapsedTime(&kms, ctx->ev_k0, ctx->ev_k1));
+    ctx->last_kernel_ms = kms;
+    ctx->last_kernel = ctx->kernel_name;
+  } else {
+    ctx->last_kernel_ms = ms;
+    ctx->last_kernel = "";
+  }
   ctx->last_launches = launches;
   ctx->last_iterations = iterations;
   return INDIGO_OK;
@@ -155,15 +181,14 @@ static int ctx_init(int device, indigo_ctx* c)
   IG_HIP_TRY(hipGetDeviceProperties(&prop, device));
   c->compute_units = prop.multiProcessorCount;
   c->max_threads_per_cu = prop.maxThreadsPerMultiProcessor;
-  int coop = 0;
-  IG_HIP_TRY(hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, device));
-  if (!coop) return set_error(INDIGO_ERR_NO_DEVICE, "device %d lacks cooperative launch", device);
   int wall_khz = 0;
   if (hipDeviceGetAttribute(&wall_khz, hipDeviceAttributeWallClockRate, device) == hipSuccess && wall_khz > 0)
     c->wall_hz = static_cast<long long>(wall_khz) * 1000;
   IG_HIP_TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
   IG_HIP_TRY(hipEventCreate(&c->ev_start));
   IG_HIP_TRY(hipEventCreate(&c->ev_stop));
+  IG_HIP_TRY(hipEventCreate(&c->ev_k0));
+  IG_HIP_TRY(hipEventCreate(&c->ev_k1));
   return INDIGO_OK;
 }
 
@@ -278,6 +303,8 @@ int indigo_ctx_destroy(indigo_ctx* ctx)
   if (ctx->host_pinned) (void)hipHostFree(ctx->host_pinned);
   if (ctx->ev_start) (void)hipEventDestroy(ctx->ev_start);
   if (ctx->ev_stop) (void)hipEventDestroy(ctx->ev_stop);
+  if (ctx->ev_k0) (void)hipEventDestroy(ctx->ev_k0);
+  if (ctx->ev_k1) (void)hipEventDestroy(ctx->ev_k1);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
   return INDIGO_OK;
@@ -295,6 +322,29 @@ int indigo_ctx_last_stats(indigo_ctx* ctx, int* launches, int* iterations)
   if (ctx == nullptr) return set_error(INDIGO_ERR_INVALID_ARG, "ctx_last_stats: null ctx");
   if (launches) *launches = ctx->last_launches;
   if (iterations) *iterations = ctx->last_iterations;
+  return INDIGO_OK;
+}
+
+int indigo_ctx_last_kernel(indigo_ctx* ctx, float* ms, char* name, int name_len)
+{
+  if (ctx == nullptr || ms == nullptr) return set_error(INDIGO_ERR_INVALID_ARG, "ctx_last_kernel: null argument");
+  *ms = ctx->last_kernel_ms;
+  if (name != nullptr && name_len > 0) {
+    std::snprintf(name, static_cast<size_t>(name_len), "%s", ctx->last_kernel);
+  }
+  return INDIGO_OK;
+}
+
+int indigo_ctx_copy_ms(indigo_ctx* ctx, void* d_dst, const void* d_src, size_t bytes, float* ms)
+{
+  if (ctx == nullptr || ms == nullptr || (bytes > 0 && (d_dst == nullptr || d_src == nullptr)))
+    return set_error(INDIGO_ERR_INVALID_ARG, "ctx_copy_ms: null argument");
+  IG_HIP_TRY(hipSetDevice(ctx->device));
+  IG_HIP_TRY(hipEventRecord(ctx->ev_k0, ctx->stream));
+  IG_HIP_TRY(hipMemcpyAsync(d_dst, d_src, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+  IG_HIP_TRY(hipEventRecord(ctx->ev_k1, ctx->stream));
+  IG_HIP_TRY(hipEventSynchronize(ctx->ev_k1));
+  IG_HIP_TRY(hipEventElapsedTime(ms, ctx->ev_k0, ctx->ev_k1));
   return INDIGO_OK;
 }
 

@@ -1,5 +1,5 @@
 // Host-side runtime shared by the kernels' launchers: the context object
-// behind indigo_ctx, workspace carving, HIP error mapping and cooperative
+// behind indigo_ctx, workspace carving, HIP error mapping and persistent
 // (persistent) launches.
 #pragma once
 
@@ -18,6 +18,11 @@ struct indigo_ctx {
   long long wall_hz = 100000000;  // wall_clock64() frequency
   hipStream_t stream = nullptr;
   hipEvent_t ev_start = nullptr, ev_stop = nullptr;
+  // events around the main kernel launch of the last call (indigo_ctx_last_kernel)
+  hipEvent_t ev_k0 = nullptr, ev_k1 = nullptr;
+  const char* kernel_name = nullptr;  // set by kernel_begin, a string literal
+  float last_kernel_ms = 0.f;
+  const char* last_kernel = "";
   void* ws = nullptr;      // cached device workspace
   size_t ws_bytes = 0;
   void* host_pinned = nullptr;  // small pinned staging buffer (readbacks)
@@ -34,7 +39,7 @@ struct indigo_ctx {
   unsigned trace_nf[256] = {0};
   double trace_m[256] = {0};
   int trace_mode[256] = {0};
-  int last_grid = 0;    // workgroups of the last cooperative launch
+  int last_grid = 0;    // workgroups of the last persistent launch
   int last_per_cu = 0;  // co-resident workgroups per CU it was sized for
 };
 
@@ -81,12 +86,19 @@ inline long long budget_ticks(const indigo_ctx* ctx)
   return static_cast<long long>(sec * static_cast<double>(ctx->wall_hz));
 }
 
-// Launches `kernel` cooperatively with as many workgroups of `block` threads
+// Launches a persistent `kernel` with as many workgroups of `block` threads
 // as can be co-resident: occupancy x compute units, capped at max_blocks and
-// by the per-kernel residency cap learned from earlier launches.
+// by the per-kernel residency cap learned from earlier launches.  It is a
+// plain launch: the kernels' grid barrier is software (device atomics), and
+// co-residency is checked by the kernel itself (residency_barrier), which
+// aborts a grid that does not assemble so the host relaunches it smaller.
+// (A cooperative launch would add nothing, and on ROCm it fails while
+// another process holds a context on the same GPU.)
+// `name` (a string literal, or null) marks the launch as the call's main
+// kernel, timed by kernel_begin / kernel_end.
 int coop_launch(indigo_ctx* ctx, const void* kernel, int block, size_t shmem, void** args, int max_blocks,
-                int* grid_out);
-// Internal status: the last cooperative launch did not assemble (see
+                int* grid_out, const char* name = nullptr);
+// Internal status: the last persistent launch did not assemble (see
 // residency_barrier); the cap of `kernel` has been lowered, launch again.
 constexpr int kNotResident = -100;
 // Checks a finished persistent kernel's barrier abort code: 0 ok,
@@ -104,6 +116,10 @@ int check_graph_args(int nodes, int edges, const void* nindex, const void* nlist
 
 // Starts / stops the HIP-event timing of a resident call.
 int timing_begin(indigo_ctx* ctx);
+// Events around the main kernel of a call (the last marked launch wins);
+// `name` must be a string literal.
+int kernel_begin(indigo_ctx* ctx, const char* name);
+int kernel_end(indigo_ctx* ctx);
 int timing_end(indigo_ctx* ctx, int launches, int iterations);
 
 // Process-wide default context for the drop-in entries (created lazily on

@@ -5,7 +5,7 @@
 // relaxations d = dist[v] + w in int arithmetic; results equal the serial
 // Dijkstra of sssp_vertex_cuda.h:74-99.
 //
-// Design: one cooperative persistent kernel (coherent-mode barriers: dist,
+// Design: one persistent kernel (coherent-mode barriers: dist,
 // stamps, queues, piles and counters only see device-scope atomic accesses)
 // running near-far
 // label-correcting SSSP (Davidson et al., IPDPS'14):
@@ -315,7 +315,7 @@ int sssp_run(indigo_ctx* ctx, int nodes, int edges, const int* nindex, const int
   a.start_ticks = residency_ticks(ctx);
   IG_TRY(zero_async(ctx, a.ctl, sizeof(SsspControl)));
   void* args[] = {&a};
-  IG_TRY(coop_launch(ctx, reinterpret_cast<const void*>(sssp_kernel), kSsspBlock, 0, args, 0, nullptr));
+  IG_TRY(coop_launch(ctx, reinterpret_cast<const void*>(sssp_kernel), kSsspBlock, 0, args, 0, nullptr, "sssp_kernel"));
   return INDIGO_OK;
 }
 

@@ -60,6 +60,7 @@ constexpr int kMidList = 2048;
 constexpr int kMidSlots = 8192;
 constexpr int kBigBlock = 1024;
 constexpr int kBigSlots = 32768;
+static_assert(kMidBlock == 512 && kMidSlots == 8192, "the main-kernel name below spells these out");
 constexpr int kWaveWork = 1 << 15;  // probe bound of a wave-pass task
 constexpr int kPartItems = 1024;    // items per workgroup-pass part
 constexpr int kUnroll = 4;          // windows of 64 per step over a long list
@@ -511,8 +512,10 @@ int tc_run(indigo_ctx* ctx, int nodes, int edges, const int* nindex, const int* 
   const int hint = ctx->hints[INDIGO_HINT_TC_LDS_LIST];
   const int mid_list = hint > 0 && hint < kMidSlots / 4 ? hint : kMidSlots / 4;
   const int big_list = hint > 0 && hint < kBigSlots / 4 ? hint : kBigSlots / 4;
+  IG_TRY(kernel_begin(ctx, "tc_count_group<512, 8192>"));
   tc_count_group<kMidBlock, kMidSlots><<<ctx->compute_units * 4, kMidBlock, 0, s>>>(
       oindex, olist, iindex, ilist, mid, &ctl->nmid, &ctl->next_mid, mid_list, ctl);
+  IG_TRY(kernel_end(ctx));
   tc_count_group<kBigBlock, kBigSlots><<<ctx->compute_units, kBigBlock, 0, s>>>(
       oindex, olist, iindex, ilist, big, &ctl->nbig, &ctl->next_big, big_list, ctl);
   IG_HIP_TRY(hipGetLastError());

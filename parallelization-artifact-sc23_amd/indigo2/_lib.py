@@ -85,6 +85,8 @@ _SIGNATURES = {
     "indigo_ctx_last_stats": [c_void_p, _PI, _PI],
     "indigo_ctx_trim": [c_void_p],
     "indigo_ctx_last_grid": [c_void_p, _PI, _PI],
+    "indigo_ctx_last_kernel": [c_void_p, _PF, ctypes.c_char_p, c_int],
+    "indigo_ctx_copy_ms": [c_void_p, c_void_p, c_void_p, c_size_t, _PF],
     "indigo_ctx_last_trace": [c_void_p, c_int, _PF, _PU, _PD, _PI, _PI],
     "indigo_ctx_set_hint": [c_void_p, c_int, c_int],
     "indigo_bfs": [c_void_p, c_int, c_int, c_void_p, c_void_p, c_int, c_void_p],

@@ -64,6 +64,20 @@ class Context:
         _lib.call("indigo_ctx_last_stats", self.h, ctypes.byref(launches), ctypes.byref(iters))
         return launches.value, iters.value
 
+    @property
+    def last_kernel(self) -> tuple[str, float]:
+        """(name, ms) of the main kernel launch of the last call."""
+        ms = ctypes.c_float()
+        buf = ctypes.create_string_buffer(64)
+        _lib.call("indigo_ctx_last_kernel", self.h, ctypes.byref(ms), buf, 64)
+        return buf.value.decode(), ms.value
+
+    def copy_ms(self, dst: DeviceBuffer, src: DeviceBuffer, nbytes: int) -> float:
+        """Device-to-device copy on the context stream, timed with its events."""
+        ms = ctypes.c_float()
+        _lib.call("indigo_ctx_copy_ms", self.h, dst.ptr, src.ptr, nbytes, ctypes.byref(ms))
+        return ms.value
+
     def last_trace(self) -> list[tuple[float, int, float, int]]:
         """Per-level (start_us, frontier, degree_sum, mode) of the last BFS /
         SSSP call; mode 0 top-down, 1 bottom-up, 2 one-workgroup, 3 split."""

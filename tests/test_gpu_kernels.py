@@ -146,3 +146,12 @@ def test_dropin_entries(ig, orc):
     assert cnt == orc.tc(g)
     cnt, _ = ig.GPUtc_edge(dg, d_sp)
     assert cnt == orc.tc(g)
+
+
+def test_copy_ms_copies_and_times(ig, ctx):
+    src = np.arange(1 << 20, dtype=np.int32)
+    a = ig.DeviceBuffer.from_array(src)
+    b = ig.DeviceBuffer(src.nbytes)
+    ms = ctx.copy_ms(b, a, src.nbytes)
+    assert ms > 0
+    assert np.array_equal(b.to_array(np.int32, src.size), src)
