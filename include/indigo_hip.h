@@ -180,6 +180,42 @@ int indigo_pagerank(indigo_ctx* ctx, int nodes, int edges, const int* d_nindex, 
 int indigo_tc(indigo_ctx* ctx, int nodes, int edges, const int* d_nindex, const int* d_nlist,
               long long* count);
 
+/* ---- batched engine ------------------------------------------------------- */
+/* Many graphs of uneven sizes and many sources per graph in ONE persistent
+ * launch fed by a device-side task queue (no counterpart in the reference:
+ * its harnesses run one query per process).  Results are bit-identical to
+ * the resident single-query entries above. */
+#define INDIGO_QUERY_BFS 1  /* multi-source BFS: a 64-bit visited mask per vertex */
+#define INDIGO_QUERY_SSSP 2 /* multi-source SSSP: a per-source distance slab */
+#define INDIGO_QUERY_CC 3
+#define INDIGO_QUERY_MIS 4  /* unsigned priority order (the C++ path) */
+#define INDIGO_MAX_BFS_SOURCES 64
+#define INDIGO_MAX_SSSP_SOURCES 32
+
+typedef struct indigo_graph_ref {
+  int nodes, edges;
+  const int* d_nindex; /* device int[nodes + 1] */
+  const int* d_nlist;  /* device int[edges] */
+  const int* d_eweight; /* device int[edges]; needed by SSSP queries only */
+} indigo_graph_ref;
+
+typedef struct indigo_query {
+  int alg;   /* INDIGO_QUERY_* */
+  int graph; /* index into the graph table */
+  int nsrc;  /* BFS 1..64, SSSP 1..32 sources; ignored for CC / MIS */
+  int src[INDIGO_MAX_BFS_SOURCES];
+  /* device output: BFS / SSSP int32[nsrc][nodes], source-major (row i is the
+   * distance array of src[i], INT_MAX = unreachable); CC label[nodes]
+   * (smallest id of the component); MIS status[nodes] (1 in, 2 out) */
+  int* d_out;
+} indigo_query;
+
+/* Runs all queries and returns when every result is in device memory.
+ * max_workgroups caps the persistent grid (0: the whole GPU), so that several
+ * batches can share the GPU on different contexts. */
+int indigo_batch(indigo_ctx* ctx, const indigo_graph_ref* graphs, int ngraphs, const indigo_query* queries,
+                 int nqueries, int max_workgroups);
+
 #ifdef __cplusplus
 }
 #endif

@@ -4,6 +4,7 @@
 #include <chrono>
 #include <climits>
 
+#include "batch.hpp"
 #include "bfs.hpp"
 #include "cc.hpp"
 #include "device_common.hpp"
@@ -82,7 +83,7 @@ int bfs_device(indigo_ctx* ctx, int nodes, int edges, const int* nindex, const i
   IG_HIP_TRY(hipStreamSynchronize(ctx->stream));
   const double t0 = wall_seconds();
   BfsStats st;
-  for (int attempt = 0;; attempt++) {
+  for (;;) {
     IG_TRY(timing_begin(ctx));
     IG_TRY(bfs_run(ctx, nodes, edges, nindex, nlist, src, dist, &st));
     IG_TRY(timing_end(ctx, st.launches, 0));
@@ -400,6 +401,13 @@ int indigo_tc_edge(long long* count, int edges, const int* d_nindex, const int* 
   IG_TRY(read_device_int(d_sp + edges - 1, &last));
   const int nodes = last + 1;
   return tc_device(ctx, nodes, edges, d_nindex, d_nlist, count, seconds);
+}
+
+int indigo_batch(indigo_ctx* ctx, const indigo_graph_ref* graphs, int ngraphs, const indigo_query* queries,
+                 int nqueries, int max_workgroups)
+{
+  IG_TRY(ctx_prologue(ctx));
+  return batch_run(ctx, graphs, ngraphs, queries, nqueries, max_workgroups);
 }
 
 }  // extern "C"

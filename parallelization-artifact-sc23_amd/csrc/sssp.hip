@@ -199,9 +199,6 @@ __global__ __launch_bounds__(kSsspBlock) void sssp_kernel(SsspArgs a)
   int epoch = 1;  // far pile generation (farstamp value); far[epoch % 3] is current
   int it = 0;     // near iterations (stamp values start at 1)
   int L = 0;      // counter rotation index
-  const int wpb = kSsspBlock / kWave;
-  const unsigned gwave = blockIdx.x * wpb + threadIdx.x / kWave;
-  const unsigned nwaves = gridDim.x * wpb;
   for (;;) {
     FrontierCounters* ic = &ctl->cnt[L % 3];
     const unsigned n = ld_relaxed(&ic->n);

@@ -14,6 +14,7 @@ from .kernels import (
     INF,
     UNDECIDED,
     Context,
+    Query,
     GPUbfs_edge,
     GPUbfs_vertex,
     GPUcc_edge,
@@ -29,7 +30,7 @@ from .kernels import (
 
 __all__ = [
     "_lib", "IndigoError", "load", "DeviceBuffer", "DeviceGraph", "HostGraph", "erdos_renyi", "from_edges",
-    "grid", "read_egr", "rmat", "road", "EXCLUDED", "INCLUDED", "INF", "UNDECIDED", "Context", "GPUbfs_edge",
+    "grid", "read_egr", "rmat", "road", "EXCLUDED", "INCLUDED", "INF", "UNDECIDED", "Context", "Query", "GPUbfs_edge",
     "GPUbfs_vertex", "GPUcc_edge", "GPUcc_vertex", "GPUmis_edge", "GPUmis_vertex", "GPUsssp_edge",
     "GPUsssp_vertex", "GPUtc_edge", "GPUtc_vertex", "PR_GPU",
 ]

@@ -51,6 +51,34 @@ class HostGraphStruct(ctypes.Structure):
     ]
 
 
+class GraphRefStruct(ctypes.Structure):
+    """struct indigo_graph_ref: one device-resident graph of a batch."""
+
+    _fields_ = [
+        ("nodes", c_int),
+        ("edges", c_int),
+        ("d_nindex", c_void_p),
+        ("d_nlist", c_void_p),
+        ("d_eweight", c_void_p),
+    ]
+
+
+QUERY_BFS, QUERY_SSSP, QUERY_CC, QUERY_MIS = 1, 2, 3, 4
+MAX_BFS_SOURCES, MAX_SSSP_SOURCES = 64, 32
+
+
+class QueryStruct(ctypes.Structure):
+    """struct indigo_query: one query of a batch."""
+
+    _fields_ = [
+        ("alg", c_int),
+        ("graph", c_int),
+        ("nsrc", c_int),
+        ("src", c_int * MAX_BFS_SOURCES),
+        ("d_out", c_void_p),
+    ]
+
+
 _PI = POINTER(c_int)
 _PU = POINTER(c_uint)
 _PF = POINTER(c_float)
@@ -87,6 +115,7 @@ _SIGNATURES = {
     "indigo_ctx_last_grid": [c_void_p, _PI, _PI],
     "indigo_ctx_last_kernel": [c_void_p, _PF, ctypes.c_char_p, c_int],
     "indigo_ctx_copy_ms": [c_void_p, c_void_p, c_void_p, c_size_t, _PF],
+    "indigo_batch": [c_void_p, POINTER(GraphRefStruct), c_int, POINTER(QueryStruct), c_int, c_int],
     "indigo_ctx_last_trace": [c_void_p, c_int, _PF, _PU, _PD, _PI, _PI],
     "indigo_ctx_set_hint": [c_void_p, c_int, c_int],
     "indigo_bfs": [c_void_p, c_int, c_int, c_void_p, c_void_p, c_int, c_void_p],

@@ -121,6 +121,22 @@ class Context:
         _lib.call("indigo_tc", self.h, g.nodes, g.edges, g.d_nindex.ptr, g.d_nlist.ptr, ctypes.byref(c))
         return int(c.value)
 
+    def batch(self, graphs: list[DeviceGraph], queries: list["Query"], max_workgroups: int = 0) -> None:
+        """Runs all queries in one persistent launch (indigo_batch); each
+        query's output buffer receives its result."""
+        gt = (_lib.GraphRefStruct * max(1, len(graphs)))()
+        for i, g in enumerate(graphs):
+            gt[i].nodes, gt[i].edges = g.nodes, g.edges
+            gt[i].d_nindex, gt[i].d_nlist = g.d_nindex.ptr, g.d_nlist.ptr
+            gt[i].d_eweight = g.d_eweight.ptr if g.d_eweight is not None else None
+        qt = (_lib.QueryStruct * max(1, len(queries)))()
+        for i, q in enumerate(queries):
+            qt[i].alg, qt[i].graph, qt[i].nsrc = q.alg, q.graph, len(q.sources)
+            for j, s_ in enumerate(q.sources):
+                qt[i].src[j] = s_
+            qt[i].d_out = q.out.ptr
+        _lib.call("indigo_batch", self.h, gt, len(graphs), qt, len(queries), max_workgroups)
+
     # ---- host-output conveniences -------------------------------------------
     def bfs(self, g: DeviceGraph, src: int) -> np.ndarray:
         out = DeviceBuffer(4 * g.nodes)
@@ -146,6 +162,28 @@ class Context:
         out = DeviceBuffer(4 * g.nodes)
         it = self.pagerank_into(g, out)
         return out.to_array(np.float32, g.nodes), it
+
+
+class Query:
+    """One query of a batch: alg is "bfs", "sssp", "cc" or "mis"; graph is
+    the index into the batch's graph list; BFS / SSSP take up to 64 / 32
+    sources and write an int32 [len(sources), nodes] slab."""
+
+    ALGS = {"bfs": _lib.QUERY_BFS, "sssp": _lib.QUERY_SSSP, "cc": _lib.QUERY_CC, "mis": _lib.QUERY_MIS}
+
+    def __init__(self, alg: str, graph: int, g: DeviceGraph, sources=()):
+        self.name = alg
+        self.alg = self.ALGS[alg]
+        self.graph = graph
+        self.sources = [int(s) for s in sources]
+        rows = len(self.sources) if alg in ("bfs", "sssp") else 1
+        self.nodes = g.nodes
+        self.out = DeviceBuffer(4 * g.nodes * max(rows, 1))
+
+    def result(self) -> np.ndarray:
+        rows = len(self.sources) if self.name in ("bfs", "sssp") else 1
+        a = self.out.to_array(np.int32, rows * self.nodes)
+        return a.reshape(rows, self.nodes) if self.name in ("bfs", "sssp") else a
 
 
 # ---- the reference's entry functions (include/*_cuda.h) ----------------------

@@ -109,3 +109,24 @@ def test_context_fails_cleanly_without_gpu(ig):
     out = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=120)
     rc = int(out.stdout.strip().splitlines()[-1])
     assert rc in (0, ig._lib.ERR_NO_DEVICE)
+
+
+def test_batch_structs_match_the_c_layout(ig, tmp_path):
+    """ctypes mirrors of indigo_graph_ref / indigo_query against the C compiler."""
+    import ctypes
+    import shutil
+    import subprocess
+
+    cc = shutil.which("gcc") or shutil.which("cc")
+    if cc is None:
+        pytest.skip("no C compiler")
+    src = tmp_path / "layout.c"
+    src.write_text('#include <stdio.h>\n#include <stddef.h>\n#include "indigo_hip.h"\n'
+                   'int main(void) { printf("%zu %zu %zu %zu %zu\\n", sizeof(indigo_query), '
+                   'offsetof(indigo_query, src), offsetof(indigo_query, d_out), sizeof(indigo_graph_ref), '
+                   'offsetof(indigo_graph_ref, d_eweight)); return 0; }\n')
+    exe = tmp_path / "layout"
+    subprocess.run([cc, f"-I{REPO}/include", str(src), "-o", str(exe)], check=True)
+    got = [int(x) for x in subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()]
+    Q, G = ig._lib.QueryStruct, ig._lib.GraphRefStruct
+    assert got == [ctypes.sizeof(Q), Q.src.offset, Q.d_out.offset, ctypes.sizeof(G), G.d_eweight.offset]
