@@ -189,6 +189,7 @@ int indigo_tc(indigo_ctx* ctx, int nodes, int edges, const int* d_nindex, const 
 #define INDIGO_QUERY_SSSP 2 /* multi-source SSSP: a per-source distance slab */
 #define INDIGO_QUERY_CC 3
 #define INDIGO_QUERY_MIS 4  /* unsigned priority order (the C++ path) */
+#define INDIGO_QUERY_PR 5   /* PageRank from 1/nodes, same recurrence and stop rule as indigo_pagerank */
 #define INDIGO_MAX_BFS_SOURCES 64
 #define INDIGO_MAX_SSSP_SOURCES 32
 
@@ -206,7 +207,9 @@ typedef struct indigo_query {
   int src[INDIGO_MAX_BFS_SOURCES];
   /* device output: BFS / SSSP int32[nsrc][nodes], source-major (row i is the
    * distance array of src[i], INT_MAX = unreachable); CC label[nodes]
-   * (smallest id of the component); MIS status[nodes] (1 in, 2 out) */
+   * (smallest id of the component); MIS status[nodes] (1 in, 2 out);
+   * PR float scores[nodes] followed by the int iteration count (nodes + 1
+   * words) */
   int* d_out;
 } indigo_query;
 

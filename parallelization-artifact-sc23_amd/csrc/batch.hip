@@ -33,6 +33,12 @@
 //         component minima), then flattening.
 //   MIS : the greedy fixed point in descending (hash(v + 712313887), v)
 //         order, two phases per round (decide, then exclude).
+//   PR  : the Jacobi recurrence of the single-query kernel (FP32 scores,
+//         Neumaier-compensated gathers: one lane per short list in entry
+//         order, one wave per list of kPrWaveList or more), stopping when the
+//         FP64 L1 change is <= 1e-4 or after 100 sweeps; every gang member
+//         adds the members' FP64 partials in the same order, so all take the
+//         same decision.
 // Lists are walked 64 entries per wave step with the in-wave load balancer,
 // so skewed degrees do not serialise a lane.
 #include "batch.hpp"
@@ -42,6 +48,7 @@
 #include <cstring>
 #include <vector>
 
+#include "csum.hpp"
 #include "device_common.hpp"
 
 namespace indigo {
@@ -52,6 +59,8 @@ constexpr int kBatchBlock = 512;
 constexpr int kBatchWaves = kBatchBlock / kWave;
 constexpr int kMaxGang = 64;
 constexpr int kGangShift = 18;  // CSR entries per gang member
+constexpr int kPrWaveList = 32;  // PR: lists walked by a whole wave
+constexpr float kDamp = 0.85f;
 
 struct alignas(128) GangState {
   unsigned joined;   // members that took a slot (grows past the gang size)
@@ -67,7 +76,7 @@ struct BatchTask {
   const int* nlist;
   const int* eweight;
   int* out;
-  void* scratch;  // BFS: 3 x u64[nodes]; SSSP: 3 x u32[nodes]
+  void* scratch;  // BFS: 3 x u64[nodes]; SSSP: 3 x u32[nodes]; PR: 2 x f32[nodes] + 2 x f64[kMaxGang]
   int nsrc;
   int src[INDIGO_MAX_BFS_SOURCES];
 };
@@ -418,6 +427,76 @@ IG_DEV bool run_mis(const BatchTask& T, const Team& tm, int (*s_flag)[kWave])
   return true;
 }
 
+// ---- PageRank ----------------------------------------------------------------------
+IG_DEV bool run_pr(const BatchTask& T, const Team& tm, double* s_red)
+{
+  const int V = T.nodes;
+  const int lane = lane_id();
+  const int wid = threadIdx.x / kWave;
+  float* scores = reinterpret_cast<float*>(T.out);
+  float* contrib[2] = {static_cast<float*>(T.scratch), static_cast<float*>(T.scratch) + V};
+  double* slots = reinterpret_cast<double*>(static_cast<float*>(T.scratch) + 2 * static_cast<size_t>(V) +
+                                            ((2 * static_cast<size_t>(V)) & 1));  // 8-byte aligned
+  const float inv_n = 1.0f / static_cast<float>(V);
+  const float base = (1.0f - kDamp) / static_cast<float>(V);
+  const Chunks ch = team_chunks(tm, V);
+  for (int c = ch.first; c < ch.count; c += ch.step) {
+    const int v = c * kWave + lane;
+    if (v >= V) continue;
+    scores[v] = inv_n;
+    st_relaxed(&contrib[0][v], inv_n / static_cast<float>(T.nindex[v + 1] - T.nindex[v]));
+  }
+  if (!team_sync(tm)) return false;
+  int it = 0;
+  for (;;) {
+    it++;
+    const float* cin = contrib[(it - 1) & 1];
+    float* cout = contrib[it & 1];
+    double d = 0.0;
+    for (int c = ch.first; c < ch.count; c += ch.step) {
+      const int v = c * kWave + lane;
+      const int beg = v < V ? T.nindex[v] : 0;
+      const int deg = v < V ? T.nindex[v + 1] - beg : 0;
+      auto finish = [&](int x, int xdeg, float total) {
+        const float old = scores[x];
+        const float s = base + kDamp * total;
+        scores[x] = s;
+        st_relaxed(&cout[x], s / static_cast<float>(xdeg));
+        return fabs(static_cast<double>(s) - static_cast<double>(old));
+      };
+      if (v < V && deg < kPrWaveList) {
+        CSum acc;
+        for (int e = beg; e < beg + deg; e++) acc.add(ld_relaxed(&cin[T.nlist[e]]));
+        d += finish(v, deg, acc.value());
+      }
+      for (unsigned long long m = __ballot(v < V && deg >= kPrWaveList); m != 0ull; m &= m - 1ull) {
+        const int o = __ffsll(static_cast<long long>(m)) - 1;
+        const int x = c * kWave + o;
+        const int xb = wave_bcast(beg, o), xd = wave_bcast(deg, o);
+        CSum acc;
+        for (int e = xb + lane; e < xb + xd; e += kWave) acc.add(ld_relaxed(&cin[T.nlist[e]]));
+        acc = wave_merge(acc);
+        if (lane == 0) d += finish(x, xd, acc.value());
+      }
+    }
+    // the member's FP64 partial (fixed order: wave tree, then waves in order)
+    d = wave_sum(d);
+    if (lane == 0) s_red[wid] = d;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double t = 0.0;
+      for (int w = 0; w < kBatchWaves; w++) t += s_red[w];
+      st_relaxed(&slots[(it & 1) * kMaxGang + tm.rank], t);
+    }
+    if (!team_sync(tm)) return false;
+    double total = 0.0;
+    for (int r = 0; r < tm.size; r++) total += ld_relaxed(&slots[(it & 1) * kMaxGang + r]);
+    if (!(total > static_cast<double>(0.0001f) && it < 100)) break;
+  }
+  if (tm.rank == 0 && threadIdx.x == 0) st_relaxed(&T.out[V], it);
+  return true;
+}
+
 __global__ __launch_bounds__(kBatchBlock) void batch_kernel(const BatchTask* __restrict__ tasks, int ntasks,
                                                             GangState* gangs, BatchControl* ctl, long long budget,
                                                             long long residency_ticks)
@@ -425,6 +504,7 @@ __global__ __launch_bounds__(kBatchBlock) void batch_kernel(const BatchTask* __r
   __shared__ int s_task, s_rank;
   __shared__ unsigned long long s_acc[kBatchWaves][kWave];
   __shared__ int s_flag[kBatchWaves][kWave];
+  __shared__ double s_red[kBatchWaves];
   if (!residency_barrier(&ctl->bar, residency_ticks)) return;
   const long long deadline = now_ticks() + budget;
   for (;;) {
@@ -456,6 +536,7 @@ __global__ __launch_bounds__(kBatchBlock) void batch_kernel(const BatchTask* __r
     if (T.alg == INDIGO_QUERY_BFS) ok = run_bfs(T, tm, s_acc);
     else if (T.alg == INDIGO_QUERY_SSSP) ok = run_sssp(T, tm);
     else if (T.alg == INDIGO_QUERY_CC) ok = run_cc(T, tm);
+    else if (T.alg == INDIGO_QUERY_PR) ok = run_pr(T, tm, s_red);
     else ok = run_mis(T, tm, s_flag);
     if (!ok) break;
   }
@@ -466,6 +547,7 @@ size_t scratch_bytes(int alg, int nodes)
   const size_t v = static_cast<size_t>(nodes);
   if (alg == INDIGO_QUERY_BFS) return Carver::round(3 * v * 8);
   if (alg == INDIGO_QUERY_SSSP) return Carver::round(3 * v * 4);
+  if (alg == INDIGO_QUERY_PR) return Carver::round(2 * v * 4 + 8 + 2 * kMaxGang * 8);
   return 0;
 }
 
@@ -503,7 +585,7 @@ int batch_run(indigo_ctx* ctx, const indigo_graph_ref* graphs, int ngraphs, cons
           return set_error(INDIGO_ERR_INVALID_ARG, "batch: query %d source %d out of range", q, Q.src[i]);
       if (Q.alg == INDIGO_QUERY_SSSP && G.d_eweight == nullptr && G.edges > 0)
         return set_error(INDIGO_ERR_INVALID_ARG, "batch: query %d is SSSP on a graph without weights", q);
-    } else if (Q.alg != INDIGO_QUERY_CC && Q.alg != INDIGO_QUERY_MIS) {
+    } else if (Q.alg != INDIGO_QUERY_CC && Q.alg != INDIGO_QUERY_MIS && Q.alg != INDIGO_QUERY_PR) {
       return set_error(INDIGO_ERR_INVALID_ARG, "batch: query %d has unknown algorithm %d", q, Q.alg);
     }
   }

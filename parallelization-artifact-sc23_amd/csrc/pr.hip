@@ -33,6 +33,7 @@
 #include <vector>
 
 #include "device_common.hpp"
+#include "csum.hpp"
 #include "frontier.hpp"
 
 namespace indigo {
@@ -74,34 +75,6 @@ struct PrArgs {
   long long budget;
   long long start_ticks;
 };
-
-// Neumaier-compensated accumulator: value = s + c.
-struct CSum {
-  float s = 0.f, c = 0.f;
-  IG_DEV void add(float x)
-  {
-    const float t = s + x;
-    c += (fabsf(s) >= fabsf(x)) ? ((s - t) + x) : ((x - t) + s);
-    s = t;
-  }
-  IG_DEV void merge(float s2, float c2)
-  {
-    add(s2);
-    c += c2;
-  }
-  IG_DEV float value() const { return s + c; }
-};
-
-IG_DEV CSum wave_merge(CSum a)
-{
-#pragma unroll
-  for (int d = kWave / 2; d > 0; d >>= 1) {
-    const float s2 = __shfl_xor(a.s, d, kWave);
-    const float c2 = __shfl_xor(a.c, d, kWave);
-    a.merge(s2, c2);
-  }
-  return a;
-}
 
 IG_DEV int deg_of(const PrArgs& a, int v)
 {

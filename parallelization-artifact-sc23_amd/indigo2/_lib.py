@@ -63,7 +63,7 @@ class GraphRefStruct(ctypes.Structure):
     ]
 
 
-QUERY_BFS, QUERY_SSSP, QUERY_CC, QUERY_MIS = 1, 2, 3, 4
+QUERY_BFS, QUERY_SSSP, QUERY_CC, QUERY_MIS, QUERY_PR = 1, 2, 3, 4, 5
 MAX_BFS_SOURCES, MAX_SSSP_SOURCES = 64, 32
 
 

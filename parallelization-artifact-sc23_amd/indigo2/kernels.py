@@ -165,11 +165,13 @@ class Context:
 
 
 class Query:
-    """One query of a batch: alg is "bfs", "sssp", "cc" or "mis"; graph is
-    the index into the batch's graph list; BFS / SSSP take up to 64 / 32
-    sources and write an int32 [len(sources), nodes] slab."""
+    """One query of a batch: alg is "bfs", "sssp", "cc", "mis" or "pr"; graph
+    is the index into the batch's graph list; BFS / SSSP take up to 64 / 32
+    sources and write an int32 [len(sources), nodes] slab; PR writes float
+    scores followed by the iteration count."""
 
-    ALGS = {"bfs": _lib.QUERY_BFS, "sssp": _lib.QUERY_SSSP, "cc": _lib.QUERY_CC, "mis": _lib.QUERY_MIS}
+    ALGS = {"bfs": _lib.QUERY_BFS, "sssp": _lib.QUERY_SSSP, "cc": _lib.QUERY_CC, "mis": _lib.QUERY_MIS,
+            "pr": _lib.QUERY_PR}
 
     def __init__(self, alg: str, graph: int, g: DeviceGraph, sources=()):
         self.name = alg
@@ -178,9 +180,13 @@ class Query:
         self.sources = [int(s) for s in sources]
         rows = len(self.sources) if alg in ("bfs", "sssp") else 1
         self.nodes = g.nodes
-        self.out = DeviceBuffer(4 * g.nodes * max(rows, 1))
+        self.out = DeviceBuffer(4 * (g.nodes * max(rows, 1) + (1 if alg == "pr" else 0)))
 
-    def result(self) -> np.ndarray:
+    def result(self):
+        """int32 slab / labels / status; for PR (float32 scores, iterations)."""
+        if self.name == "pr":
+            a = self.out.to_array(np.int32, self.nodes + 1)
+            return a[: self.nodes].view(np.float32).copy(), int(a[self.nodes])
         rows = len(self.sources) if self.name in ("bfs", "sssp") else 1
         a = self.out.to_array(np.int32, rows * self.nodes)
         return a.reshape(rows, self.nodes) if self.name in ("bfs", "sssp") else a

@@ -109,3 +109,23 @@ def test_batch_rejects_bad_queries(ig, ctx):
     q4 = ig.Query("bfs", 0, dg, [16])
     with pytest.raises(ig.IndigoError):
         ctx.batch([dg], [q4])
+
+
+def test_batch_pagerank(ig, orc, ctx):
+    """PR tasks: FP32 scores within 1e-6 per vertex and 1e-4 in L1 of the FP64
+    oracle at the same iteration count; the count within +-1 of the FP32
+    reference recurrence (both as for the single-query kernel)."""
+    from test_gpu_kernels import _pr_check
+
+    gs = [g for g in catalog(ig)] + [ig.rmat(15, 8, seed=9), ig.rmat(17, 8, seed=4)]
+    dgs = [ig.DeviceGraph(g) for g in gs]
+    qs = [ig.Query("pr", i, dgs[i]) for i in range(len(gs))]
+    print(f"batch of {len(qs)} PR queries", flush=True)
+    ctx.batch(dgs, qs)
+    for q in qs:
+        g = gs[q.graph]
+        scores, iters = q.result()
+        assert 1 <= iters <= 100, (g.name, iters)
+        _pr_check(ig, orc, g, scores, iters)
+        _, it32 = orc.pr_ref32(g)
+        assert abs(it32 - iters) <= 1, (g.name, it32, iters)
