@@ -164,6 +164,8 @@ int indigo_ctx_set_hint(indigo_ctx* ctx, int key, int value);
 #define INDIGO_HINT_BUDGET_MS 4      /* bound on device-side waits in ms (0: 60000) */
 #define INDIGO_HINT_BLOCKS_PER_CU 5  /* cap on co-resident workgroups per CU of persistent kernels (0: occupancy) */
 #define INDIGO_HINT_TC_LDS_LIST 6    /* longest oriented list TC hashes in LDS (0: 16384); longer ones use binary search */
+#define INDIGO_HINT_RESIDENCY_MS 7   /* how long a persistent grid may take to assemble (0: 100 ms); longer when
+                                        the GPU is shared with other work */
 
 /* All resident entries take the device CSR as (nodes, edges, d_nindex,
  * d_nlist[, d_eweight]) and return after the result is in device memory. */
@@ -218,6 +220,52 @@ typedef struct indigo_query {
  * batches can share the GPU on different contexts. */
 int indigo_batch(indigo_ctx* ctx, const indigo_graph_ref* graphs, int ngraphs, const indigo_query* queries,
                  int nqueries, int max_workgroups);
+
+/* ---- serving ----------------------------------------------------------------- */
+/* A request loop over resident graphs (no counterpart in the reference).
+ * Requests arrive over time; every free slot (a context with its own stream
+ * and a share of the GPU) takes the oldest waiting requests that fit the
+ * memory budget, merges BFS / SSSP requests on the same graph into
+ * multi-source tasks and runs them as one batched launch, while the other
+ * slots keep running theirs.  A triangle count runs alone on its slot. */
+#define INDIGO_QUERY_TC 6 /* serving only */
+
+typedef struct indigo_request {
+  int alg;   /* INDIGO_QUERY_BFS, _SSSP, _CC, _MIS, _PR or _TC */
+  int graph; /* index into the graph table */
+  int src;   /* BFS / SSSP source */
+  int reserved;
+  double arrival; /* seconds after the start of the run */
+  /* NULL, or a device buffer for the result: BFS / SSSP / CC / MIS
+   * int[nodes], PR float[nodes] then the int iteration count, TC one
+   * long long */
+  void* d_out;
+  /* filled in by indigo_serve, in seconds after the start of the run */
+  double admitted; /* taken into a launch */
+  double finished; /* result complete (and copied to d_out) */
+  int batch;       /* requests served by the same launch */
+  int slot;
+} indigo_request;
+
+typedef struct indigo_serve_config {
+  int device;
+  int slots;         /* concurrent launches, 1..16 */
+  int max_batch;     /* requests per launch */
+  size_t mem_budget; /* device bytes for the outputs of admitted requests */
+} indigo_serve_config;
+
+typedef struct indigo_serve_stats {
+  double makespan;        /* seconds from the start to the last result */
+  int launches;           /* batched launches */
+  int tc_launches;        /* triangle counts */
+  size_t peak_bytes;      /* largest output footprint held at once */
+  int max_waiting;        /* longest queue of arrived, unadmitted requests */
+} indigo_serve_stats;
+
+/* Serves reqs (any order of arrival) in real time and returns when every
+ * result is complete; per-request times are written back into reqs. */
+int indigo_serve(const indigo_graph_ref* graphs, int ngraphs, indigo_request* reqs, int nreqs,
+                 const indigo_serve_config* cfg, indigo_serve_stats* stats);
 
 #ifdef __cplusplus
 }

@@ -560,13 +560,49 @@ int gang_for(long long entries, int grid)
 
 }  // namespace
 
-int batch_run(indigo_ctx* ctx, const indigo_graph_ref* graphs, int ngraphs, const indigo_query* queries,
-              int nqueries, int max_workgroups)
+// (Re)launches the batch recorded in ctx->pending_batch; records the stop
+// event without waiting.
+int batch_grid_limit(indigo_ctx* ctx, int* limit)
 {
+  int per_cu = 0;
+  IG_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(batch_kernel),
+                                                         kBatchBlock, 0));
+  *limit = std::max(1, per_cu * ctx->compute_units);
+  return INDIGO_OK;
+}
+
+static int launch_pending(indigo_ctx* ctx)
+{
+  BatchTask* d_tasks = static_cast<BatchTask*>(ctx->pending_batch.tasks);
+  GangState* gangs = static_cast<GangState*>(ctx->pending_batch.gangs);
+  BatchControl* ctl = static_cast<BatchControl*>(ctx->pending_batch.ctl);
+  int ntasks = ctx->pending_batch.ntasks;
+  long long budget = budget_ticks(ctx);
+  long long res_ticks = residency_ticks(ctx);
+  IG_TRY(timing_begin(ctx));
+  IG_TRY(zero_async(ctx, ctl, sizeof(BatchControl)));
+  IG_TRY(zero_async(ctx, gangs, static_cast<size_t>(ntasks) * sizeof(GangState)));
+  void* args[] = {&d_tasks, &ntasks, &gangs, &ctl, &budget, &res_ticks};
+  IG_TRY(coop_launch(ctx, reinterpret_cast<const void*>(batch_kernel), kBatchBlock, 0, args,
+                     ctx->pending_batch.grid_cap, nullptr, "batch_kernel"));
+  return timing_record_stop(ctx);
+}
+
+int batch_launch(indigo_ctx* ctx, const indigo_graph_ref* graphs, int ngraphs, const indigo_query* queries,
+                 int nqueries, int max_workgroups)
+{
+  if (ctx->pending_batch.active) return set_error(INDIGO_ERR_INVALID_ARG, "batch: the context has a batch in flight");
   if (nqueries < 0 || ngraphs < 0 || (nqueries > 0 && (graphs == nullptr || queries == nullptr)))
     return set_error(INDIGO_ERR_INVALID_ARG, "batch: bad graph / query table");
   if (max_workgroups < 0) return set_error(INDIGO_ERR_INVALID_ARG, "batch: max_workgroups < 0");
-  if (nqueries == 0) return INDIGO_OK;
+  if (nqueries == 0) {
+    // an empty batch completes at once: record a stop event for batch_ready
+    IG_TRY(timing_begin(ctx));
+    IG_TRY(timing_record_stop(ctx));
+    ctx->pending_batch = {};
+    ctx->pending_batch.active = true;
+    return INDIGO_OK;
+  }
   for (int g = 0; g < ngraphs; g++) {
     const indigo_graph_ref& G = graphs[g];
     IG_TRY(check_graph_args(G.nodes, G.edges, G.d_nindex, G.d_nlist));
@@ -637,27 +673,63 @@ int batch_run(indigo_ctx* ctx, const indigo_graph_ref* graphs, int ngraphs, cons
     for (int s = 0; s < T.nsrc; s++) T.src[s] = Q.src[s];
   }
   IG_HIP_TRY(hipMemcpyAsync(d_tasks, h_tasks, n * sizeof(BatchTask), hipMemcpyHostToDevice, ctx->stream));
-  const long long budget = budget_ticks(ctx);
-  const long long res_ticks = residency_ticks(ctx);
-  int ntasks = nqueries;
+  ctx->pending_batch.active = true;
+  ctx->pending_batch.tasks = d_tasks;
+  ctx->pending_batch.gangs = gangs;
+  ctx->pending_batch.ctl = ctl;
+  ctx->pending_batch.ntasks = nqueries;
+  ctx->pending_batch.grid_cap = grid_cap;
+  return launch_pending(ctx);
+}
+
+int batch_ready(indigo_ctx* ctx, int* ready)
+{
+  if (!ctx->pending_batch.active) return set_error(INDIGO_ERR_INVALID_ARG, "batch: nothing in flight");
+  const hipError_t e = hipEventQuery(ctx->ev_stop);
+  if (e == hipErrorNotReady) {
+    *ready = 0;
+    return INDIGO_OK;
+  }
+  IG_HIP_TRY(e);
+  *ready = 1;
+  return INDIGO_OK;
+}
+
+int batch_wait(indigo_ctx* ctx)
+{
+  if (!ctx->pending_batch.active) return set_error(INDIGO_ERR_INVALID_ARG, "batch: nothing in flight");
   for (;;) {
-    IG_TRY(timing_begin(ctx));
-    IG_TRY(zero_async(ctx, ctl, sizeof(BatchControl)));
-    IG_TRY(zero_async(ctx, gangs, n * sizeof(GangState)));
-    void* args[] = {&d_tasks, &ntasks, &gangs, &ctl, const_cast<long long*>(&budget),
-                    const_cast<long long*>(&res_ticks)};
-    IG_TRY(coop_launch(ctx, reinterpret_cast<const void*>(batch_kernel), kBatchBlock, 0, args, grid_cap, nullptr,
-                       "batch_kernel"));
-    IG_TRY(timing_end(ctx, 3, nqueries));
+    const int ntasks = ctx->pending_batch.ntasks;
+    const int rc0 = timing_finish(ctx, 3, ntasks);
+    if (rc0 != INDIGO_OK || ntasks == 0) {
+      ctx->pending_batch.active = false;
+      return rc0;
+    }
+    BatchControl* ctl = static_cast<BatchControl*>(ctx->pending_batch.ctl);
     IG_HIP_TRY(hipMemcpyAsync(ctx->host_pinned, &ctl->bar.abort.v, sizeof(unsigned), hipMemcpyDeviceToHost,
                               ctx->stream));
     IG_HIP_TRY(hipStreamSynchronize(ctx->stream));
     const unsigned abort_code = *static_cast<unsigned*>(ctx->host_pinned);
     const int rc = check_abort(ctx, reinterpret_cast<const void*>(batch_kernel), abort_code, "batch");
-    if (rc != kNotResident) return rc;
+    if (rc != kNotResident) {
+      ctx->pending_batch.active = false;
+      return rc;
+    }
     // the grid did not assemble: relaunch with the lowered residency cap (the
     // device copy of the task table is intact)
+    const int lrc = launch_pending(ctx);
+    if (lrc != INDIGO_OK) {
+      ctx->pending_batch.active = false;
+      return lrc;
+    }
   }
+}
+
+int batch_run(indigo_ctx* ctx, const indigo_graph_ref* graphs, int ngraphs, const indigo_query* queries,
+              int nqueries, int max_workgroups)
+{
+  IG_TRY(batch_launch(ctx, graphs, ngraphs, queries, nqueries, max_workgroups));
+  return batch_wait(ctx);
 }
 
 }  // namespace indigo

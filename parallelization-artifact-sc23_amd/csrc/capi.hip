@@ -6,6 +6,7 @@
 
 #include "batch.hpp"
 #include "bfs.hpp"
+#include "server.hpp"
 #include "cc.hpp"
 #include "device_common.hpp"
 #include "mis.hpp"
@@ -408,6 +409,44 @@ int indigo_batch(indigo_ctx* ctx, const indigo_graph_ref* graphs, int ngraphs, c
 {
   IG_TRY(ctx_prologue(ctx));
   return batch_run(ctx, graphs, ngraphs, queries, nqueries, max_workgroups);
+}
+
+int indigo_serve(const indigo_graph_ref* graphs, int ngraphs, indigo_request* reqs, int nreqs,
+                 const indigo_serve_config* cfg, indigo_serve_stats* stats)
+{
+  clear_error();
+  if (cfg == nullptr || cfg->slots < 1 || cfg->slots > 16 || cfg->max_batch < 1 || cfg->mem_budget == 0)
+    return set_error(INDIGO_ERR_INVALID_ARG, "serve: bad configuration");
+  if (nreqs < 0 || ngraphs < 1 || graphs == nullptr || (nreqs > 0 && reqs == nullptr))
+    return set_error(INDIGO_ERR_INVALID_ARG, "serve: bad graph / request table");
+  for (int g = 0; g < ngraphs; g++) IG_TRY(check_graph_args(graphs[g].nodes, graphs[g].edges, graphs[g].d_nindex,
+                                                           graphs[g].d_nlist));
+  for (int i = 0; i < nreqs; i++) {
+    const indigo_request& r = reqs[i];
+    if (r.graph < 0 || r.graph >= ngraphs) return set_error(INDIGO_ERR_INVALID_ARG, "serve: request %d graph", i);
+    const indigo_graph_ref& G = graphs[r.graph];
+    switch (r.alg) {
+      case INDIGO_QUERY_SSSP:
+        if (G.d_eweight == nullptr && G.edges > 0)
+          return set_error(INDIGO_ERR_INVALID_ARG, "serve: request %d is SSSP on a graph without weights", i);
+        [[fallthrough]];
+      case INDIGO_QUERY_BFS:
+        if (r.src < 0 || r.src >= G.nodes) return set_error(INDIGO_ERR_INVALID_ARG, "serve: request %d source", i);
+        break;
+      case INDIGO_QUERY_CC:
+      case INDIGO_QUERY_MIS:
+      case INDIGO_QUERY_PR:
+        break;
+      case INDIGO_QUERY_TC:
+        if (G.edges == 0) return set_error(INDIGO_ERR_INVALID_ARG, "serve: request %d is TC on an empty graph", i);
+        break;
+      default:
+        return set_error(INDIGO_ERR_INVALID_ARG, "serve: request %d has unknown algorithm %d", i, r.alg);
+    }
+  }
+  if (nreqs == 0) return INDIGO_OK;
+  IG_HIP_TRY(hipSetDevice(cfg->device));
+  return serve_run(graphs, ngraphs, reqs, nreqs, cfg, stats);
 }
 
 }  // extern "C"

@@ -146,9 +146,14 @@ int kernel_end(indigo_ctx* ctx)
   return INDIGO_OK;
 }
 
-int timing_end(indigo_ctx* ctx, int launches, int iterations)
+int timing_record_stop(indigo_ctx* ctx)
 {
   IG_HIP_TRY(hipEventRecord(ctx->ev_stop, ctx->stream));
+  return INDIGO_OK;
+}
+
+int timing_finish(indigo_ctx* ctx, int launches, int iterations)
+{
   IG_HIP_TRY(hipEventSynchronize(ctx->ev_stop));
   float ms = 0.f;
   IG_HIP_TRY(hipEventElapsedTime(&ms, ctx->ev_start, ctx->ev_stop));
@@ -165,6 +170,12 @@ int timing_end(indigo_ctx* ctx, int launches, int iterations)
   ctx->last_launches = launches;
   ctx->last_iterations = iterations;
   return INDIGO_OK;
+}
+
+int timing_end(indigo_ctx* ctx, int launches, int iterations)
+{
+  IG_TRY(timing_record_stop(ctx));
+  return timing_finish(ctx, launches, iterations);
 }
 
 static int ctx_init(int device, indigo_ctx* c)

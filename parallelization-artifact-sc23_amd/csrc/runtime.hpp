@@ -39,6 +39,15 @@ struct indigo_ctx {
   unsigned trace_nf[256] = {0};
   double trace_m[256] = {0};
   int trace_mode[256] = {0};
+  // batch launched by batch_launch and not yet waited for
+  struct {
+    bool active = false;
+    void* tasks = nullptr;
+    void* gangs = nullptr;
+    void* ctl = nullptr;
+    int ntasks = 0;
+    int grid_cap = 0;
+  } pending_batch;
   int last_grid = 0;    // workgroups of the last persistent launch
   int last_per_cu = 0;  // co-resident workgroups per CU it was sized for
 };
@@ -105,7 +114,11 @@ constexpr int kNotResident = -100;
 // kNotResident (cap lowered), or INDIGO_ERR_TIMEOUT.
 int check_abort(indigo_ctx* ctx, const void* kernel, unsigned abort_code, const char* what);
 // Ticks a persistent kernel allows for its grid to assemble.
-inline long long residency_ticks(const indigo_ctx* ctx) { return ctx->wall_hz / 10; }
+inline long long residency_ticks(const indigo_ctx* ctx)
+{
+  const int ms = ctx->hints[INDIGO_HINT_RESIDENCY_MS];
+  return ms > 0 ? ctx->wall_hz / 1000 * ms : ctx->wall_hz / 10;
+}
 
 // Zeroes `bytes` of device memory with a kernel (no memset node, so the
 // sequence stays safe to capture into a HIP graph).
@@ -116,6 +129,10 @@ int check_graph_args(int nodes, int edges, const void* nindex, const void* nlist
 
 // Starts / stops the HIP-event timing of a resident call.
 int timing_begin(indigo_ctx* ctx);
+// timing_end = timing_record_stop + timing_finish: the first records the stop
+// event without waiting, the second waits for it and stores the times.
+int timing_record_stop(indigo_ctx* ctx);
+int timing_finish(indigo_ctx* ctx, int launches, int iterations);
 // Events around the main kernel of a call (the last marked launch wins);
 // `name` must be a string literal.
 int kernel_begin(indigo_ctx* ctx, const char* name);

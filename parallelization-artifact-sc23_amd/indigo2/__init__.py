@@ -15,6 +15,7 @@ from .kernels import (
     UNDECIDED,
     Context,
     Query,
+    serve,
     GPUbfs_edge,
     GPUbfs_vertex,
     GPUcc_edge,
@@ -30,7 +31,7 @@ from .kernels import (
 
 __all__ = [
     "_lib", "IndigoError", "load", "DeviceBuffer", "DeviceGraph", "HostGraph", "erdos_renyi", "from_edges",
-    "grid", "read_egr", "rmat", "road", "EXCLUDED", "INCLUDED", "INF", "UNDECIDED", "Context", "Query", "GPUbfs_edge",
+    "grid", "read_egr", "rmat", "road", "EXCLUDED", "INCLUDED", "INF", "UNDECIDED", "Context", "Query", "serve", "GPUbfs_edge",
     "GPUbfs_vertex", "GPUcc_edge", "GPUcc_vertex", "GPUmis_edge", "GPUmis_vertex", "GPUsssp_edge",
     "GPUsssp_vertex", "GPUtc_edge", "GPUtc_vertex", "PR_GPU",
 ]

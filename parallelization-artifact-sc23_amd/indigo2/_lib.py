@@ -29,6 +29,7 @@ HINT_SMALL_FRONTIER = 3
 HINT_BUDGET_MS = 4
 HINT_BLOCKS_PER_CU = 5
 HINT_TC_LDS_LIST = 6
+HINT_RESIDENCY_MS = 7
 
 
 class IndigoError(RuntimeError):
@@ -79,6 +80,35 @@ class QueryStruct(ctypes.Structure):
     ]
 
 
+QUERY_TC = 6
+
+
+class RequestStruct(ctypes.Structure):
+    """struct indigo_request: one request of a serving run."""
+
+    _fields_ = [
+        ("alg", c_int),
+        ("graph", c_int),
+        ("src", c_int),
+        ("reserved", c_int),
+        ("arrival", c_double),
+        ("d_out", c_void_p),
+        ("admitted", c_double),
+        ("finished", c_double),
+        ("batch", c_int),
+        ("slot", c_int),
+    ]
+
+
+class ServeConfigStruct(ctypes.Structure):
+    _fields_ = [("device", c_int), ("slots", c_int), ("max_batch", c_int), ("mem_budget", c_size_t)]
+
+
+class ServeStatsStruct(ctypes.Structure):
+    _fields_ = [("makespan", c_double), ("launches", c_int), ("tc_launches", c_int), ("peak_bytes", c_size_t),
+                ("max_waiting", c_int)]
+
+
 _PI = POINTER(c_int)
 _PU = POINTER(c_uint)
 _PF = POINTER(c_float)
@@ -116,6 +146,8 @@ _SIGNATURES = {
     "indigo_ctx_last_kernel": [c_void_p, _PF, ctypes.c_char_p, c_int],
     "indigo_ctx_copy_ms": [c_void_p, c_void_p, c_void_p, c_size_t, _PF],
     "indigo_batch": [c_void_p, POINTER(GraphRefStruct), c_int, POINTER(QueryStruct), c_int, c_int],
+    "indigo_serve": [POINTER(GraphRefStruct), c_int, POINTER(RequestStruct), c_int, POINTER(ServeConfigStruct),
+                     POINTER(ServeStatsStruct)],
     "indigo_ctx_last_trace": [c_void_p, c_int, _PF, _PU, _PD, _PI, _PI],
     "indigo_ctx_set_hint": [c_void_p, c_int, c_int],
     "indigo_bfs": [c_void_p, c_int, c_int, c_void_p, c_void_p, c_int, c_void_p],

@@ -124,11 +124,7 @@ class Context:
     def batch(self, graphs: list[DeviceGraph], queries: list["Query"], max_workgroups: int = 0) -> None:
         """Runs all queries in one persistent launch (indigo_batch); each
         query's output buffer receives its result."""
-        gt = (_lib.GraphRefStruct * max(1, len(graphs)))()
-        for i, g in enumerate(graphs):
-            gt[i].nodes, gt[i].edges = g.nodes, g.edges
-            gt[i].d_nindex, gt[i].d_nlist = g.d_nindex.ptr, g.d_nlist.ptr
-            gt[i].d_eweight = g.d_eweight.ptr if g.d_eweight is not None else None
+        gt = graph_table(graphs)
         qt = (_lib.QueryStruct * max(1, len(queries)))()
         for i, q in enumerate(queries):
             qt[i].alg, qt[i].graph, qt[i].nsrc = q.alg, q.graph, len(q.sources)
@@ -190,6 +186,45 @@ class Query:
         rows = len(self.sources) if self.name in ("bfs", "sssp") else 1
         a = self.out.to_array(np.int32, rows * self.nodes)
         return a.reshape(rows, self.nodes) if self.name in ("bfs", "sssp") else a
+
+
+SERVE_ALGS = {"bfs": _lib.QUERY_BFS, "sssp": _lib.QUERY_SSSP, "cc": _lib.QUERY_CC, "mis": _lib.QUERY_MIS,
+              "pr": _lib.QUERY_PR, "tc": _lib.QUERY_TC}
+
+
+def graph_table(graphs: list[DeviceGraph]):
+    gt = (_lib.GraphRefStruct * max(1, len(graphs)))()
+    for i, g in enumerate(graphs):
+        gt[i].nodes, gt[i].edges = g.nodes, g.edges
+        gt[i].d_nindex, gt[i].d_nlist = g.d_nindex.ptr, g.d_nlist.ptr
+        gt[i].d_eweight = g.d_eweight.ptr if g.d_eweight is not None else None
+    return gt
+
+
+def serve(graphs: list[DeviceGraph], algs, graph_ids, sources, arrivals, outs=None, slots: int = 4,
+          max_batch: int = 256, mem_budget: int = 1 << 30, device: int = 0):
+    """Replays requests through indigo_serve.  algs: names ("bfs", "sssp",
+    "cc", "mis", "pr", "tc"); outs: optional DeviceBuffer per request (None
+    for a discarded result).  Returns (admitted, finished, batch, slot) numpy
+    arrays in seconds and the run statistics as a dict."""
+    n = len(algs)
+    rq = (_lib.RequestStruct * max(1, n))()
+    for i in range(n):
+        rq[i].alg = SERVE_ALGS[algs[i]]
+        rq[i].graph = int(graph_ids[i])
+        rq[i].src = int(sources[i])
+        rq[i].arrival = float(arrivals[i])
+        rq[i].d_out = outs[i].ptr if outs is not None and outs[i] is not None else None
+    cfg = _lib.ServeConfigStruct(device, slots, max_batch, mem_budget)
+    st = _lib.ServeStatsStruct()
+    _lib.call("indigo_serve", graph_table(graphs), len(graphs), rq, n, ctypes.byref(cfg), ctypes.byref(st))
+    admitted = np.array([rq[i].admitted for i in range(n)])
+    finished = np.array([rq[i].finished for i in range(n)])
+    batch = np.array([rq[i].batch for i in range(n)])
+    slot = np.array([rq[i].slot for i in range(n)])
+    stats = {"makespan": st.makespan, "launches": st.launches, "tc_launches": st.tc_launches,
+             "peak_bytes": st.peak_bytes, "max_waiting": st.max_waiting}
+    return admitted, finished, batch, slot, stats
 
 
 # ---- the reference's entry functions (include/*_cuda.h) ----------------------

@@ -124,9 +124,14 @@ def test_batch_structs_match_the_c_layout(ig, tmp_path):
     src.write_text('#include <stdio.h>\n#include <stddef.h>\n#include "indigo_hip.h"\n'
                    'int main(void) { printf("%zu %zu %zu %zu %zu\\n", sizeof(indigo_query), '
                    'offsetof(indigo_query, src), offsetof(indigo_query, d_out), sizeof(indigo_graph_ref), '
-                   'offsetof(indigo_graph_ref, d_eweight)); return 0; }\n')
+                   'offsetof(indigo_graph_ref, d_eweight)); '
+                   'printf("%zu %zu %zu %zu %zu\\n", sizeof(indigo_request), offsetof(indigo_request, d_out), '
+                   'offsetof(indigo_request, slot), sizeof(indigo_serve_config), sizeof(indigo_serve_stats)); '
+                   'return 0; }\n')
     exe = tmp_path / "layout"
     subprocess.run([cc, f"-I{REPO}/include", str(src), "-o", str(exe)], check=True)
     got = [int(x) for x in subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()]
-    Q, G = ig._lib.QueryStruct, ig._lib.GraphRefStruct
-    assert got == [ctypes.sizeof(Q), Q.src.offset, Q.d_out.offset, ctypes.sizeof(G), G.d_eweight.offset]
+    Q, G, R = ig._lib.QueryStruct, ig._lib.GraphRefStruct, ig._lib.RequestStruct
+    assert got == [ctypes.sizeof(Q), Q.src.offset, Q.d_out.offset, ctypes.sizeof(G), G.d_eweight.offset,
+                   ctypes.sizeof(R), R.d_out.offset, R.slot.offset, ctypes.sizeof(ig._lib.ServeConfigStruct),
+                   ctypes.sizeof(ig._lib.ServeStatsStruct)]
