@@ -293,6 +293,72 @@ def measure_batched() -> dict:
     return line
 
 
+def serving_pool(ig, seed: int = 77):
+    """The serving configuration's 64 resident graphs: 32 R-MAT (edge factor
+    16, a/b/c/d .57/.19/.19/.05) with scales 14, 15, ..., 22 in turn, and 32
+    square grids with sides 256, 512, 1024, 2048 in turn; weights [1, 255]."""
+    gs, names = [], []
+    for i in range(32):
+        sc = 14 + i % 9
+        gs.append(ig.rmat(sc, 16, seed=seed + i, wmin=1, wmax=255))
+        names.append(f"rmat{sc}")
+    for i in range(32):
+        side = 256 << (i % 4)
+        gs.append(ig.grid(side, side, seed=seed + 100 + i, wmin=1, wmax=255))
+        names.append(f"grid{side}")
+    return gs, names
+
+
+def measure_serving(rates=(1000.0,), nq: int = 10000, seed: int = 5, slots: int = 4, max_batch: int = 256,
+                    mem_budget: int = 8 << 30) -> dict:
+    """10,000 seeded requests with Poisson arrivals at each rate over the pool;
+    mix BFS 40% / SSSP 30% / CC 10% / PageRank 10% / TC 10%, uniform graph
+    and source.  Latency = finish - arrival; aggregate GTEPS = 1e-9 * sum of
+    the requests' CSR entries / makespan."""
+    import indigo2 as ig
+
+    t0 = time.time()
+    gs, names = serving_pool(ig)
+    log(f"serving pool: 64 graphs, V={sum(g.nodes for g in gs)} E={sum(g.edges for g in gs)} "
+        f"({time.time() - t0:.1f}s)")
+    dgs = [ig.DeviceGraph(g) for g in gs]
+    rng = np.random.default_rng(seed)
+    algs = list(rng.choice(["bfs", "sssp", "cc", "pr", "tc"], size=nq, p=[0.4, 0.3, 0.1, 0.1, 0.1]))
+    gid = rng.integers(0, len(gs), nq)
+    src = np.array([rng.integers(0, gs[g].nodes) for g in gid])
+    gaps = rng.exponential(1.0, nq)
+    edges = np.array([gs[g].edges for g in gid], dtype=np.float64)
+    # warm-up: a short run touches every code path and sizes the workspaces
+    log("serving warm-up: 200 requests at once")
+    ig.serve(dgs, algs[:200], gid[:200], src[:200], np.zeros(200), None, slots=slots, max_batch=max_batch,
+             mem_budget=mem_budget, progress=10.0)
+    runs = []
+    for rate in rates:
+        arrivals = np.cumsum(gaps / rate)
+        log(f"serving {nq} requests at {rate:.0f} q/s")
+        adm, fin, batch, slot, st = ig.serve(dgs, algs, gid, src, arrivals, None, slots=slots,
+                                             max_batch=max_batch, mem_budget=mem_budget, progress=10.0)
+        lat = fin - arrivals
+        per = {}
+        for a in ("bfs", "sssp", "cc", "pr", "tc"):
+            m = np.array([x == a for x in algs])
+            per[a] = {"n": int(m.sum()), "p50_ms": round(1e3 * float(np.percentile(lat[m], 50)), 3),
+                      "p99_ms": round(1e3 * float(np.percentile(lat[m], 99)), 3)}
+        runs.append({"rate_qps": rate, "p50_ms": round(1e3 * float(np.percentile(lat, 50)), 3),
+                     "p99_ms": round(1e3 * float(np.percentile(lat, 99)), 3),
+                     "mean_batch": round(float(np.mean(batch)), 2), "makespan_s": round(st["makespan"], 3),
+                     "arrival_span_s": round(float(arrivals[-1]), 3),
+                     "aggregate_gteps": round(1e-9 * edges.sum() / st["makespan"], 3),
+                     "launches": st["launches"], "tc_launches": st["tc_launches"],
+                     "peak_output_bytes": st["peak_bytes"], "max_waiting": st["max_waiting"],
+                     "per_algorithm": per})
+        log(f"serving at {rate:.0f} q/s: p50 {runs[-1]['p50_ms']} ms, p99 {runs[-1]['p99_ms']} ms, "
+            f"{runs[-1]['aggregate_gteps']} GTEPS, mean batch {runs[-1]['mean_batch']}")
+    return {"workload": "serving: 64 resident graphs (32 R-MAT ef16 scales 14-22, 32 grids 256^2-2048^2), "
+                        f"{nq} requests, Poisson arrivals, mix BFS .4 / SSSP .3 / CC .1 / PR .1 / TC .1",
+            "slots": slots, "max_batch": max_batch, "mem_budget": mem_budget, "runs": runs}
+
+
 def cpu_baseline(scale: int, threads: int) -> dict | None:
     """The reference's own best C++-threads variants (compiled from its
     sources into oracle/_ref) on a bounded sample of the workload."""
@@ -339,8 +405,12 @@ def main() -> int:
     ap.add_argument("--cpu-scale", type=int, default=20)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched-vs-plain sub-measurement")
+    ap.add_argument("--no-serving", action="store_true", help="skip the serving sub-measurement")
     ap.add_argument("--algs", default=",".join(ALGS), help="subset to run (profiling); the JSON line needs all six")
-    ap.add_argument("--workload", choices=["rmat", "batched"], default="rmat",
+    ap.add_argument("--rates", default="500,1000,2000", help="serving workload: arrival rates in requests/s")
+    ap.add_argument("--slots", type=int, default=4, help="serving workload: concurrent slots")
+    ap.add_argument("--max-batch", type=int, default=256, help="serving workload: requests per launch")
+    ap.add_argument("--workload", choices=["rmat", "batched", "serving"], default="rmat",
                     help="rmat: the six algorithms on the R-MAT graph (the default line); batched: batched vs plain")
     ap.add_argument("--profile-only", action="store_true",
                     help="run the timed steps and stop (no TC model, no JSON): the command profiled by rocprofv3")
@@ -360,6 +430,10 @@ def main() -> int:
 
     if args.workload == "batched":
         return run_batched(args)
+    if args.workload == "serving":
+        print(json.dumps(measure_serving(tuple(float(r) for r in args.rates.split(",")), slots=args.slots,
+                                         max_batch=args.max_batch)), flush=True)
+        return 0
     if args.copy_calibration:
         import indigo2 as ig
 
@@ -509,6 +583,16 @@ def main() -> int:
         except Exception as exc:  # reported beside the main line, never required by it
             log(f"batched measurement skipped: {exc}")
             line["batched"] = None
+    if not args.no_serving:
+        if time.time() - t0 > 240:
+            log("serving measurement skipped: the run has already taken over 4 minutes")
+            line["serving"] = None
+        else:
+            try:
+                line["serving"] = measure_serving((150.0,), slots=args.slots, max_batch=args.max_batch)
+            except Exception as exc:  # reported beside the main line, never required by it
+                log(f"serving measurement skipped: {exc}")
+                line["serving"] = None
     print(json.dumps(line), flush=True)
     return 0
 

@@ -227,7 +227,9 @@ int indigo_batch(indigo_ctx* ctx, const indigo_graph_ref* graphs, int ngraphs, c
  * and a share of the GPU) takes the oldest waiting requests that fit the
  * memory budget, merges BFS / SSSP requests on the same graph into
  * multi-source tasks and runs them as one batched launch, while the other
- * slots keep running theirs.  A triangle count runs alone on its slot. */
+ * slots keep running theirs.  A triangle count runs alone on its slot, and
+ * so does an SSSP request on a deep, sparse graph (average degree below 6,
+ * at least 65536 vertices), on the single-query near-far kernel. */
 #define INDIGO_QUERY_TC 6 /* serving only */
 
 typedef struct indigo_request {
@@ -252,6 +254,7 @@ typedef struct indigo_serve_config {
   int slots;         /* concurrent launches, 1..16 */
   int max_batch;     /* requests per launch */
   size_t mem_budget; /* device bytes for the outputs of admitted requests */
+  double progress_interval; /* > 0: a progress line on stderr every so many seconds */
 } indigo_serve_config;
 
 typedef struct indigo_serve_stats {

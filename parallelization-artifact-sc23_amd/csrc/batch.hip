@@ -2,45 +2,72 @@
 // launch (MI355X, gfx950).
 //
 // Every query becomes a task: multi-source BFS (up to 64 sources), multi-
-// source SSSP (up to 32), CC or MIS on one graph.  The host sorts tasks by
-// size (largest first) into a device-side queue; one persistent kernel,
-// sized to co-residency and checked by a residency barrier, serves it.
+// source SSSP (up to 32), CC, MIS or PageRank on one graph.  The host sorts
+// tasks by size (largest first) into a device-side queue; one persistent
+// kernel, sized to co-residency and checked by a residency barrier, serves
+// it.
 //
 // Gangs.  A task runs on a gang of k workgroups, k = the task's CSR entries
-// / 2^18 rounded up to a power of two (1..64): small graphs take one
-// workgroup and synchronise with __syncthreads only, large ones a gang with
-// its own bounded barrier.  A free workgroup joins the first task in queue
+// / 2^18 rounded up to a power of two: small graphs take one workgroup and
+// synchronise with __syncthreads only, large ones a gang with its own bounded
+// barrier.  The cap depends on how much work a barrier separates: 256 for
+// the dense sweeps of PR, CC and MIS, 128 for BFS, 64 for SSSP's many short
+// rounds, and 16 for BFS on deep sparse graphs (average degree below 6),
+// whose thousands of levels are bound by barrier latency.  A free workgroup joins the first task in queue
 // order that still has a free slot (one atomic on the task's `joined`
 // counter); the gang starts when all k members have arrived.  Because every
 // workgroup is resident and gangs are filled strictly in queue order, a gang
 // only ever waits for workgroups that are busy with earlier tasks, so
 // formation cannot deadlock.
 //
+// Load balance inside a task.  Sweeps give each wave 64 consecutive vertices
+// (or queue entries) and walk their lists 64 entries per step with the
+// in-wave balancer.  Lists of kHeavyDeg entries or more would tie a wave up
+// (R-MAT hubs reach 10^5 entries), so each task first lists its heavy
+// vertices as kItem-entry items that all waves of the gang share; sweeps
+// skip heavy lists and a second loop walks the items.
+//
 // Memory discipline (as the coherent-mode persistent kernels): data one
-// thread writes and another reads within a task goes through device-scope
-// relaxed loads / stores / atomics, and each barrier first drains the
-// wave's memory operations; barriers are relaxed counters.
+// thread writes and another reads within a task, or that two threads write,
+// goes through device-scope relaxed loads / stores / atomics (the eight XCDs'
+// L2 caches are not coherent for plain accesses, and these barriers do not
+// write them back), and each barrier first drains the wave's memory
+// operations; barriers are relaxed counters.  Plain stores remain only for
+// slab entries written exactly once.
 //
 // Algorithms (same results as the single-query kernels and the reference):
-//   BFS : level-synchronous pull over 64-bit masks: frontier[v] holds the
-//         sources whose BFS reached v at the previous level; a vertex ORs
-//         its neighbours' frontier masks, keeps the bits not yet seen and
-//         writes the level into the slab rows of those sources.
-//   SSSP: Bellman-Ford over a per-source distance slab with per-vertex
-//         32-bit active masks (triple-buffered); relaxations by atomicMin,
-//         a vertex is re-activated for a source when its distance drops.
+//   BFS : level-synchronous and direction-optimising over 64-bit masks:
+//         frontier[v] holds the sources whose BFS reached v at the previous
+//         level, and the frontier vertices are also kept in a queue.  While
+//         the frontier's lists hold fewer than E / 14 entries it pushes:
+//         each frontier vertex ORs its mask, minus the bits a neighbour has
+//         seen, into the neighbour's next mask, and the first push into a
+//         vertex queues it.  Otherwise it pulls: every vertex with sources
+//         still missing ORs its neighbours' frontier masks.  A second phase
+//         turns the next masks into the new frontier and writes the level
+//         into the slab rows of the new bits.  Deep graphs (grids, roads)
+//         cost their edge work plus two barriers per level.
+//   SSSP: Bellman-Ford driven by a queue of active vertices over a
+//         per-source distance slab, with per-vertex 32-bit active masks;
+//         relaxations by atomicMin, a vertex is queued for the next round
+//         when its distance for some source drops.
 //   CC  : union-find hooking the larger root under the smaller (labels =
 //         component minima), then flattening.
 //   MIS : the greedy fixed point in descending (hash(v + 712313887), v)
-//         order, two phases per round (decide, then exclude).
+//         order; per round a decide phase (a vertex joins when no neighbour
+//         is included and no undecided neighbour has a larger key) and an
+//         exclude phase.  A heavy vertex's items raise per-vertex flags
+//         (blocked, has an included neighbour) in the decide phase, and the
+//         vertex is decided from them in the exclude phase.  Deferring a
+//         decision by a phase changes no result: a vertex decides only once
+//         every larger undecided neighbour has, as in the greedy order.
 //   PR  : the Jacobi recurrence of the single-query kernel (FP32 scores,
-//         Neumaier-compensated gathers: one lane per short list in entry
-//         order, one wave per list of kPrWaveList or more), stopping when the
-//         FP64 L1 change is <= 1e-4 or after 100 sweeps; every gang member
-//         adds the members' FP64 partials in the same order, so all take the
-//         same decision.
-// Lists are walked 64 entries per wave step with the in-wave load balancer,
-// so skewed degrees do not serialise a lane.
+//         Neumaier-compensated gathers: one lane per list of fewer than
+//         kPrWaveList entries in entry order, one wave per longer one,
+//         per-item partials combined in item order for heavy lists),
+//         stopping when the FP64 L1 change is <= 1e-4 or after 100 sweeps;
+//         every gang member adds the members' FP64 partials in the same
+//         order, so all take the same decision.
 #include "batch.hpp"
 
 #include <algorithm>
@@ -57,17 +84,23 @@ namespace {
 
 constexpr int kBatchBlock = 512;
 constexpr int kBatchWaves = kBatchBlock / kWave;
-constexpr int kMaxGang = 64;
-constexpr int kGangShift = 18;  // CSR entries per gang member
+constexpr int kMaxGang = 256;
+constexpr int kGangShift = 18;   // CSR entries per gang member
+constexpr int kHeavyDeg = 1024;  // lists this long are split into items
+constexpr int kItem = 1024;      // entries per item
 constexpr int kPrWaveList = 32;  // PR: lists walked by a whole wave
 constexpr float kDamp = 0.85f;
 
 struct alignas(128) GangState {
-  unsigned joined;   // members that took a slot (grows past the gang size)
-  unsigned arrive;   // barrier arrivals of the current generation
-  unsigned gen;      // barrier generation
-  unsigned flag[3];  // per-round "something changed" flags, triple-buffered
-  unsigned pad[26];
+  unsigned long long mf[4];  // BFS: degree sums of the queued frontiers, rotated by level
+  unsigned joined;           // members that took a slot (grows past the gang size)
+  unsigned arrive;           // barrier arrivals of the current generation
+  unsigned gen;              // barrier generation
+  unsigned flag[3];          // per-round "something changed" flags, triple-buffered
+  unsigned qn[4];            // frontier queue sizes, rotated by level / round
+  unsigned nitems;           // heavy-list items
+  unsigned grp[8];           // barrier arrivals per rank group (gangs above 16)
+  unsigned pad[5];
 };
 
 struct BatchTask {
@@ -76,7 +109,8 @@ struct BatchTask {
   const int* nlist;
   const int* eweight;
   int* out;
-  void* scratch;  // BFS: 3 x u64[nodes]; SSSP: 3 x u32[nodes]; PR: 2 x f32[nodes] + 2 x f64[kMaxGang]
+  void* scratch;  // per algorithm, see scratch_bytes
+  int2* items;    // heavy-list items (vertex, first entry)
   int nsrc;
   int src[INDIGO_MAX_BFS_SOURCES];
 };
@@ -94,7 +128,9 @@ struct Team {
   long long deadline;
 };
 
-// Gang barrier (bounded).  Size 1: the workgroup barrier suffices.
+// Gang barrier (bounded).  Size 1: the workgroup barrier suffices; up to 64
+// members count on one counter, larger gangs first on a counter per rank
+// group (rank mod 8), then the groups' last arrivals on the shared one.
 IG_DEV bool team_sync(const Team& tm)
 {
   __shared__ int s_ok;
@@ -105,8 +141,19 @@ IG_DEV bool team_sync(const Team& tm)
     if (tm.size > 1) {
       const unsigned gen0 = ld_relaxed(&tm.gs->gen);
       wave_drain();
-      const unsigned a = atomic_add_relaxed(&tm.gs->arrive, 1u);
-      if (a == static_cast<unsigned>(tm.size) - 1u) {
+      bool last = false;
+      if (tm.size <= 64) {
+        last = atomic_add_relaxed(&tm.gs->arrive, 1u) == static_cast<unsigned>(tm.size) - 1u;
+      } else {
+        const unsigned g = static_cast<unsigned>(tm.rank) & 7u;
+        const unsigned gsize = (static_cast<unsigned>(tm.size) - g + 7u) >> 3;  // ranks r < size with r % 8 == g
+        if (atomic_add_relaxed(&tm.gs->grp[g], 1u) == gsize - 1u) {
+          st_relaxed(&tm.gs->grp[g], 0u);
+          wave_drain();
+          last = atomic_add_relaxed(&tm.gs->arrive, 1u) == 7u;
+        }
+      }
+      if (last) {
         st_relaxed(&tm.gs->arrive, 0u);
         wave_drain();
         st_relaxed(&tm.gs->gen, gen0 + 1u);
@@ -120,8 +167,8 @@ IG_DEV bool team_sync(const Team& tm)
                                                  __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             break;
           }
-          for (int k = 0; k < backoff; k++) __builtin_amdgcn_s_sleep(2);
-          backoff = backoff < 64 ? backoff * 2 : 64;
+          for (int k = 0; k < backoff; k++) __builtin_amdgcn_s_sleep(1);
+          backoff = backoff < 8 ? backoff * 2 : 8;  // short polls: gangs wait a few microseconds per level
         }
       }
     } else if (now_ticks() > tm.deadline) {
@@ -143,16 +190,18 @@ IG_DEV void wave_lds_sync()
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// Wave-collective: walks the CSR entries of the lanes' vertices v0 + lane
-// (those with `active`), 64 per step; f(o, e, valid) gets the owner lane o
-// and the entry e, with all lanes converged.
+IG_DEV bool heavy(int deg) { return deg >= kHeavyDeg; }
+
+// Wave-collective: walks the CSR entries of the vertex `v` of each lane with
+// `active` and a list shorter than kHeavyDeg, 64 per step; f(o, e, valid)
+// gets the owner lane o and the entry e, with all lanes converged.
 template <typename F>
-IG_DEV void wave_entries(int v0, int nodes, const int* __restrict__ nindex, bool active, F&& f)
+IG_DEV void wave_entries_of(int v, bool active, const int* __restrict__ nindex, F&& f)
 {
   const int lane = lane_id();
-  const int v = v0 + lane;
-  const int b = active && v < nodes ? nindex[v] : 0;
-  const int len = active && v < nodes ? nindex[v + 1] - b : 0;
+  const int b = active ? nindex[v] : 0;
+  const int l = active ? nindex[v + 1] - b : 0;
+  const int len = heavy(l) ? 0 : l;
   const int incl = wave_inclusive_scan(len);
   const int total = wave_bcast(incl, kWave - 1);
   const int shift = b - (incl - len);
@@ -169,15 +218,70 @@ struct Chunks {
   int first, step, count;
 };
 
-IG_DEV Chunks team_chunks(const Team& tm, int nodes)
+IG_DEV Chunks team_chunks(const Team& tm, int n)
 {
   return Chunks{tm.rank * kBatchWaves + static_cast<int>(threadIdx.x) / kWave, tm.size * kBatchWaves,
-                (nodes + kWave - 1) / kWave};
+                (n + kWave - 1) / kWave};
+}
+
+// Wave-collective: f(v, e_begin, e_end, item) for the team's share of the
+// heavy items (one item per wave at a time).
+template <typename F>
+IG_DEV void for_items(const BatchTask& T, const Team& tm, int nitems, F&& f)
+{
+  for (int i = tm.rank * kBatchWaves + static_cast<int>(threadIdx.x) / kWave; i < nitems;
+       i += tm.size * kBatchWaves) {
+    const int v = ld_relaxed(&T.items[i].x), eb = ld_relaxed(&T.items[i].y);
+    f(v, eb, min(eb + kItem, T.nindex[v + 1]), i);
+  }
+}
+
+// Gang-collective, once per task: lists the heavy vertices' items.
+IG_DEV bool build_items(const BatchTask& T, const Team& tm, int* nitems)
+{
+  const int lane = lane_id();
+  const Chunks ch = team_chunks(tm, T.nodes);
+  for (int c = ch.first; c < ch.count; c += ch.step) {
+    const int v = c * kWave + lane;
+    const int b = v < T.nodes ? T.nindex[v] : 0;
+    const int deg = v < T.nodes ? T.nindex[v + 1] - b : 0;
+    const int n = heavy(deg) ? (deg + kItem - 1) / kItem : 0;
+    const int incl = wave_inclusive_scan(n);
+    const int total = wave_bcast(incl, kWave - 1);
+    if (total == 0) continue;
+    unsigned base = 0;
+    if (lane == 0) base = atomic_add_relaxed(&tm.gs->nitems, static_cast<unsigned>(total));
+    base = static_cast<unsigned>(wave_shfl(static_cast<int>(base), 0));
+    for (int j = 0; j < n; j++) {
+      const unsigned slot = base + static_cast<unsigned>(incl - n + j);
+      st_relaxed(&T.items[slot].x, v);
+      st_relaxed(&T.items[slot].y, b + j * kItem);
+    }
+  }
+  if (!team_sync(tm)) return false;
+  *nitems = static_cast<int>(ld_relaxed(&tm.gs->nitems));
+  return true;
 }
 
 IG_DEV void set_flag(unsigned* flag, bool any)
 {
   if (__ballot(any) != 0ull && lane_id() == 0) st_relaxed(flag, 1u);
+}
+
+IG_DEV unsigned long long wave_or(unsigned long long x)
+{
+#pragma unroll
+  for (int d = kWave / 2; d > 0; d >>= 1) x |= __shfl_xor(x, d, kWave);
+  return x;
+}
+
+// Wave-collective append of u (where pred) to a queue, adding the degrees of
+// the appended vertices to *mf.
+IG_DEV void append_frontier(bool pred, int u, int deg, int* queue, unsigned* count, unsigned long long* mf)
+{
+  wave_append(pred, u, queue, count);
+  const unsigned long long d = wave_sum(static_cast<unsigned long long>(pred ? deg : 0));
+  if (lane_id() == 0 && d != 0ull) atomic_add_relaxed(mf, d);
 }
 
 // ---- multi-source BFS -------------------------------------------------------
@@ -186,8 +290,13 @@ IG_DEV bool run_bfs(const BatchTask& T, const Team& tm, unsigned long long (*s_a
   const int V = T.nodes;
   const int lane = lane_id();
   const int wid = threadIdx.x / kWave;
+  const size_t Vs = static_cast<size_t>(V);
   unsigned long long* seen = static_cast<unsigned long long*>(T.scratch);
-  unsigned long long* fr[2] = {seen + V, seen + 2 * static_cast<size_t>(V)};
+  unsigned long long* fr[2] = {seen + Vs, seen + 2 * Vs};  // frontier masks by level parity
+  unsigned long long* nxt = seen + 3 * Vs;                 // next-level masks
+  int* queue[2] = {reinterpret_cast<int*>(seen + 4 * Vs), reinterpret_cast<int*>(seen + 4 * Vs) + V};
+  unsigned* qn = tm.gs->qn;
+  unsigned long long* mf = tm.gs->mf;
   const unsigned long long all = T.nsrc >= 64 ? ~0ull : ((1ull << T.nsrc) - 1ull);
   const Chunks ch = team_chunks(tm, V);
   for (int c = ch.first; c < ch.count; c += ch.step) {
@@ -196,48 +305,113 @@ IG_DEV bool run_bfs(const BatchTask& T, const Team& tm, unsigned long long (*s_a
       st_relaxed(&seen[v], 0ull);
       st_relaxed(&fr[0][v], 0ull);
       st_relaxed(&fr[1][v], 0ull);
+      st_relaxed(&nxt[v], 0ull);
     }
   }
-  if (!team_sync(tm)) return false;
-  if (tm.rank == 0 && static_cast<int>(threadIdx.x) < T.nsrc) {
-    const int i = threadIdx.x;
-    const int s = T.src[i];
-    __hip_atomic_fetch_or(&seen[s], 1ull << i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __hip_atomic_fetch_or(&fr[0][s], 1ull << i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    st_relaxed(&T.out[static_cast<size_t>(i) * V + s], 0);
+  int nitems = 0;
+  if (!build_items(T, tm, &nitems)) return false;
+  if (tm.rank == 0 && threadIdx.x < kWave) {  // one wave queues the distinct sources
+    const bool has = lane < T.nsrc;
+    const int i = lane;
+    const int s = has ? T.src[i] : 0;
+    unsigned long long old = 0ull;
+    if (has) {
+      __hip_atomic_fetch_or(&seen[s], 1ull << i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      old = __hip_atomic_fetch_or(&fr[0][s], 1ull << i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      st_relaxed(&T.out[static_cast<size_t>(i) * V + s], 0);
+    }
+    // a source listed twice is queued by the lane that set its first bit
+    append_frontier(has && old == 0ull, s, has ? T.nindex[s + 1] - T.nindex[s] : 0, queue[0], &qn[0], &mf[0]);
   }
   if (!team_sync(tm)) return false;
   for (int lvl = 1;; lvl++) {
     const unsigned long long* cur = fr[(lvl - 1) & 1];
-    unsigned long long* nxt = fr[lvl & 1];
-    unsigned* flag = &tm.gs->flag[lvl % 3];
-    if (tm.rank == 0 && threadIdx.x == 0) st_relaxed(&tm.gs->flag[(lvl + 1) % 3], 0u);
-    for (int c = ch.first; c < ch.count; c += ch.step) {
-      const int v = c * kWave + lane;
-      const unsigned long long sv = v < V ? ld_relaxed(&seen[v]) : all;
-      const unsigned long long missing = all & ~sv;
-      s_acc[wid][lane] = 0ull;
-      wave_lds_sync();
-      wave_entries(c * kWave, V, T.nindex, missing != 0ull, [&](int o, int e, bool valid) {
+    unsigned long long* nfr = fr[lvl & 1];
+    const int* qcur = queue[(lvl - 1) & 1];
+    int* qnext = queue[lvl & 1];
+    unsigned* ncount = &qn[lvl & 3];
+    unsigned long long* nmf = &mf[lvl & 3];
+    const int ncur = static_cast<int>(ld_relaxed(&qn[(lvl - 1) & 3]));
+    const unsigned long long mcur = ld_relaxed(&mf[(lvl - 1) & 3]);
+    if (tm.rank == 0 && threadIdx.x == 0) {
+      st_relaxed(&qn[(lvl + 1) & 3], 0u);
+      st_relaxed(&mf[(lvl + 1) & 3], 0ull);
+    }
+    if (mcur * 14ull < static_cast<unsigned long long>(T.edges)) {
+      // push from the frontier queue
+      const int nq = (ncur + kWave - 1) / kWave;
+      auto push = [&](unsigned long long m, int e, bool valid) {
+        bool first = false;
+        int u = 0;
         if (valid) {
-          const unsigned long long m = ld_relaxed(&cur[T.nlist[e]]);
-          if (m != 0ull) atomicOr(&s_acc[wid][o], m);
+          u = T.nlist[e];
+          const unsigned long long add = m & ~ld_relaxed(&seen[u]);
+          if (add != 0ull)
+            first = __hip_atomic_fetch_or(&nxt[u], add, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0ull;
         }
-      });
-      wave_lds_sync();
-      const unsigned long long nb = s_acc[wid][lane] & missing;
-      if (v < V) {
-        st_relaxed(&nxt[v], nb);
-        if (nb != 0ull) {
-          st_relaxed(&seen[v], sv | nb);
-          for (unsigned long long m = nb; m != 0ull; m &= m - 1ull)
-            T.out[static_cast<size_t>(__ffsll(static_cast<long long>(m)) - 1) * V + v] = lvl;
-        }
+        append_frontier(first, u, first ? T.nindex[u + 1] - T.nindex[u] : 0, qnext, ncount, nmf);
+      };
+      for (int c = ch.first; c < nq; c += ch.step) {
+        const int k = c * kWave + lane;
+        const bool act = k < ncur;
+        const int v = act ? ld_relaxed(&qcur[k]) : 0;
+        const unsigned long long m = act ? ld_relaxed(&cur[v]) : 0ull;
+        wave_entries_of(v, act, T.nindex, [&](int o, int e, bool valid) { push(__shfl(m, o, kWave), e, valid); });
       }
-      set_flag(flag, nb != 0ull);
+      for_items(T, tm, nitems, [&](int v, int eb, int ee, int) {
+        const unsigned long long m = ld_relaxed(&cur[v]);
+        if (m == 0ull) return;
+        for (int e0 = eb; e0 < ee; e0 += kWave) push(m, e0 + lane, e0 + lane < ee);
+      });
+    } else {
+      // pull: every vertex with sources still missing
+      for (int c = ch.first; c < ch.count; c += ch.step) {
+        const int v = c * kWave + lane;
+        const unsigned long long sv = v < V ? ld_relaxed(&seen[v]) : all;
+        const unsigned long long missing = all & ~sv;
+        s_acc[wid][lane] = 0ull;
+        wave_lds_sync();
+        wave_entries_of(v, missing != 0ull, T.nindex, [&](int o, int e, bool valid) {
+          if (valid) {
+            const unsigned long long m = ld_relaxed(&cur[T.nlist[e]]);
+            if (m != 0ull) atomicOr(&s_acc[wid][o], m);
+          }
+        });
+        wave_lds_sync();
+        const unsigned long long nb = s_acc[wid][lane] & missing;
+        if (nb != 0ull) st_relaxed(&nxt[v], nb);
+        append_frontier(nb != 0ull, v, nb != 0ull ? T.nindex[v + 1] - T.nindex[v] : 0, qnext, ncount, nmf);
+      }
+      for_items(T, tm, nitems, [&](int v, int eb, int ee, int) {
+        const unsigned long long missing = all & ~ld_relaxed(&seen[v]);
+        if (missing == 0ull) return;
+        unsigned long long acc = 0ull;
+        for (int e = eb + lane; e < ee; e += kWave) acc |= ld_relaxed(&cur[T.nlist[e]]);
+        const unsigned long long nb = wave_or(acc) & missing;
+        bool first = false;
+        if (nb != 0ull && lane == 0)
+          first = __hip_atomic_fetch_or(&nxt[v], nb, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0ull;
+        append_frontier(first, v, first ? T.nindex[v + 1] - T.nindex[v] : 0, qnext, ncount, nmf);
+      });
     }
     if (!team_sync(tm)) return false;
-    if (ld_relaxed(flag) == 0u) break;
+    // the new frontier: masks, seen bits and levels; the old frontier's masks
+    // are cleared (that buffer holds the next level's frontier)
+    const int nnext = static_cast<int>(ld_relaxed(ncount));
+    for (int k = tm.rank * kBatchBlock + static_cast<int>(threadIdx.x); k < ncur; k += tm.size * kBatchBlock)
+      st_relaxed(&fr[(lvl - 1) & 1][ld_relaxed(&qcur[k])], 0ull);
+    for (int k = tm.rank * kBatchBlock + static_cast<int>(threadIdx.x); k < nnext; k += tm.size * kBatchBlock) {
+      const int u = ld_relaxed(&qnext[k]);
+      const unsigned long long sv = ld_relaxed(&seen[u]);
+      const unsigned long long nb = ld_relaxed(&nxt[u]) & ~sv;
+      st_relaxed(&nxt[u], 0ull);
+      st_relaxed(&seen[u], sv | nb);
+      st_relaxed(&nfr[u], nb);
+      for (unsigned long long m = nb; m != 0ull; m &= m - 1ull)
+        T.out[static_cast<size_t>(__ffsll(static_cast<long long>(m)) - 1) * V + u] = lvl;
+    }
+    if (!team_sync(tm)) return false;
+    if (nnext == 0) break;
   }
   for (int c = ch.first; c < ch.count; c += ch.step) {
     const int v = c * kWave + lane;
@@ -253,8 +427,12 @@ IG_DEV bool run_sssp(const BatchTask& T, const Team& tm)
 {
   const int V = T.nodes;
   const int lane = lane_id();
-  unsigned* act[3] = {static_cast<unsigned*>(T.scratch), static_cast<unsigned*>(T.scratch) + V,
-                      static_cast<unsigned*>(T.scratch) + 2 * static_cast<size_t>(V)};
+  const size_t Vs = static_cast<size_t>(V);
+  unsigned* base = static_cast<unsigned*>(T.scratch);
+  unsigned* act[3] = {base, base + Vs, base + 2 * Vs};  // active masks by round mod 3
+  int* queue[3] = {reinterpret_cast<int*>(base + 3 * Vs), reinterpret_cast<int*>(base + 4 * Vs),
+                   reinterpret_cast<int*>(base + 5 * Vs)};
+  unsigned* qn = tm.gs->qn;
   const Chunks ch = team_chunks(tm, V);
   for (int c = ch.first; c < ch.count; c += ch.step) {
     const int v = c * kWave + lane;
@@ -264,48 +442,68 @@ IG_DEV bool run_sssp(const BatchTask& T, const Team& tm)
     st_relaxed(&act[2][v], 0u);
     for (int i = 0; i < T.nsrc; i++) st_relaxed(&T.out[static_cast<size_t>(i) * V + v], INT_MAX);
   }
-  if (!team_sync(tm)) return false;
-  if (tm.rank == 0 && static_cast<int>(threadIdx.x) < T.nsrc) {
-    const int i = threadIdx.x;
-    const int s = T.src[i];
-    st_relaxed(&T.out[static_cast<size_t>(i) * V + s], 0);
-    __hip_atomic_fetch_or(&act[0][s], 1u << i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  int nitems = 0;
+  if (!build_items(T, tm, &nitems)) return false;
+  if (tm.rank == 0 && threadIdx.x < kWave) {
+    const bool has = lane < T.nsrc;
+    const int i = lane;
+    const int s = has ? T.src[i] : 0;
+    unsigned old = 0u;
+    if (has) {
+      st_relaxed(&T.out[static_cast<size_t>(i) * V + s], 0);
+      old = __hip_atomic_fetch_or(&act[0][s], 1u << i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    wave_append(has && old == 0u, s, queue[0], &qn[0]);
   }
   if (!team_sync(tm)) return false;
   for (int r = 0;; r++) {
     const unsigned* cur = act[r % 3];
     unsigned* nxt = act[(r + 1) % 3];
-    unsigned* clr = act[(r + 2) % 3];
-    unsigned* flag = &tm.gs->flag[r % 3];
-    if (tm.rank == 0 && threadIdx.x == 0) st_relaxed(&tm.gs->flag[(r + 1) % 3], 0u);
-    for (int c = ch.first; c < ch.count; c += ch.step) {
-      const int v = c * kWave + lane;
-      const unsigned m = v < V ? ld_relaxed(&cur[v]) : 0u;
-      if (v < V) st_relaxed(&clr[v], 0u);
-      bool changed = false;
-      wave_entries(c * kWave, V, T.nindex, m != 0u, [&](int o, int e, bool valid) {
-        const unsigned mo = static_cast<unsigned>(wave_shfl(static_cast<int>(m), o));
-        if (!valid) return;
-        const int ov = c * kWave + o;
-        const int u = T.nlist[e];
+    unsigned* prev = act[(r + 2) % 3];  // round r-1's masks: cleared now
+    const int* qcur = queue[r % 3];
+    int* qnext = queue[(r + 1) % 3];
+    const int* qprev = queue[(r + 2) % 3];
+    unsigned* ncount = &qn[(r + 1) & 3];
+    const int ncur = static_cast<int>(ld_relaxed(&qn[r & 3]));
+    const int nprev = r > 0 ? static_cast<int>(ld_relaxed(&qn[(r + 3) & 3])) : 0;
+    if (tm.rank == 0 && threadIdx.x == 0) st_relaxed(&qn[(r + 2) & 3], 0u);
+    for (int k = tm.rank * kBatchBlock + static_cast<int>(threadIdx.x); k < nprev; k += tm.size * kBatchBlock)
+      st_relaxed(&prev[ld_relaxed(&qprev[k])], 0u);
+    auto relax = [&](unsigned m, int v, int e, bool valid) {
+      bool first = false;
+      int u = 0;
+      if (valid) {
+        u = T.nlist[e];
         const int w = T.eweight[e];
-        for (unsigned bits = mo; bits != 0u; bits &= bits - 1u) {
+        unsigned gained = 0u;
+        for (unsigned bits = m; bits != 0u; bits &= bits - 1u) {
           const int i = __ffs(static_cast<int>(bits)) - 1;
           const size_t row = static_cast<size_t>(i) * V;
-          const int nd = ld_relaxed(&T.out[row + ov]) + w;
-          if (nd < ld_relaxed(&T.out[row + u])) {
-            const int old = atomic_min_relaxed(&T.out[row + u], nd);
-            if (nd < old) {
-              __hip_atomic_fetch_or(&nxt[u], 1u << i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-              changed = true;
-            }
-          }
+          const int nd = ld_relaxed(&T.out[row + v]) + w;
+          if (nd < ld_relaxed(&T.out[row + u]) && nd < atomic_min_relaxed(&T.out[row + u], nd)) gained |= 1u << i;
         }
+        if (gained != 0u)
+          first = __hip_atomic_fetch_or(&nxt[u], gained, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0u;
+      }
+      wave_append(first, u, qnext, ncount);
+    };
+    const int nq = (ncur + kWave - 1) / kWave;
+    for (int c = ch.first; c < nq; c += ch.step) {
+      const int k = c * kWave + lane;
+      const bool a = k < ncur;
+      const int v = a ? ld_relaxed(&qcur[k]) : 0;
+      const unsigned m = a ? ld_relaxed(&cur[v]) : 0u;
+      wave_entries_of(v, a, T.nindex, [&](int o, int e, bool valid) {
+        relax(static_cast<unsigned>(__shfl(static_cast<int>(m), o, kWave)), __shfl(v, o, kWave), e, valid);
       });
-      set_flag(flag, changed);
     }
+    for_items(T, tm, nitems, [&](int v, int eb, int ee, int) {
+      const unsigned m = ld_relaxed(&cur[v]);
+      if (m == 0u) return;
+      for (int e0 = eb; e0 < ee; e0 += kWave) relax(m, v, e0 + lane, e0 + lane < ee);
+    });
     if (!team_sync(tm)) return false;
-    if (ld_relaxed(flag) == 0u) break;
+    if (ld_relaxed(ncount) == 0u) break;
   }
   return true;
 }
@@ -350,15 +548,23 @@ IG_DEV bool run_cc(const BatchTask& T, const Team& tm)
     const int v = c * kWave + lane;
     if (v < V) st_relaxed(&parent[v], v);
   }
-  if (!team_sync(tm)) return false;
+  int nitems = 0;
+  if (!build_items(T, tm, &nitems)) return false;
   for (int c = ch.first; c < ch.count; c += ch.step) {
-    wave_entries(c * kWave, V, T.nindex, true, [&](int o, int e, bool valid) {
+    const int v = c * kWave + lane;
+    wave_entries_of(v, v < V, T.nindex, [&](int o, int e, bool valid) {
       if (!valid) return;
-      const int v = c * kWave + o;
+      const int ov = c * kWave + o;
       const int u = T.nlist[e];
-      if (u < v) link(parent, v, u);
+      if (u < ov) link(parent, ov, u);
     });
   }
+  for_items(T, tm, nitems, [&](int v, int eb, int ee, int) {
+    for (int e = eb + lane; e < ee; e += kWave) {
+      const int u = T.nlist[e];
+      if (u < v) link(parent, v, u);
+    }
+  });
   if (!team_sync(tm)) return false;
   for (int c = ch.first; c < ch.count; c += ch.step) {
     const int v = c * kWave + lane;
@@ -380,23 +586,30 @@ IG_DEV bool run_mis(const BatchTask& T, const Team& tm, int (*s_flag)[kWave])
   const int lane = lane_id();
   const int wid = threadIdx.x / kWave;
   int* status = T.out;
+  unsigned* blocked = static_cast<unsigned*>(T.scratch);  // heavy vertices: a blocking neighbour seen
+  unsigned* near_in = blocked + V;                         // heavy vertices: an included neighbour seen
   const Chunks ch = team_chunks(tm, V);
   for (int c = ch.first; c < ch.count; c += ch.step) {
     const int v = c * kWave + lane;
-    if (v < V) st_relaxed(&status[v], 0);
+    if (v < V) {
+      st_relaxed(&status[v], 0);
+      st_relaxed(&blocked[v], 0u);
+      st_relaxed(&near_in[v], 0u);
+    }
   }
-  if (!team_sync(tm)) return false;
+  int nitems = 0;
+  if (!build_items(T, tm, &nitems)) return false;
   for (int r = 0;; r++) {
     unsigned* flag = &tm.gs->flag[r % 3];
     if (tm.rank == 0 && threadIdx.x == 0) st_relaxed(&tm.gs->flag[(r + 1) % 3], 0u);
-    // decide: an undecided vertex joins when no neighbour is included and no
-    // undecided neighbour has a larger key
+    // decide (lists shorter than kHeavyDeg) and gather the heavy vertices' flags
     for (int c = ch.first; c < ch.count; c += ch.step) {
       const int v = c * kWave + lane;
-      const bool und = v < V && ld_relaxed(&status[v]) == 0;
+      const int deg = v < V ? T.nindex[v + 1] - T.nindex[v] : 0;
+      const bool und = v < V && !heavy(deg) && ld_relaxed(&status[v]) == 0;
       s_flag[wid][lane] = 0;
       wave_lds_sync();
-      wave_entries(c * kWave, V, T.nindex, und, [&](int o, int e, bool valid) {
+      wave_entries_of(v, und, T.nindex, [&](int o, int e, bool valid) {
         if (!valid) return;
         const int ov = c * kWave + o;
         const int u = T.nlist[e];
@@ -406,20 +619,47 @@ IG_DEV bool run_mis(const BatchTask& T, const Team& tm, int (*s_flag)[kWave])
       wave_lds_sync();
       if (und && s_flag[wid][lane] == 0) st_relaxed(&status[v], 1);
     }
+    for_items(T, tm, nitems, [&](int v, int eb, int ee, int) {
+      if (ld_relaxed(&status[v]) != 0) return;
+      const unsigned long long kv = mis_key(v);
+      bool blk = false, inc = false;
+      for (int e = eb + lane; e < ee; e += kWave) {
+        const int u = T.nlist[e];
+        const int su = ld_relaxed(&status[u]);
+        inc |= su == 1;
+        blk |= su == 1 || (su == 0 && u != v && mis_key(u) > kv);
+      }
+      const bool any_blk = __ballot(blk) != 0ull, any_inc = __ballot(inc) != 0ull;
+      if (lane == 0 && any_blk) st_relaxed(&blocked[v], 1u);
+      if (lane == 0 && any_inc) st_relaxed(&near_in[v], 1u);
+    });
     if (!team_sync(tm)) return false;
-    // exclude the undecided neighbours of included vertices
+    // exclude the undecided neighbours of included vertices; decide the heavy
+    // vertices from their flags
     for (int c = ch.first; c < ch.count; c += ch.step) {
       const int v = c * kWave + lane;
+      const int deg = v < V ? T.nindex[v + 1] - T.nindex[v] : 0;
       const bool und = v < V && ld_relaxed(&status[v]) == 0;
+      const bool hv = und && heavy(deg);
       s_flag[wid][lane] = 0;
       wave_lds_sync();
-      wave_entries(c * kWave, V, T.nindex, und, [&](int o, int e, bool valid) {
+      wave_entries_of(v, und && !hv, T.nindex, [&](int o, int e, bool valid) {
         if (valid && ld_relaxed(&status[T.nlist[e]]) == 1) s_flag[wid][o] = 1;
       });
       wave_lds_sync();
-      const bool excl = und && s_flag[wid][lane] != 0;
-      if (excl) st_relaxed(&status[v], 2);
-      set_flag(flag, und && !excl);
+      bool still = false;
+      if (und && !hv) {
+        if (s_flag[wid][lane] != 0) st_relaxed(&status[v], 2);
+        else still = true;
+      }
+      if (hv) {
+        if (ld_relaxed(&near_in[v]) != 0u) st_relaxed(&status[v], 2);
+        else if (ld_relaxed(&blocked[v]) == 0u) st_relaxed(&status[v], 1);
+        else still = true;
+        st_relaxed(&blocked[v], 0u);
+        st_relaxed(&near_in[v], 0u);
+      }
+      set_flag(flag, still);
     }
     if (!team_sync(tm)) return false;
     if (ld_relaxed(flag) == 0u) break;
@@ -433,20 +673,33 @@ IG_DEV bool run_pr(const BatchTask& T, const Team& tm, double* s_red)
   const int V = T.nodes;
   const int lane = lane_id();
   const int wid = threadIdx.x / kWave;
+  const size_t Vs = static_cast<size_t>(V);
   float* scores = reinterpret_cast<float*>(T.out);
-  float* contrib[2] = {static_cast<float*>(T.scratch), static_cast<float*>(T.scratch) + V};
-  double* slots = reinterpret_cast<double*>(static_cast<float*>(T.scratch) + 2 * static_cast<size_t>(V) +
-                                            ((2 * static_cast<size_t>(V)) & 1));  // 8-byte aligned
+  double* slots = static_cast<double*>(T.scratch);  // [2][kMaxGang] FP64 partials
+  float* contrib[2] = {reinterpret_cast<float*>(slots + 2 * kMaxGang),
+                       reinterpret_cast<float*>(slots + 2 * kMaxGang) + Vs};
+  float2* hpart = reinterpret_cast<float2*>(contrib[1] + Vs);  // per item (sum, error)
   const float inv_n = 1.0f / static_cast<float>(V);
   const float base = (1.0f - kDamp) / static_cast<float>(V);
   const Chunks ch = team_chunks(tm, V);
   for (int c = ch.first; c < ch.count; c += ch.step) {
     const int v = c * kWave + lane;
     if (v >= V) continue;
-    scores[v] = inv_n;
+    st_relaxed(&scores[v], inv_n);
     st_relaxed(&contrib[0][v], inv_n / static_cast<float>(T.nindex[v + 1] - T.nindex[v]));
   }
-  if (!team_sync(tm)) return false;
+  int nitems = 0;
+  if (!build_items(T, tm, &nitems)) return false;
+  // a heavy vertex's score is written at init by its chunk's wave and later by
+  // the wave of its first item, possibly on another XCD: device-scope stores
+  // keep the two L2s from writing back stale bytes
+  auto finish = [&](float* cout, int x, int xdeg, float total) {
+    const float old = ld_relaxed(&scores[x]);
+    const float s = base + kDamp * total;
+    st_relaxed(&scores[x], s);
+    st_relaxed(&cout[x], s / static_cast<float>(xdeg));
+    return fabs(static_cast<double>(s) - static_cast<double>(old));
+  };
   int it = 0;
   for (;;) {
     it++;
@@ -457,27 +710,42 @@ IG_DEV bool run_pr(const BatchTask& T, const Team& tm, double* s_red)
       const int v = c * kWave + lane;
       const int beg = v < V ? T.nindex[v] : 0;
       const int deg = v < V ? T.nindex[v + 1] - beg : 0;
-      auto finish = [&](int x, int xdeg, float total) {
-        const float old = scores[x];
-        const float s = base + kDamp * total;
-        scores[x] = s;
-        st_relaxed(&cout[x], s / static_cast<float>(xdeg));
-        return fabs(static_cast<double>(s) - static_cast<double>(old));
-      };
       if (v < V && deg < kPrWaveList) {
         CSum acc;
         for (int e = beg; e < beg + deg; e++) acc.add(ld_relaxed(&cin[T.nlist[e]]));
-        d += finish(v, deg, acc.value());
+        d += finish(cout, v, deg, acc.value());
       }
-      for (unsigned long long m = __ballot(v < V && deg >= kPrWaveList); m != 0ull; m &= m - 1ull) {
+      for (unsigned long long m = __ballot(v < V && deg >= kPrWaveList && !heavy(deg)); m != 0ull; m &= m - 1ull) {
         const int o = __ffsll(static_cast<long long>(m)) - 1;
         const int x = c * kWave + o;
         const int xb = wave_bcast(beg, o), xd = wave_bcast(deg, o);
         CSum acc;
         for (int e = xb + lane; e < xb + xd; e += kWave) acc.add(ld_relaxed(&cin[T.nlist[e]]));
         acc = wave_merge(acc);
-        if (lane == 0) d += finish(x, xd, acc.value());
+        if (lane == 0) d += finish(cout, x, xd, acc.value());
       }
+    }
+    if (nitems > 0) {
+      for_items(T, tm, nitems, [&](int, int eb, int ee, int i) {
+        CSum acc;
+        for (int e = eb + lane; e < ee; e += kWave) acc.add(ld_relaxed(&cin[T.nlist[e]]));
+        acc = wave_merge(acc);
+        if (lane == 0) {
+          st_relaxed(&hpart[i].x, acc.s);
+          st_relaxed(&hpart[i].y, acc.c);
+        }
+      });
+      if (!team_sync(tm)) return false;
+      // one wave per heavy vertex (at its first item) combines its items in order
+      for_items(T, tm, nitems, [&](int v, int eb, int, int i) {
+        if (eb != T.nindex[v]) return;
+        const int deg = T.nindex[v + 1] - eb;
+        const int n = (deg + kItem - 1) / kItem;
+        CSum acc;
+        for (int j = lane; j < n; j += kWave) acc.merge(ld_relaxed(&hpart[i + j].x), ld_relaxed(&hpart[i + j].y));
+        acc = wave_merge(acc);
+        if (lane == 0) d += finish(cout, v, deg, acc.value());
+      });
     }
     // the member's FP64 partial (fixed order: wave tree, then waves in order)
     d = wave_sum(d);
@@ -542,19 +810,35 @@ __global__ __launch_bounds__(kBatchBlock) void batch_kernel(const BatchTask* __r
   }
 }
 
-size_t scratch_bytes(int alg, int nodes)
+size_t items_cap(int edges) { return 2 * (static_cast<size_t>(edges) / kItem) + 2; }
+
+// Per-task scratch: the algorithm's arrays (alg_bytes), then the heavy items.
+size_t alg_bytes(int alg, int nodes, int edges)
 {
   const size_t v = static_cast<size_t>(nodes);
-  if (alg == INDIGO_QUERY_BFS) return Carver::round(3 * v * 8);
-  if (alg == INDIGO_QUERY_SSSP) return Carver::round(3 * v * 4);
-  if (alg == INDIGO_QUERY_PR) return Carver::round(2 * v * 4 + 8 + 2 * kMaxGang * 8);
-  return 0;
+  size_t b = 0;
+  if (alg == INDIGO_QUERY_BFS) b = 4 * v * 8 + 2 * v * 4;             // seen, 2 frontiers, next; 2 queues
+  else if (alg == INDIGO_QUERY_SSSP) b = 6 * v * 4;                  // 3 active masks, 3 queues
+  else if (alg == INDIGO_QUERY_MIS) b = 2 * v * 4;                   // heavy-vertex flags
+  else if (alg == INDIGO_QUERY_PR) b = 2 * kMaxGang * 8 + 2 * v * 4 + items_cap(edges) * 8;  // slots, contrib, partials
+  return Carver::round(b);
 }
 
-int gang_for(long long entries, int grid)
+size_t scratch_bytes(int alg, int nodes, int edges)
 {
+  return alg_bytes(alg, nodes, edges) + Carver::round(items_cap(edges) * sizeof(int2));
+}
+
+int gang_for(int alg, int nodes, int edges, int grid)
+{
+  const long long entries = static_cast<long long>(edges) + nodes;
+  int cap = kMaxGang;
+  if (alg == INDIGO_QUERY_BFS || alg == INDIGO_QUERY_SSSP) {
+    cap = alg == INDIGO_QUERY_BFS ? 128 : 64;
+    if (alg == INDIGO_QUERY_BFS && static_cast<long long>(edges) < 6LL * nodes) cap = 16;
+  }
   int k = 1;
-  while (k < kMaxGang && (static_cast<long long>(k) << kGangShift) < entries) k <<= 1;
+  while (k < cap && (static_cast<long long>(k) << kGangShift) < entries) k <<= 1;
   return std::min(k, std::max(1, grid));
 }
 
@@ -641,7 +925,8 @@ int batch_launch(indigo_ctx* ctx, const indigo_graph_ref* graphs, int ngraphs, c
   }
   std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return work[a] > work[b]; });
   size_t scratch_total = 0;
-  for (int q = 0; q < nqueries; q++) scratch_total += scratch_bytes(queries[q].alg, graphs[queries[q].graph].nodes);
+  for (int q = 0; q < nqueries; q++)
+    scratch_total += scratch_bytes(queries[q].alg, graphs[queries[q].graph].nodes, graphs[queries[q].graph].edges);
   const size_t n = static_cast<size_t>(nqueries);
   const size_t bytes = Carver::round(sizeof(BatchControl)) + Carver::round(n * sizeof(GangState)) +
                        Carver::round(n * sizeof(BatchTask)) + scratch_total + 4096;
@@ -660,7 +945,7 @@ int batch_launch(indigo_ctx* ctx, const indigo_graph_ref* graphs, int ngraphs, c
     BatchTask& T = h_tasks[i];
     std::memset(&T, 0, sizeof(T));
     T.alg = Q.alg;
-    T.gang = gang_for(work[order[i]], grid_cap);
+    T.gang = gang_for(Q.alg, G.nodes, G.edges, grid_cap);
     T.nodes = G.nodes;
     T.edges = G.edges;
     T.nindex = G.d_nindex;
@@ -668,7 +953,8 @@ int batch_launch(indigo_ctx* ctx, const indigo_graph_ref* graphs, int ngraphs, c
     T.eweight = G.d_eweight;
     T.out = Q.d_out;
     T.scratch = scratch + off;
-    off += scratch_bytes(Q.alg, G.nodes);
+    T.items = reinterpret_cast<int2*>(scratch + off + alg_bytes(Q.alg, G.nodes, G.edges));
+    off += scratch_bytes(Q.alg, G.nodes, G.edges);
     T.nsrc = (Q.alg == INDIGO_QUERY_BFS || Q.alg == INDIGO_QUERY_SSSP) ? Q.nsrc : 0;
     for (int s = 0; s < T.nsrc; s++) T.src[s] = Q.src[s];
   }

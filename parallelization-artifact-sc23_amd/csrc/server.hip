@@ -7,13 +7,18 @@
 // arena, first fit): BFS and SSSP requests on the same graph become one
 // multi-source task (up to 64 / 32 sources), CC, MIS and PR one task each,
 // all in one batched launch (indigo_batch); a triangle count at the head of
-// the queue takes the slot alone (its kernels are ordinary launches).  The
+// the queue takes the slot alone (its kernels are ordinary launches), and so
+// does an SSSP request on a deep, sparse graph (average degree below 6, at
+// least 65536 vertices: grids, road networks), on the single-query
+// near-far kernel: the batched Bellman-Ford rounds re-relax such graphs many
+// times over, while near-far settles them in distance order.  The
 // other slots keep running, so requests that arrive while the GPU is busy
 // are admitted as soon as a slot and memory are free, and their work
 // overlaps on the device.  Completion is polled with events; results are
 // copied to the caller's buffers and the arena space released.
 #include <algorithm>
 #include <chrono>
+#include <cstdio>
 #include <map>
 #include <string>
 #include <thread>
@@ -22,6 +27,7 @@
 #include "server.hpp"
 
 #include "batch.hpp"
+#include "sssp.hpp"
 #include "runtime.hpp"
 #include "tc.hpp"
 
@@ -105,12 +111,13 @@ struct Route {
 
 struct Slot {
   indigo_ctx* ctx = nullptr;
-  int state = 0;  // 0 idle, 1 batch, 2 triangle count
+  int state = 0;  // 0 idle, 1 batch, 2 triangle count, 3 single SSSP
   std::vector<int> reqs;
   std::vector<Piece> pieces;
   std::vector<indigo_query> queries;
   std::vector<Route> routes;
   int tc_req = -1;
+  int single_req = -1;
 };
 
 struct Server {
@@ -127,6 +134,12 @@ struct Server {
   Clock::time_point t0;
 
   double now() const { return std::chrono::duration<double>(Clock::now() - t0).count(); }
+
+  bool single(const indigo_request& r) const
+  {
+    const indigo_graph_ref& G = graphs[r.graph];
+    return r.alg == INDIGO_QUERY_SSSP && G.nodes >= (1 << 16) && static_cast<long long>(G.edges) < 6LL * G.nodes;
+  }
 
   size_t out_bytes(const indigo_request& r) const
   {
@@ -149,7 +162,7 @@ struct Server {
     int count = 0;
     for (size_t i = 0; i < waiting.size() && count < cfg.max_batch; i++) {
       const indigo_request& r = reqs[waiting[i]];
-      if (r.alg == INDIGO_QUERY_TC) continue;
+      if (r.alg == INDIGO_QUERY_TC || single(r)) continue;
       const size_t b = round_up(out_bytes(r));
       if (need + b > arena.free_bytes()) break;  // keep arrival order: no overtaking
       const bool multi = r.alg == INDIGO_QUERY_BFS || r.alg == INDIGO_QUERY_SSSP;
@@ -233,10 +246,71 @@ struct Server {
     return INDIGO_OK;
   }
 
+  // An SSSP request on a deep graph alone on slot s; launched false if its
+  // output does not fit yet.
+  int dispatch_single(Slot& s, int q, bool* launched)
+  {
+    *launched = false;
+    const indigo_request& r = reqs[q];
+    const indigo_graph_ref& G = graphs[r.graph];
+    size_t off = 0;
+    if (!arena.alloc(out_bytes(r), &off)) return INDIGO_OK;
+    waiting.erase(std::find(waiting.begin(), waiting.end(), q));
+    reqs[q].admitted = now();
+    reqs[q].batch = 1;
+    reqs[q].slot = static_cast<int>(&s - slots.data());
+    s.pieces.assign(1, Piece{off, out_bytes(r)});
+    s.routes.assign(1, Route{q, arena.ptr(off), out_bytes(r)});
+    s.reqs.assign(1, q);
+    s.single_req = q;
+    IG_TRY(ensure_workspace(s.ctx, sssp_workspace_bytes(G.nodes)));
+    IG_TRY(ensure_host_pinned(s.ctx, 4096));
+    IG_TRY(timing_begin(s.ctx));
+    IG_TRY(sssp_run(s.ctx, G.nodes, G.edges, G.d_nindex, G.d_nlist, G.d_eweight, r.src,
+                    reinterpret_cast<int*>(arena.ptr(off))));
+    IG_TRY(timing_record_stop(s.ctx));
+    s.state = 3;
+    stats.launches++;
+    *launched = true;
+    return INDIGO_OK;
+  }
+
   int complete(Slot& s, int* done)
   {
+    if (s.state == 3) {
+      IG_TRY(timing_finish(s.ctx, 2, 0));
+      const indigo_request& r = reqs[s.single_req];
+      const indigo_graph_ref& G = graphs[r.graph];
+      int it = 0, ep = 0;
+      int rc = sssp_collect(s.ctx, &it, &ep);
+      while (rc == kNotResident) {  // the grid did not assemble: again, with the lowered cap
+        IG_TRY(timing_begin(s.ctx));
+        IG_TRY(sssp_run(s.ctx, G.nodes, G.edges, G.d_nindex, G.d_nlist, G.d_eweight, r.src,
+                        reinterpret_cast<int*>(const_cast<void*>(s.routes[0].src))));
+        IG_TRY(timing_end(s.ctx, 2, 0));
+        rc = sssp_collect(s.ctx, &it, &ep);
+      }
+      IG_TRY(rc);
+      return finish_routes(s, done);
+    }
     if (s.state == 1) {
       IG_TRY(batch_wait(s.ctx));
+      return finish_routes(s, done);
+    }
+    IG_TRY(timing_finish(s.ctx, 0, 1));
+    long long count = 0;
+    IG_TRY(tc_collect(s.ctx, &count));
+    indigo_request& r = reqs[s.tc_req];
+    if (r.d_out != nullptr) IG_HIP_TRY(hipMemcpy(r.d_out, &count, sizeof(count), hipMemcpyHostToDevice));
+    r.finished = now();
+    *done += 1;
+    s.state = 0;
+    return INDIGO_OK;
+  }
+
+  int finish_routes(Slot& s, int* done)
+  {
+    {
       bool copies = false;
       for (const Route& rt : s.routes) {
         if (reqs[rt.req].d_out == nullptr) continue;
@@ -248,15 +322,6 @@ struct Server {
       for (int q : s.reqs) reqs[q].finished = t;
       for (const Piece& p : s.pieces) arena.release(p.off, p.bytes);
       *done += static_cast<int>(s.reqs.size());
-    } else {
-      IG_TRY(timing_finish(s.ctx, 0, 1));
-      long long count = 0;
-      IG_TRY(tc_collect(s.ctx, &count));
-      indigo_request& r = reqs[s.tc_req];
-      if (r.d_out != nullptr)
-        IG_HIP_TRY(hipMemcpy(r.d_out, &count, sizeof(count), hipMemcpyHostToDevice));
-      r.finished = now();
-      *done += 1;
     }
     s.state = 0;
     return INDIGO_OK;
@@ -269,8 +334,14 @@ struct Server {
     std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return reqs[a].arrival < reqs[b].arrival; });
     int next = 0, done = 0;
     t0 = Clock::now();
+    double next_report = cfg.progress_interval;
     while (done < nreqs) {
       const double t = now();
+      if (cfg.progress_interval > 0.0 && t >= next_report) {
+        fprintf(stderr, "serve: %d of %d requests done, %zu waiting, %.1f s\n", done, nreqs, waiting.size(), t);
+        fflush(stderr);
+        next_report = t + cfg.progress_interval;
+      }
       while (next < nreqs && reqs[order[next]].arrival <= t) waiting.push_back(order[next++]);
       stats.max_waiting = std::max(stats.max_waiting, static_cast<int>(waiting.size()));
       bool progressed = false;
@@ -286,6 +357,13 @@ struct Server {
         if (s.state != 0 || waiting.empty()) continue;
         if (reqs[waiting.front()].alg == INDIGO_QUERY_TC) {
           IG_TRY(dispatch_tc(s, waiting.front()));
+          progressed = true;
+          continue;
+        }
+        if (single(reqs[waiting.front()])) {
+          bool launched = false;
+          IG_TRY(dispatch_single(s, waiting.front(), &launched));
+          if (!launched) break;  // its output does not fit until something completes
           progressed = true;
           continue;
         }
@@ -331,13 +409,18 @@ int serve_run(const indigo_graph_ref* graphs, int ngraphs, indigo_request* reqs,
     if (rc != INDIGO_OK) break;
     // the GPU is shared: give persistent grids time to assemble
     s.ctx->hints[INDIGO_HINT_RESIDENCY_MS] = 2000;
+    // single-query persistent kernels take a share of the CUs, not the GPU
+    s.ctx->hints[INDIGO_HINT_BLOCKS_PER_CU] = 1;
   }
   // workspaces sized up front for the largest triangle count, so that no
   // slot reallocates (hipFree waits for the device) while others run
   size_t ws = 0;
   for (int i = 0; i < nreqs; i++)
-    if (reqs[i].alg == INDIGO_QUERY_TC)
-      ws = std::max(ws, tc_workspace_bytes(graphs[reqs[i].graph].nodes, graphs[reqs[i].graph].edges));
+  {
+    const indigo_graph_ref& G = graphs[reqs[i].graph];
+    if (reqs[i].alg == INDIGO_QUERY_TC) ws = std::max(ws, tc_workspace_bytes(G.nodes, G.edges));
+    if (sv.single(reqs[i])) ws = std::max(ws, sssp_workspace_bytes(G.nodes));
+  }
   for (Slot& s : sv.slots)
     if (rc == INDIGO_OK && s.ctx != nullptr) rc = ensure_workspace(s.ctx, std::max(ws, size_t(1) << 28));
   int limit = 0;

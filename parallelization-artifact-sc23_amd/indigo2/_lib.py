@@ -101,7 +101,8 @@ class RequestStruct(ctypes.Structure):
 
 
 class ServeConfigStruct(ctypes.Structure):
-    _fields_ = [("device", c_int), ("slots", c_int), ("max_batch", c_int), ("mem_budget", c_size_t)]
+    _fields_ = [("device", c_int), ("slots", c_int), ("max_batch", c_int), ("mem_budget", c_size_t),
+                ("progress_interval", c_double)]
 
 
 class ServeStatsStruct(ctypes.Structure):

@@ -129,3 +129,22 @@ def test_batch_pagerank(ig, orc, ctx):
         _pr_check(ig, orc, g, scores, iters)
         _, it32 = orc.pr_ref32(g)
         assert abs(it32 - iters) <= 1, (g.name, it32, iters)
+
+
+@pytest.mark.parametrize("cap", [0, 100])
+def test_batch_big_gangs(ig, orc, ctx, cap):
+    """R-MAT 20 (31M CSR entries): PR, CC and MIS take gangs of 128
+    workgroups and the two-level gang barrier, uncapped and with the grid
+    capped at 100 (gangs of 100: not a power of two); BFS / SSSP 64 / 128."""
+    from test_gpu_kernels import _pr_check
+
+    g = ig.rmat(20, 16, seed=12, wmin=1, wmax=255)
+    dg = ig.DeviceGraph(g)
+    rng = np.random.default_rng(12)
+    qs = [ig.Query("bfs", 0, dg, rng.integers(0, g.nodes, 16)), ig.Query("sssp", 0, dg, rng.integers(0, g.nodes, 4)),
+          ig.Query("cc", 0, dg), ig.Query("mis", 0, dg), ig.Query("pr", 0, dg)]
+    print(f"batch on rmat20, cap {cap}", flush=True)
+    ctx.batch([dg], qs, max_workgroups=cap)
+    _check_against_plain(ctx, [g], [dg], qs[:4])
+    scores, iters = qs[4].result()
+    _pr_check(ig, orc, g, scores, iters)

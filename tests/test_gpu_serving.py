@@ -14,6 +14,8 @@ def _pool(ig):
     gs = [ig.rmat(s, 8, seed=20 + s, wmin=1, wmax=255) for s in (9, 10, 11, 12, 13)]
     gs += [ig.grid(r, c, seed=r, wmin=1, wmax=1000) for r, c in ((16, 16), (32, 20), (64, 64))]
     gs += [ig.erdos_renyi(3000, 12000, seed=4, wmin=1, wmax=255), weighted(ig, ig.road(30, 30, seed=2))]
+    # deep, sparse graphs: thousands of levels for the frontier-driven tasks
+    gs += [ig.grid(300, 300, seed=7, wmin=1, wmax=255), weighted(ig, ig.road(260, 260, seed=3))]
     return gs
 
 
@@ -69,7 +71,7 @@ def test_serve_mix_equals_alone(ig, orc, ctx):
     algs, gid, src, arrivals = _requests(rng, gs, 400, rate=20000.0)
     outs = [_out(ig, a, gs[g]) for a, g in zip(algs, gid)]
     print("serving 400 requests on 4 slots", flush=True)
-    adm, fin, batch, slot, st = ig.serve(dgs, algs, gid, src, arrivals, outs, slots=4, max_batch=64)
+    adm, fin, batch, slot, st = ig.serve(dgs, algs, gid, src, arrivals, outs, slots=4, max_batch=64, progress=5.0)
     assert np.all(adm >= arrivals - 1e-9) and np.all(fin >= adm)
     assert batch.max() > 1  # requests were batched together
     assert len(set(slot.tolist())) > 1  # and spread over slots
