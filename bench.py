@@ -359,6 +359,68 @@ def measure_serving(rates=(1000.0,), nq: int = 10000, seed: int = 5, slots: int 
             "slots": slots, "max_batch": max_batch, "mem_budget": mem_budget, "runs": runs}
 
 
+def measure_deep(kind: str, steps: int, warmup: int) -> dict:
+    """High-diameter single-graph workloads, each algorithm from vertex 0.
+    grid: BFS on the 1024 x 1024 4-neighbour grid (row-major ids, sorted
+    lists); the result is checked against the closed form r + c.
+    road: BFS, SSSP and CC on the 4096 x 4096 road-like grid (each
+    4-neighbour edge kept with probability 0.85, one diagonal edge for 2% of
+    the vertices, weights uniform [1, 1000], seed 1); reports the size of
+    vertex 0's component.  Times are medians over `steps` calls with device-
+    resident inputs and outputs; the roofline uses the same per-entry bytes
+    as the R-MAT line."""
+    import numpy as np
+
+    import indigo2 as ig
+
+    t0 = time.time()
+    if kind == "grid":
+        g = ig.grid(1024, 1024, seed=1, wmin=1, wmax=255)
+        algs = ("bfs",)
+        desc = "BFS from vertex 0 on the 1024x1024 4-neighbour grid"
+    else:
+        g = ig.road(4096, 4096, keep_p=0.85, diag_p=0.02, seed=1, wmin=1, wmax=1000)
+        algs = ("bfs", "sssp", "cc")
+        desc = ("BFS, SSSP and CC from vertex 0 on the 4096x4096 road-like grid (keep .85, diagonals .02, "
+                "weights [1,1000], seed 1)")
+    log(f"{kind}: V={g.nodes} E={g.edges} ({time.time() - t0:.1f}s)")
+    dg = ig.DeviceGraph(g)
+    out = ig.DeviceBuffer(4 * g.nodes)
+    extra = {}
+    with ig.Context(0) as ctx:
+        calls = {"bfs": lambda: ctx.bfs_into(dg, 0, out), "sssp": lambda: ctx.sssp_into(dg, 0, out),
+                 "cc": lambda: ctx.cc_into(dg, out)}
+        res = {}
+        for a in algs:
+            for _ in range(warmup):
+                calls[a]()
+            ms, kms = [], []
+            for _ in range(steps):
+                calls[a]()
+                ms.append(ctx.last_ms)
+                name, k = ctx.last_kernel
+                kms.append(k)
+            b = kernel_bytes(a, g, 0, {})
+            km = statistics.median(kms)
+            res[a] = {"call_ms": round(statistics.median(ms), 4), "gteps": round(1e-9 * g.edges / (statistics.median(ms) * 1e-3), 4),
+                      "kernel": name, "kernel_ms": round(km, 4), "algorithmic_bytes": b,
+                      "frac": round(b / (km * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+            log(f"{kind} {a}: {res[a]['call_ms']:.3f} ms, {res[a]['gteps']:.3f} GTEPS")
+        if kind == "grid":
+            dist = ctx.bfs(dg, 0)
+            r, c = np.divmod(np.arange(g.nodes, dtype=np.int64), 1024)
+            extra["closed_form_ok"] = bool(np.array_equal(dist, (r + c).astype(np.int32)))
+            extra["levels"] = int(dist.max()) + 1
+        else:
+            labels = ctx.cc(dg)
+            extra["component_of_0"] = int(np.count_nonzero(labels == labels[0]))
+            dist = ctx.bfs(dg, 0)
+            extra["bfs_levels"] = int(dist[dist != np.iinfo(np.int32).max].max()) + 1
+    dg.close()
+    return {"workload": desc, "vertices": g.nodes, "csr_entries": g.edges, "steps": steps, "per_algorithm": res,
+            **extra}
+
+
 def cpu_baseline(scale: int, threads: int) -> dict | None:
     """The reference's own best C++-threads variants (compiled from its
     sources into oracle/_ref) on a bounded sample of the workload."""
@@ -410,8 +472,10 @@ def main() -> int:
     ap.add_argument("--rates", default="500,1000,2000", help="serving workload: arrival rates in requests/s")
     ap.add_argument("--slots", type=int, default=4, help="serving workload: concurrent slots")
     ap.add_argument("--max-batch", type=int, default=256, help="serving workload: requests per launch")
-    ap.add_argument("--workload", choices=["rmat", "batched", "serving"], default="rmat",
-                    help="rmat: the six algorithms on the R-MAT graph (the default line); batched: batched vs plain")
+    ap.add_argument("--workload", choices=["rmat", "batched", "serving", "grid", "road"], default="rmat",
+                    help="rmat: the six algorithms on the R-MAT graph (the default line); batched: batched vs "
+                         "plain; serving: request loop; grid: BFS on the 1024^2 grid; road: BFS/SSSP/CC on the "
+                         "4096^2 road-like grid")
     ap.add_argument("--profile-only", action="store_true",
                     help="run the timed steps and stop (no TC model, no JSON): the command profiled by rocprofv3")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -430,6 +494,9 @@ def main() -> int:
 
     if args.workload == "batched":
         return run_batched(args)
+    if args.workload in ("grid", "road"):
+        print(json.dumps(measure_deep(args.workload, args.steps, args.warmup)), flush=True)
+        return 0
     if args.workload == "serving":
         print(json.dumps(measure_serving(tuple(float(r) for r in args.rates.split(",")), slots=args.slots,
                                          max_batch=args.max_batch)), flush=True)

@@ -194,6 +194,7 @@ int indigo_tc(indigo_ctx* ctx, int nodes, int edges, const int* d_nindex, const 
 #define INDIGO_QUERY_PR 5   /* PageRank from 1/nodes, same recurrence and stop rule as indigo_pagerank */
 #define INDIGO_MAX_BFS_SOURCES 64
 #define INDIGO_MAX_SSSP_SOURCES 32
+#define INDIGO_MAX_GANG 256
 
 typedef struct indigo_graph_ref {
   int nodes, edges;
@@ -206,6 +207,10 @@ typedef struct indigo_query {
   int alg;   /* INDIGO_QUERY_* */
   int graph; /* index into the graph table */
   int nsrc;  /* BFS 1..64, SSSP 1..32 sources; ignored for CC / MIS */
+  /* workgroups that run this query together: 0 = the engine's rule
+   * (indigo_batch_gang), else 1..INDIGO_MAX_GANG (an autotuner's choice);
+   * clamped to the launch's grid */
+  int gang;
   int src[INDIGO_MAX_BFS_SOURCES];
   /* device output: BFS / SSSP int32[nsrc][nodes], source-major (row i is the
    * distance array of src[i], INT_MAX = unreachable); CC label[nodes]
@@ -220,6 +225,10 @@ typedef struct indigo_query {
  * batches can share the GPU on different contexts. */
 int indigo_batch(indigo_ctx* ctx, const indigo_graph_ref* graphs, int ngraphs, const indigo_query* queries,
                  int nqueries, int max_workgroups);
+
+/* The engine's own gang size for a query of `alg` on a graph of this size
+ * (before clamping to the grid), or 0 for an unknown algorithm.  Host only. */
+int indigo_batch_gang(int alg, int nodes, int edges);
 
 /* ---- serving ----------------------------------------------------------------- */
 /* A request loop over resident graphs (no counterpart in the reference).

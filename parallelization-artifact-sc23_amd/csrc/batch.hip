@@ -84,7 +84,7 @@ namespace {
 
 constexpr int kBatchBlock = 512;
 constexpr int kBatchWaves = kBatchBlock / kWave;
-constexpr int kMaxGang = 256;
+constexpr int kMaxGang = INDIGO_MAX_GANG;
 constexpr int kGangShift = 18;   // CSR entries per gang member
 constexpr int kHeavyDeg = 1024;  // lists this long are split into items
 constexpr int kItem = 1024;      // entries per item
@@ -829,8 +829,11 @@ size_t scratch_bytes(int alg, int nodes, int edges)
   return alg_bytes(alg, nodes, edges) + Carver::round(items_cap(edges) * sizeof(int2));
 }
 
-int gang_for(int alg, int nodes, int edges, int grid)
+}  // namespace
+
+int batch_gang(int alg, int nodes, int edges)
 {
+  if (alg < INDIGO_QUERY_BFS || alg > INDIGO_QUERY_PR || nodes < 0 || edges < 0) return 0;
   const long long entries = static_cast<long long>(edges) + nodes;
   int cap = kMaxGang;
   if (alg == INDIGO_QUERY_BFS || alg == INDIGO_QUERY_SSSP) {
@@ -839,8 +842,12 @@ int gang_for(int alg, int nodes, int edges, int grid)
   }
   int k = 1;
   while (k < cap && (static_cast<long long>(k) << kGangShift) < entries) k <<= 1;
-  return std::min(k, std::max(1, grid));
+  return k;
 }
+
+namespace {
+
+int gang_for(int alg, int nodes, int edges, int grid) { return std::min(batch_gang(alg, nodes, edges), std::max(1, grid)); }
 
 }  // namespace
 
@@ -908,6 +915,8 @@ int batch_launch(indigo_ctx* ctx, const indigo_graph_ref* graphs, int ngraphs, c
     } else if (Q.alg != INDIGO_QUERY_CC && Q.alg != INDIGO_QUERY_MIS && Q.alg != INDIGO_QUERY_PR) {
       return set_error(INDIGO_ERR_INVALID_ARG, "batch: query %d has unknown algorithm %d", q, Q.alg);
     }
+    if (Q.gang < 0 || Q.gang > kMaxGang)
+      return set_error(INDIGO_ERR_INVALID_ARG, "batch: query %d gang %d outside 0..%d", q, Q.gang, kMaxGang);
   }
   // grid: occupancy x CUs (capped), learned residency caps as for the other persistent kernels
   int per_cu = 0;
@@ -945,7 +954,7 @@ int batch_launch(indigo_ctx* ctx, const indigo_graph_ref* graphs, int ngraphs, c
     BatchTask& T = h_tasks[i];
     std::memset(&T, 0, sizeof(T));
     T.alg = Q.alg;
-    T.gang = gang_for(Q.alg, G.nodes, G.edges, grid_cap);
+    T.gang = Q.gang > 0 ? std::min(Q.gang, grid_cap) : gang_for(Q.alg, G.nodes, G.edges, grid_cap);
     T.nodes = G.nodes;
     T.edges = G.edges;
     T.nindex = G.d_nindex;

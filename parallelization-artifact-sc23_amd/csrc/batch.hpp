@@ -19,5 +19,7 @@ int batch_ready(indigo_ctx* ctx, int* ready);
 int batch_wait(indigo_ctx* ctx);
 // Workgroups a batched launch can keep resident on the whole device.
 int batch_grid_limit(indigo_ctx* ctx, int* limit);
+// The engine's gang size for a query (before clamping to the grid).
+int batch_gang(int alg, int nodes, int edges);
 
 }  // namespace indigo

@@ -411,6 +411,8 @@ int indigo_batch(indigo_ctx* ctx, const indigo_graph_ref* graphs, int ngraphs, c
   return batch_run(ctx, graphs, ngraphs, queries, nqueries, max_workgroups);
 }
 
+int indigo_batch_gang(int alg, int nodes, int edges) { return batch_gang(alg, nodes, edges); }
+
 int indigo_serve(const indigo_graph_ref* graphs, int ngraphs, indigo_request* reqs, int nreqs,
                  const indigo_serve_config* cfg, indigo_serve_stats* stats)
 {

@@ -65,7 +65,7 @@ class GraphRefStruct(ctypes.Structure):
 
 
 QUERY_BFS, QUERY_SSSP, QUERY_CC, QUERY_MIS, QUERY_PR = 1, 2, 3, 4, 5
-MAX_BFS_SOURCES, MAX_SSSP_SOURCES = 64, 32
+MAX_BFS_SOURCES, MAX_SSSP_SOURCES, MAX_GANG = 64, 32, 256
 
 
 class QueryStruct(ctypes.Structure):
@@ -75,6 +75,7 @@ class QueryStruct(ctypes.Structure):
         ("alg", c_int),
         ("graph", c_int),
         ("nsrc", c_int),
+        ("gang", c_int),
         ("src", c_int * MAX_BFS_SOURCES),
         ("d_out", c_void_p),
     ]
@@ -147,6 +148,7 @@ _SIGNATURES = {
     "indigo_ctx_last_kernel": [c_void_p, _PF, ctypes.c_char_p, c_int],
     "indigo_ctx_copy_ms": [c_void_p, c_void_p, c_void_p, c_size_t, _PF],
     "indigo_batch": [c_void_p, POINTER(GraphRefStruct), c_int, POINTER(QueryStruct), c_int, c_int],
+    "indigo_batch_gang": [c_int, c_int, c_int],
     "indigo_serve": [POINTER(GraphRefStruct), c_int, POINTER(RequestStruct), c_int, POINTER(ServeConfigStruct),
                      POINTER(ServeStatsStruct)],
     "indigo_ctx_last_trace": [c_void_p, c_int, _PF, _PU, _PD, _PI, _PI],

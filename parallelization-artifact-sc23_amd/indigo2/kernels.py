@@ -127,7 +127,7 @@ class Context:
         gt = graph_table(graphs)
         qt = (_lib.QueryStruct * max(1, len(queries)))()
         for i, q in enumerate(queries):
-            qt[i].alg, qt[i].graph, qt[i].nsrc = q.alg, q.graph, len(q.sources)
+            qt[i].alg, qt[i].graph, qt[i].nsrc, qt[i].gang = q.alg, q.graph, len(q.sources), q.gang
             for j, s_ in enumerate(q.sources):
                 qt[i].src[j] = s_
             qt[i].d_out = q.out.ptr
@@ -164,13 +164,15 @@ class Query:
     """One query of a batch: alg is "bfs", "sssp", "cc", "mis" or "pr"; graph
     is the index into the batch's graph list; BFS / SSSP take up to 64 / 32
     sources and write an int32 [len(sources), nodes] slab; PR writes float
-    scores followed by the iteration count."""
+    scores followed by the iteration count.  gang = 0 lets the engine size
+    the team of workgroups; an autotuner passes its own choice."""
 
     ALGS = {"bfs": _lib.QUERY_BFS, "sssp": _lib.QUERY_SSSP, "cc": _lib.QUERY_CC, "mis": _lib.QUERY_MIS,
             "pr": _lib.QUERY_PR}
 
-    def __init__(self, alg: str, graph: int, g: DeviceGraph, sources=()):
+    def __init__(self, alg: str, graph: int, g: DeviceGraph, sources=(), gang: int = 0):
         self.name = alg
+        self.gang = int(gang)
         self.alg = self.ALGS[alg]
         self.graph = graph
         self.sources = [int(s) for s in sources]
@@ -186,6 +188,11 @@ class Query:
         rows = len(self.sources) if self.name in ("bfs", "sssp") else 1
         a = self.out.to_array(np.int32, rows * self.nodes)
         return a.reshape(rows, self.nodes) if self.name in ("bfs", "sssp") else a
+
+
+def batch_gang(alg: str, nodes: int, edges: int) -> int:
+    """The batched engine's own gang size for a query (indigo_batch_gang)."""
+    return int(_lib.lib().indigo_batch_gang(Query.ALGS[alg], int(nodes), int(edges)))
 
 
 SERVE_ALGS = {"bfs": _lib.QUERY_BFS, "sssp": _lib.QUERY_SSSP, "cc": _lib.QUERY_CC, "mis": _lib.QUERY_MIS,

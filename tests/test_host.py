@@ -122,8 +122,9 @@ def test_batch_structs_match_the_c_layout(ig, tmp_path):
         pytest.skip("no C compiler")
     src = tmp_path / "layout.c"
     src.write_text('#include <stdio.h>\n#include <stddef.h>\n#include "indigo_hip.h"\n'
-                   'int main(void) { printf("%zu %zu %zu %zu %zu\\n", sizeof(indigo_query), '
+                   'int main(void) { printf("%zu %zu %zu %zu %zu %zu\\n", sizeof(indigo_query), '
                    'offsetof(indigo_query, src), offsetof(indigo_query, d_out), sizeof(indigo_graph_ref), '
+                   'offsetof(indigo_query, gang), '
                    'offsetof(indigo_graph_ref, d_eweight)); '
                    'printf("%zu %zu %zu %zu %zu\\n", sizeof(indigo_request), offsetof(indigo_request, d_out), '
                    'offsetof(indigo_request, slot), sizeof(indigo_serve_config), sizeof(indigo_serve_stats)); '
@@ -132,6 +133,6 @@ def test_batch_structs_match_the_c_layout(ig, tmp_path):
     subprocess.run([cc, f"-I{REPO}/include", str(src), "-o", str(exe)], check=True)
     got = [int(x) for x in subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()]
     Q, G, R = ig._lib.QueryStruct, ig._lib.GraphRefStruct, ig._lib.RequestStruct
-    assert got == [ctypes.sizeof(Q), Q.src.offset, Q.d_out.offset, ctypes.sizeof(G), G.d_eweight.offset,
+    assert got == [ctypes.sizeof(Q), Q.src.offset, Q.d_out.offset, ctypes.sizeof(G), Q.gang.offset, G.d_eweight.offset,
                    ctypes.sizeof(R), R.d_out.offset, R.slot.offset, ctypes.sizeof(ig._lib.ServeConfigStruct),
                    ctypes.sizeof(ig._lib.ServeStatsStruct)]
