@@ -95,6 +95,35 @@ def copy_peak_gbs(ig, ctx) -> float:
     return best
 
 
+def measure_host_transfer(ig, ctx, g, dg) -> dict:
+    """The PCIe side of the drop-in form, outside `value`: uploading the
+    graph (nindex + nlist + eweight) through the library's pinned staging
+    (indigo_graph_upload) and through plain pageable copies, building the
+    edge-source array on the device, and copying one int32[V] result back."""
+    nbytes = 4 * (g.nodes + 1) + 8 * g.edges
+    staged = []
+    for _ in range(2):
+        d = ig.DeviceGraph.upload(ctx, g)
+        staged.append(d.upload_seconds)
+        d.close()
+    t = time.perf_counter()
+    d = ig.DeviceGraph(g)
+    pageable = time.perf_counter() - t
+    d.close()
+    sp = dg.build_sp(ctx)
+    sp_ms = ctx.last_ms
+    sp.close()
+    out = ig.DeviceBuffer(4 * g.nodes)
+    t = time.perf_counter()
+    out.to_array(np.int32, g.nodes)
+    d2h = time.perf_counter() - t
+    res = {"graph_bytes": nbytes, "staged_upload_gbs": round(nbytes / min(staged) / 1e9, 2),
+           "pageable_upload_gbs": round(nbytes / pageable / 1e9, 2), "sp_build_ms": round(sp_ms, 3),
+           "result_d2h_ms": round(1e3 * d2h, 3)}
+    log(f"host transfer: {res}")
+    return res
+
+
 def kernel_bytes(alg: str, g, pr_iters: int, tc_model: dict) -> float:
     """Algorithmic HBM bytes of each algorithm's main kernel per launch: every
     CSR entry read once (4 B) plus the per-entry lookup it drives (4 B: the
@@ -163,26 +192,6 @@ def record_traffic(paths: list[str], workload: str, kernel: str, out: str) -> in
     return 0
 
 
-def batched_workload(ig, seed: int = 2024, ngraphs: int = 512):
-    """The batched configuration: `ngraphs` seeded graphs with vertex counts
-    log-uniform in [2^10, 2^20]; even-indexed ones R-MAT edge factor 8
-    (a/b/c/d .57/.19/.19/.05, 2^round(log2 n) vertices: the generator works on
-    powers of two), odd-indexed ones Erdos-Renyi with 4n undirected samples
-    (average degree 8); weights uniform in [1, 255]; per graph 64 BFS sources
-    and 16 SSSP sources drawn uniformly, one CC and one MIS query."""
-    rng = np.random.default_rng(seed)
-    graphs, srcs = [], []
-    for i in range(ngraphs):
-        n = int(round(2.0 ** rng.uniform(10, 20)))
-        if i % 2 == 0:
-            g = ig.rmat(int(round(np.log2(n))), 8, seed=seed + i, wmin=1, wmax=255)
-        else:
-            g = ig.erdos_renyi(n, 4 * n, seed=seed + i, wmin=1, wmax=255)
-        graphs.append(g)
-        srcs.append((rng.integers(0, g.nodes, 64), rng.integers(0, g.nodes, 16)))
-    return graphs, srcs
-
-
 def run_batched(args) -> int:
     line = measure_batched()
     print(json.dumps(line), flush=True)
@@ -197,18 +206,30 @@ def measure_batched() -> dict:
     outputs)."""
     import indigo2 as ig
 
+    from indigo2.autotune import DEFAULT_DB, Tuning, classify
+    from indigo2.workloads import batched_mix
+
     t0 = time.time()
-    graphs, srcs = batched_workload(ig)
+    graphs, srcs = batched_mix()
     tv = sum(g.nodes for g in graphs)
     te = sum(g.edges for g in graphs)
     log(f"batched workload: {len(graphs)} graphs, V={tv} E={te} ({time.time() - t0:.1f}s)")
     dgs = [ig.DeviceGraph(g) for g in graphs]
     ctx = ig.Context(0)
+    # gang sizes per graph class from the autotuner's database, when there is one
+    tuning, classes = None, None
+    if os.path.exists(DEFAULT_DB):
+        tuning = Tuning.load(DEFAULT_DB)
+        classes = classify(ctx, graphs, dgs)
+
+    def gang(alg, i):
+        return tuning.gang(alg, classes[i], graphs[i].nodes, graphs[i].edges) if tuning else 0
+
     out = {}
     plan = {"bfs": lambda i: srcs[i][0], "sssp": lambda i: srcs[i][1], "cc": lambda i: [], "mis": lambda i: []}
     queries_all = {}
     for alg, sel in plan.items():
-        qs = [ig.Query(alg, i, dgs[i], sel(i)) for i in range(len(graphs))]
+        qs = [ig.Query(alg, i, dgs[i], sel(i), gang=gang(alg, i)) for i in range(len(graphs))]
         queries_all[alg] = qs
         teps = sum(max(1, len(q.sources)) * graphs[q.graph].edges for q in qs)
         ctx.batch(dgs, qs)  # warm-up
@@ -288,6 +309,8 @@ def measure_batched() -> dict:
         "speedup_vs_plain": round(plain_all / wall_all, 2),
         "per_algorithm": out,
         "parity_spot_mismatches": mismatches,
+        "autotuned": ({"db": os.path.relpath(DEFAULT_DB, REPO),
+                       "classes": {c: classes.count(c) for c in sorted(set(classes))}} if tuning else None),
     }
     ctx.close()
     return line
@@ -526,8 +549,15 @@ def main() -> int:
     out_i = ig.DeviceBuffer(4 * g.nodes)
     out_f = ig.DeviceBuffer(4 * g.nodes)
     ctx = ig.Context(0)
+    # per-algorithm hints for this graph's class from the autotuner's database
+    from indigo2.autotune import DEFAULT_DB, Tuning, bfs_depth, features, graph_class
+
+    tuning = Tuning.load(DEFAULT_DB) if os.path.exists(DEFAULT_DB) else None
+    klass = graph_class(features(g, bfs_depth(ctx, dg, g))) if tuning else None
 
     def run(alg: str):
+        if tuning is not None:
+            tuning.apply(ctx, alg, klass)
         if alg == "bfs":
             ctx.bfs_into(dg, 0, out_i)
         elif alg == "sssp":
@@ -577,6 +607,7 @@ def main() -> int:
     log("modelling TC work")
     tcm = tc_count_model(g)
     copy_gbs = copy_peak_gbs(ig, ctx)
+    host_io = measure_host_transfer(ig, ctx, g, dg)
     roof = {}
     for a in ALGS:
         b = kernel_bytes(a, g, pr_iters, tcm)
@@ -623,6 +654,9 @@ def main() -> int:
         "per_algorithm_ms": {a: round(med[a], 4) for a in ALGS},
         "pr_iterations": pr_iters,
         "triangles": tri,
+        "autotuned": ({"db": os.path.relpath(DEFAULT_DB, REPO), "class": klass,
+                       "hints": {a: {k: v for k, v in tuning.hints(a, klass).items() if v} for a in ALGS}}
+                      if tuning else None),
         "roofline": {
             "bound": "hbm",
             "kernel": kname[dom],
@@ -644,6 +678,7 @@ def main() -> int:
     }
     ctx.close()
     del dg, out_i, out_f
+    line["host_transfer"] = host_io
     if not args.no_batched:
         try:
             line["batched"] = measure_batched()

@@ -19,6 +19,8 @@
 #ifndef INDIGO_GRAPH_H
 #define INDIGO_GRAPH_H
 
+#include <stddef.h>
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -64,6 +66,23 @@ int indigo_graph_release(indigo_host_graph* g);
  * unlike the header-only loader these return status codes instead of exiting. */
 int indigo_read_egr(const char* path, indigo_host_graph* out);
 int indigo_write_egr(const char* path, const indigo_host_graph* g);
+
+/* A read-only memory map of an .egr file: the arrays point into the
+ * mapping, so nothing is read until it is touched (the reference's
+ * readECLgraph, include/ECLgraph.h:52-77, freads the whole file into
+ * malloc'd arrays).  eweight is NULL when the file carries no weights. */
+typedef struct indigo_egr_map {
+  int nodes;
+  int edges;
+  const int* nindex;
+  const int* nlist;
+  const int* eweight;
+  void* base;    /* the mapping */
+  size_t length; /* its length in bytes */
+} indigo_egr_map;
+
+int indigo_egr_open(const char* path, indigo_egr_map* out);
+int indigo_egr_close(indigo_egr_map* m);
 
 /* Checks the invariants above (sorted, no self loops, symmetric, weights
  * symmetric).  Returns 0 if they hold, else a positive code naming the first

@@ -230,6 +230,24 @@ int indigo_batch(indigo_ctx* ctx, const indigo_graph_ref* graphs, int ngraphs, c
  * (before clamping to the grid), or 0 for an unknown algorithm.  Host only. */
 int indigo_batch_gang(int alg, int nodes, int edges);
 
+/* ---- graph uploads --------------------------------------------------------- */
+/* Copies a host CSR (malloc'd arrays, or an .egr file mapped with
+ * indigo_egr_open from indigo_graph.h) into new device arrays through two
+ * 64 MiB pinned staging buffers on ctx's stream, with the host-side copy of
+ * one chunk overlapping the DMA of the previous one; returns when the graph
+ * is resident.  eweight may be NULL.  seconds (may be NULL) receives the
+ * wall time of the copies.  Release with indigo_graph_free.  (The reference
+ * uploads with blocking cudaMemcpy from malloc'd arrays in each harness,
+ * e.g. include/bfs_vertex_cuda.h:143-144.) */
+int indigo_graph_upload(indigo_ctx* ctx, int nodes, int edges, const int* nindex, const int* nlist,
+                        const int* eweight, indigo_graph_ref* out, double* seconds);
+int indigo_graph_free(indigo_graph_ref* g);
+
+/* d_sp[j] = v for every CSR entry j of vertex v, built on the device (the
+ * edge-source array of the edge-form entries; the reference builds it on the
+ * host, include/bfs_edge_cuda.h:140-146, tc_edge_cuda.h:159-165). */
+int indigo_build_sp(indigo_ctx* ctx, int nodes, int edges, const int* d_nindex, int* d_sp);
+
 /* ---- serving ----------------------------------------------------------------- */
 /* A request loop over resident graphs (no counterpart in the reference).
  * Requests arrive over time; every free slot (a context with its own stream

@@ -12,6 +12,11 @@
 #include <cstdlib>
 #include <cstring>
 #include <thread>
+
+#include <fcntl.h>
+#include <sys/mman.h>
+#include <sys/stat.h>
+#include <unistd.h>
 #include <vector>
 
 namespace {
@@ -352,6 +357,56 @@ int indigo_read_egr(const char* path, indigo_host_graph* out)
   fclose(f);
   if (rc != INDIGO_OK) indigo_graph_release(out);
   return rc;
+}
+
+int indigo_egr_open(const char* path, indigo_egr_map* out)
+{
+  if (path == nullptr || out == nullptr) return set_error(INDIGO_ERR_INVALID_ARG, "egr_open: null argument");
+  std::memset(out, 0, sizeof(*out));
+  const int fd = open(path, O_RDONLY);
+  if (fd < 0) return set_error(INDIGO_ERR_INVALID_ARG, "egr_open: could not open file");
+  struct stat st;
+  if (fstat(fd, &st) != 0 || st.st_size < static_cast<off_t>(2 * sizeof(int))) {
+    close(fd);
+    return set_error(INDIGO_ERR_INVALID_ARG, "egr_open: bad header");
+  }
+  const size_t len = static_cast<size_t>(st.st_size);
+  void* base = mmap(nullptr, len, PROT_READ, MAP_PRIVATE, fd, 0);
+  close(fd);
+  if (base == MAP_FAILED) return set_error(INDIGO_ERR_OUT_OF_MEMORY, "egr_open: mmap failed");
+  const int* w = static_cast<const int*>(base);
+  const long long nodes = w[0], edges = w[1];
+  // the reader's rules (include/ECLgraph.h:58-76): header, nindex, nlist;
+  // then no whole weight means no weights, some but fewer than `edges` is
+  // an error, and bytes after the weights are ignored
+  const size_t need = sizeof(int) * static_cast<size_t>(2 + (nodes + 1) + (edges > 0 ? edges : 0));
+  const size_t extra_ints = len >= need ? (len - need) / sizeof(int) : 0;
+  int rc = INDIGO_OK;
+  if (nodes < 1 || edges < 0) rc = set_error(INDIGO_ERR_INVALID_ARG, "egr_open: bad header");
+  else if (len < need) rc = set_error(INDIGO_ERR_INVALID_ARG, "egr_open: truncated nindex / nlist");
+  else if (extra_ints > 0 && extra_ints < static_cast<size_t>(edges))
+    rc = set_error(INDIGO_ERR_INVALID_ARG, "egr_open: truncated weights");
+  if (rc != INDIGO_OK) {
+    munmap(base, len);
+    return rc;
+  }
+  madvise(base, len, MADV_SEQUENTIAL);
+  out->nodes = static_cast<int>(nodes);
+  out->edges = static_cast<int>(edges);
+  out->nindex = w + 2;
+  out->nlist = w + 2 + nodes + 1;
+  out->eweight = (extra_ints > 0 && edges > 0) ? w + 2 + nodes + 1 + edges : nullptr;
+  out->base = base;
+  out->length = len;
+  return INDIGO_OK;
+}
+
+int indigo_egr_close(indigo_egr_map* m)
+{
+  if (m == nullptr) return INDIGO_OK;
+  if (m->base != nullptr) munmap(m->base, m->length);
+  std::memset(m, 0, sizeof(*m));
+  return INDIGO_OK;
 }
 
 int indigo_write_egr(const char* path, const indigo_host_graph* g)

@@ -312,6 +312,10 @@ int indigo_ctx_destroy(indigo_ctx* ctx)
   (void)hipStreamSynchronize(ctx->stream);
   if (ctx->ws) (void)hipFree(ctx->ws);
   if (ctx->host_pinned) (void)hipHostFree(ctx->host_pinned);
+  for (int b = 0; b < 2; b++) {
+    if (ctx->stage[b]) (void)hipHostFree(ctx->stage[b]);
+    if (ctx->ev_stage[b]) (void)hipEventDestroy(ctx->ev_stage[b]);
+  }
   if (ctx->ev_start) (void)hipEventDestroy(ctx->ev_start);
   if (ctx->ev_stop) (void)hipEventDestroy(ctx->ev_stop);
   if (ctx->ev_k0) (void)hipEventDestroy(ctx->ev_k0);

@@ -48,6 +48,10 @@ struct indigo_ctx {
     int ntasks = 0;
     int grid_cap = 0;
   } pending_batch;
+  // pinned double buffer of staged graph uploads (indigo_graph_upload)
+  void* stage[2] = {nullptr, nullptr};
+  size_t stage_bytes = 0;
+  hipEvent_t ev_stage[2] = {nullptr, nullptr};
   int last_grid = 0;    // workgroups of the last persistent launch
   int last_per_cu = 0;  // co-resident workgroups per CU it was sized for
 };

@@ -40,6 +40,20 @@ class IndigoError(RuntimeError):
         self.code = code
 
 
+class EgrMapStruct(ctypes.Structure):
+    """struct indigo_egr_map: a read-only mapping of an .egr file."""
+
+    _fields_ = [
+        ("nodes", c_int),
+        ("edges", c_int),
+        ("nindex", POINTER(c_int)),
+        ("nlist", POINTER(c_int)),
+        ("eweight", POINTER(c_int)),
+        ("base", c_void_p),
+        ("length", c_size_t),
+    ]
+
+
 class HostGraphStruct(ctypes.Structure):
     """struct indigo_host_graph (layout of the reference's struct ECLgraph)."""
 
@@ -149,6 +163,12 @@ _SIGNATURES = {
     "indigo_ctx_copy_ms": [c_void_p, c_void_p, c_void_p, c_size_t, _PF],
     "indigo_batch": [c_void_p, POINTER(GraphRefStruct), c_int, POINTER(QueryStruct), c_int, c_int],
     "indigo_batch_gang": [c_int, c_int, c_int],
+    "indigo_graph_upload": [c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p, POINTER(GraphRefStruct),
+                            POINTER(ctypes.c_double)],
+    "indigo_graph_free": [POINTER(GraphRefStruct)],
+    "indigo_build_sp": [c_void_p, c_int, c_int, c_void_p, c_void_p],
+    "indigo_egr_open": [c_char_p, POINTER(EgrMapStruct)],
+    "indigo_egr_close": [POINTER(EgrMapStruct)],
     "indigo_serve": [POINTER(GraphRefStruct), c_int, POINTER(RequestStruct), c_int, POINTER(ServeConfigStruct),
                      POINTER(ServeStatsStruct)],
     "indigo_ctx_last_trace": [c_void_p, c_int, _PF, _PU, _PD, _PI, _PI],

@@ -9,7 +9,8 @@ int32.  The generators and the CSR builder run in the library's host C++
 from __future__ import annotations
 
 import ctypes
-from dataclasses import dataclass
+import json
+from dataclasses import dataclass, field
 from typing import Optional
 
 import numpy as np
@@ -25,6 +26,9 @@ class HostGraph:
     nlist: np.ndarray
     eweight: Optional[np.ndarray] = None
     name: str = ""
+    # how the graph was made (generator, sizes, seed, weights), written
+    # beside an .egr file as its metadata sidecar
+    meta: dict = field(default_factory=dict)
 
     def __post_init__(self):
         self.nindex = np.ascontiguousarray(self.nindex, dtype=np.int32)
@@ -44,7 +48,7 @@ class HostGraph:
         return np.repeat(np.arange(self.nodes, dtype=np.int32), self.degree)
 
     def with_weights(self, weights: np.ndarray) -> "HostGraph":
-        return HostGraph(self.nodes, self.edges, self.nindex, self.nlist, weights, self.name)
+        return HostGraph(self.nodes, self.edges, self.nindex, self.nlist, weights, self.name, dict(self.meta))
 
     def _struct(self) -> _lib.HostGraphStruct:
         s = _lib.HostGraphStruct()
@@ -64,8 +68,28 @@ class HostGraph:
         return _lib.load().indigo_graph_check(ctypes.byref(s))
 
     def write_egr(self, path: str) -> None:
+        """Writes the ECLgraph file and, when the graph carries metadata, the
+        sidecar `<path>.meta.json` (generator, parameters, seed, sizes)."""
         s = self._struct()
         _lib.call("indigo_write_egr", str(path).encode(), ctypes.byref(s))
+        if self.meta:
+            with open(meta_path(path), "w") as f:
+                json.dump(dict(self.meta, nodes=self.nodes, edges=self.edges,
+                               weights=self.eweight is not None), f, indent=1, sort_keys=True)
+                f.write("\n")
+
+
+def meta_path(path: str) -> str:
+    return f"{path}.meta.json"
+
+
+def read_meta(path: str) -> Optional[dict]:
+    """The metadata sidecar of an .egr file, or None."""
+    try:
+        with open(meta_path(path)) as f:
+            return json.load(f)
+    except FileNotFoundError:
+        return None
 
 
 def _take(s: _lib.HostGraphStruct, name: str = "") -> HostGraph:
@@ -93,37 +117,48 @@ def from_edges(nodes: int, src, dst, symmetrize: bool = True, wseed: int = 0, wm
     return _take(s, name)
 
 
+def _with_meta(g: HostGraph, **meta) -> HostGraph:
+    g.meta = meta
+    return g
+
+
 def grid(rows: int, cols: int, seed: int = 0, wmin: int = 0, wmax: int = -1) -> HostGraph:
     s = _lib.HostGraphStruct()
     _lib.call("indigo_gen_grid", rows, cols, seed, wmin, wmax, ctypes.byref(s))
-    return _take(s, f"grid{rows}x{cols}")
+    return _with_meta(_take(s, f"grid{rows}x{cols}"), generator="grid", rows=rows, cols=cols, seed=seed,
+                      wmin=wmin, wmax=wmax)
 
 
 def rmat(scale: int, edge_factor: int = 16, a: float = 0.57, b: float = 0.19, c: float = 0.19, seed: int = 1,
          wmin: int = 0, wmax: int = -1, threads: int = 0) -> HostGraph:
     s = _lib.HostGraphStruct()
     _lib.call("indigo_gen_rmat", scale, edge_factor, a, b, c, seed, wmin, wmax, threads, ctypes.byref(s))
-    return _take(s, f"rmat{scale}_ef{edge_factor}")
+    return _with_meta(_take(s, f"rmat{scale}_ef{edge_factor}"), generator="rmat", scale=scale,
+                      edge_factor=edge_factor, a=a, b=b, c=c, seed=seed, wmin=wmin, wmax=wmax)
 
 
 def erdos_renyi(nodes: int, samples: int, seed: int = 1, wmin: int = 0, wmax: int = -1,
                 threads: int = 0) -> HostGraph:
     s = _lib.HostGraphStruct()
     _lib.call("indigo_gen_er", nodes, samples, seed, wmin, wmax, threads, ctypes.byref(s))
-    return _take(s, f"er{nodes}")
+    return _with_meta(_take(s, f"er{nodes}"), generator="erdos_renyi", vertices=nodes, samples=samples, seed=seed,
+                      wmin=wmin, wmax=wmax)
 
 
 def road(rows: int, cols: int, keep_p: float = 0.85, diag_p: float = 0.02, seed: int = 1, wmin: int = 0,
          wmax: int = -1, threads: int = 0) -> HostGraph:
     s = _lib.HostGraphStruct()
     _lib.call("indigo_gen_road", rows, cols, keep_p, diag_p, seed, wmin, wmax, threads, ctypes.byref(s))
-    return _take(s, f"road{rows}x{cols}")
+    return _with_meta(_take(s, f"road{rows}x{cols}"), generator="road", rows=rows, cols=cols, keep_p=keep_p,
+                      diag_p=diag_p, seed=seed, wmin=wmin, wmax=wmax)
 
 
 def read_egr(path: str) -> HostGraph:
     s = _lib.HostGraphStruct()
     _lib.call("indigo_read_egr", str(path).encode(), ctypes.byref(s))
-    return _take(s, str(path))
+    g = _take(s, str(path))
+    g.meta = read_meta(path) or {}
+    return g
 
 
 class DeviceBuffer:
@@ -134,6 +169,13 @@ class DeviceBuffer:
         p = ctypes.c_void_p()
         _lib.call("indigo_malloc", ctypes.byref(p), max(self.nbytes, 4))
         self.ptr = p.value
+
+    @classmethod
+    def adopt(cls, ptr: int, nbytes: int) -> "DeviceBuffer":
+        """Takes ownership of a device allocation made by the library."""
+        buf = cls.__new__(cls)
+        buf.nbytes, buf.ptr = int(nbytes), ptr
+        return buf
 
     @classmethod
     def from_array(cls, arr: np.ndarray) -> "DeviceBuffer":
@@ -168,9 +210,51 @@ class DeviceGraph:
         self.nodes = g.nodes
         self.edges = g.edges
         self.name = g.name
+        self.upload_seconds = None
         self.d_nindex = DeviceBuffer.from_array(g.nindex)
         self.d_nlist = DeviceBuffer.from_array(g.nlist)
         self.d_eweight = DeviceBuffer.from_array(g.eweight) if g.eweight is not None else None
+
+    @classmethod
+    def _from_ref(cls, ref: _lib.GraphRefStruct, name: str, seconds: float) -> "DeviceGraph":
+        self = cls.__new__(cls)
+        self.nodes, self.edges, self.name, self.upload_seconds = ref.nodes, ref.edges, name, seconds
+        self.d_nindex = DeviceBuffer.adopt(ref.d_nindex, 4 * (ref.nodes + 1))
+        self.d_nlist = DeviceBuffer.adopt(ref.d_nlist, 4 * ref.edges)
+        self.d_eweight = DeviceBuffer.adopt(ref.d_eweight, 4 * ref.edges) if ref.d_eweight else None
+        return self
+
+    @classmethod
+    def upload(cls, ctx, g: HostGraph) -> "DeviceGraph":
+        """Staged upload on ctx's stream (indigo_graph_upload)."""
+        ref, sec = _lib.GraphRefStruct(), ctypes.c_double()
+        pi = ctypes.c_void_p
+        _lib.call("indigo_graph_upload", ctx.h, g.nodes, g.edges, pi(g.nindex.ctypes.data), pi(g.nlist.ctypes.data),
+                  pi(g.eweight.ctypes.data) if g.eweight is not None else None, ctypes.byref(ref), ctypes.byref(sec))
+        return cls._from_ref(ref, g.name, sec.value)
+
+    @classmethod
+    def from_file(cls, ctx, path: str) -> "DeviceGraph":
+        """Maps the .egr file (indigo_egr_open) and uploads it straight from
+        the mapping through the staging buffers: no host copy of the graph
+        is made."""
+        m = _lib.EgrMapStruct()
+        _lib.call("indigo_egr_open", str(path).encode(), ctypes.byref(m))
+        try:
+            ref, sec = _lib.GraphRefStruct(), ctypes.c_double()
+            _lib.call("indigo_graph_upload", ctx.h, m.nodes, m.edges, ctypes.cast(m.nindex, ctypes.c_void_p),
+                      ctypes.cast(m.nlist, ctypes.c_void_p),
+                      ctypes.cast(m.eweight, ctypes.c_void_p) if m.eweight else None, ctypes.byref(ref),
+                      ctypes.byref(sec))
+        finally:
+            _lib.load().indigo_egr_close(ctypes.byref(m))
+        return cls._from_ref(ref, str(path), sec.value)
+
+    def build_sp(self, ctx) -> "DeviceBuffer":
+        """The edge-source array on the device (indigo_build_sp)."""
+        sp = DeviceBuffer(4 * max(1, self.edges))
+        _lib.call("indigo_build_sp", ctx.h, self.nodes, self.edges, self.d_nindex.ptr, sp.ptr)
+        return sp
 
     @property
     def footprint_bytes(self) -> int:

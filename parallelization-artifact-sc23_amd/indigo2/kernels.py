@@ -192,7 +192,7 @@ class Query:
 
 def batch_gang(alg: str, nodes: int, edges: int) -> int:
     """The batched engine's own gang size for a query (indigo_batch_gang)."""
-    return int(_lib.lib().indigo_batch_gang(Query.ALGS[alg], int(nodes), int(edges)))
+    return int(_lib.load().indigo_batch_gang(Query.ALGS[alg], int(nodes), int(edges)))
 
 
 SERVE_ALGS = {"bfs": _lib.QUERY_BFS, "sssp": _lib.QUERY_SSSP, "cc": _lib.QUERY_CC, "mis": _lib.QUERY_MIS,

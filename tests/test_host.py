@@ -136,3 +136,94 @@ def test_batch_structs_match_the_c_layout(ig, tmp_path):
     assert got == [ctypes.sizeof(Q), Q.src.offset, Q.d_out.offset, ctypes.sizeof(G), Q.gang.offset, G.d_eweight.offset,
                    ctypes.sizeof(R), R.d_out.offset, R.slot.offset, ctypes.sizeof(ig._lib.ServeConfigStruct),
                    ctypes.sizeof(ig._lib.ServeStatsStruct)]
+
+
+def test_autotune_classes_and_lookup(ig):
+    """Graph classes from the suite's features, database fallbacks, and the
+    gang scaling against the engine's own rule (host-only calls)."""
+    from indigo2 import autotune as at
+
+    g = ig.rmat(10, 8, seed=3, wmin=1, wmax=255)
+    f = at.features(g, diameter=7)
+    assert f["numv"] == g.nodes and f["nume"] == g.edges and f["dMax"] == int(g.degree.max())
+    assert abs(f["dAvg"] - g.edges / g.nodes) < 1e-12
+    assert at.graph_class(f) == "skewed-small"
+    assert at.graph_class(dict(f, diameter=2047)) == "deep-small"
+    er = ig.erdos_renyi(1 << 16, 4 << 16, seed=1)
+    assert at.graph_class(at.features(er, diameter=12)) == "uniform-medium"
+
+    db = at.Tuning({"single": {"skewed-large": {"bfs": {"hints": {"BLOCKS_PER_CU": 2}}}},
+                    "batched": {"uniform-small": {"cc": {"gang_scale": 4.0}}}})
+    h = db.hints("bfs", "skewed-medium")  # same shape, other size: falls back
+    assert h[ig._lib.HINT_BLOCKS_PER_CU] == 2 and h[ig._lib.HINT_BFS_DIRECTION] == 0
+    assert all(v == 0 for v in db.hints("bfs", "deep-large").values())
+    # the engine's rule: one workgroup per 2^18 entries, rounded up to a power of two
+    from indigo2.kernels import batch_gang
+    assert batch_gang("cc", 1 << 20, 7 << 20) == 32
+    assert batch_gang("cc", 1000, 8000) == 1
+    assert batch_gang("bfs", 1 << 22, 1 << 22) == 16  # deep sparse BFS cap
+    assert db.gang("cc", "uniform-small", 1 << 20, 7 << 20) == 128
+    assert db.gang("cc", "skewed-small", 1 << 20, 7 << 20) == 0
+    assert db.gang("bfs", "uniform-small", 1 << 20, 7 << 20) == 0
+    assert at.scaled_gang("pr", 1 << 24, 1 << 28, 4.0) == ig._lib.MAX_GANG
+
+
+def test_egr_mapping_matches_the_reader(ig, tmp_path):
+    """indigo_egr_open gives the same arrays as indigo_read_egr, and follows
+    the reference reader's rules for the optional weights
+    (include/ECLgraph.h:70-76): no whole weight means none, a partial set is
+    an error, trailing bytes after the weights are ignored."""
+    import ctypes
+
+    from indigo2 import _lib
+
+    def mapped(path):
+        m = _lib.EgrMapStruct()
+        _lib.call("indigo_egr_open", str(path).encode(), ctypes.byref(m))
+        try:
+            n, e = m.nodes, m.edges
+            out = (n, e, np.ctypeslib.as_array(m.nindex, shape=(n + 1,)).copy(),
+                   np.ctypeslib.as_array(m.nlist, shape=(max(e, 1),))[:e].copy(),
+                   np.ctypeslib.as_array(m.eweight, shape=(max(e, 1),))[:e].copy() if m.eweight else None)
+        finally:
+            _lib.load().indigo_egr_close(ctypes.byref(m))
+        return out
+
+    g = ig.rmat(9, 8, seed=4, wmin=1, wmax=99)
+    p = tmp_path / "g.egr"
+    g.write_egr(str(p))
+    n, e, ni, nl, ew = mapped(p)
+    h = ig.read_egr(str(p))
+    assert (n, e) == (h.nodes, h.edges)
+    assert np.array_equal(ni, h.nindex) and np.array_equal(nl, h.nlist) and np.array_equal(ew, h.eweight)
+    raw = p.read_bytes()
+    base = 4 * (2 + g.nodes + 1 + g.edges)
+    cases = {"noweights": raw[:base], "stub": raw[:base + 3], "trailing": raw + b"\x01\x02"}
+    for name, data in cases.items():
+        q = tmp_path / f"{name}.egr"
+        q.write_bytes(data)
+        got = mapped(q)
+        ref = ig.read_egr(str(q))
+        assert (got[4] is None) == (ref.eweight is None), name
+        assert name != "trailing" or np.array_equal(got[4], g.eweight)
+    for name, data in {"partial": raw[:base + 8], "short": raw[:base - 4], "header": raw[:4]}.items():
+        q = tmp_path / f"{name}.egr"
+        q.write_bytes(data)
+        with pytest.raises(ig.IndigoError):
+            mapped(q)
+        with pytest.raises(ig.IndigoError):
+            ig.read_egr(str(q))
+
+
+def test_egr_metadata_sidecar(ig, tmp_path):
+    g = ig.road(20, 30, keep_p=0.9, diag_p=0.05, seed=11, wmin=1, wmax=1000)
+    p = tmp_path / "road.egr"
+    g.write_egr(str(p))
+    h = ig.read_egr(str(p))
+    assert h.meta["generator"] == "road" and h.meta["seed"] == 11 and h.meta["rows"] == 20
+    assert h.meta["nodes"] == g.nodes and h.meta["edges"] == g.edges and h.meta["weights"] is True
+    # regenerating from the sidecar gives the same graph
+    m = h.meta
+    again = ig.road(m["rows"], m["cols"], keep_p=m["keep_p"], diag_p=m["diag_p"], seed=m["seed"], wmin=m["wmin"],
+                    wmax=m["wmax"])
+    assert np.array_equal(again.nlist, h.nlist) and np.array_equal(again.eweight, h.eweight)
