@@ -856,8 +856,7 @@ int gang_for(int alg, int nodes, int edges, int grid) { return std::min(batch_ga
 int batch_grid_limit(indigo_ctx* ctx, int* limit)
 {
   int per_cu = 0;
-  IG_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(batch_kernel),
-                                                         kBatchBlock, 0));
+  IG_TRY(resident_per_cu(ctx, reinterpret_cast<const void*>(batch_kernel), kBatchBlock, 0, &per_cu));
   *limit = std::max(1, per_cu * ctx->compute_units);
   return INDIGO_OK;
 }
@@ -920,8 +919,7 @@ int batch_launch(indigo_ctx* ctx, const indigo_graph_ref* graphs, int ngraphs, c
   }
   // grid: occupancy x CUs (capped), learned residency caps as for the other persistent kernels
   int per_cu = 0;
-  IG_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(batch_kernel),
-                                                         kBatchBlock, 0));
+  IG_TRY(resident_per_cu(ctx, reinterpret_cast<const void*>(batch_kernel), kBatchBlock, 0, &per_cu));
   const int grid_cap = std::max(1, max_workgroups > 0 ? std::min(max_workgroups, per_cu * ctx->compute_units)
                                                       : per_cu * ctx->compute_units);
   // tasks, largest gang and work first

@@ -66,12 +66,25 @@ std::mutex g_cap_mu;
 std::unordered_map<const void*, int> g_cap;  // learned co-residency per kernel
 }  // namespace
 
+int resident_per_cu(indigo_ctx* ctx, const void* kernel, int block, size_t shmem, int* per_cu)
+{
+  int n = 0;
+  IG_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kernel, block, shmem));
+  if (n < 1) return set_error(INDIGO_ERR_DEVICE, "kernel cannot be resident with block %d", block);
+  // A grid that needs every wave slot of every CU (4 x 512 threads on a
+  // 2048-thread CU) did not assemble on MI355X: the first such launch waited
+  // out the residency deadline and ran again at one workgroup less per CU.
+  // Leave that workgroup out from the start.
+  if (n > 1 && static_cast<long long>(n) * block >= ctx->max_threads_per_cu) n -= 1;
+  *per_cu = n;
+  return INDIGO_OK;
+}
+
 int coop_launch(indigo_ctx* ctx, const void* kernel, int block, size_t shmem, void** args, int max_blocks,
                 int* grid_out, const char* name)
 {
   int per_cu = 0;
-  IG_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, block, shmem));
-  if (per_cu < 1) return set_error(INDIGO_ERR_DEVICE, "kernel cannot be resident with block %d", block);
+  IG_TRY(resident_per_cu(ctx, kernel, block, shmem, &per_cu));
   {
     std::lock_guard<std::mutex> lock(g_cap_mu);
     auto it = g_cap.find(kernel);

@@ -111,6 +111,9 @@ inline long long budget_ticks(const indigo_ctx* ctx)
 // kernel, timed by kernel_begin / kernel_end.
 int coop_launch(indigo_ctx* ctx, const void* kernel, int block, size_t shmem, void** args, int max_blocks,
                 int* grid_out, const char* name = nullptr);
+// Workgroups of `kernel` that coop_launch places per CU before any learned
+// cap or hint (occupancy, less one when that would fill the CU).
+int resident_per_cu(indigo_ctx* ctx, const void* kernel, int block, size_t shmem, int* per_cu);
 // Internal status: the last persistent launch did not assemble (see
 // residency_barrier); the cap of `kernel` has been lowered, launch again.
 constexpr int kNotResident = -100;

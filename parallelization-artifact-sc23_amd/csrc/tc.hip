@@ -19,16 +19,21 @@
 // so every oriented edge is taken exactly once, by its endpoint with the
 // longer list (2.4x fewer probes than always iterating N+(u) on R-MAT).
 //
-// Passes (plain launches on the context stream):
-//   1. out-count : |N+(v)| per vertex; lists < kMidDeg by one lane in id
-//                  order, longer ones binned into mid (one wave each,
-//                  ballots) and huge (kHugeChunk-entry items, one workgroup
-//                  each, per-item counts kept); also flags unsorted lists
-//                  and self loops;
-//   2. scan      : exclusive prefix -> oindex;
-//   3. out-fill  : N+ lists, order-preserving (so still sorted), same split;
-//   4-6.           the same three steps for the in-lists N-(x);
-//   7. count, wave pass: a wave grabs 64 consecutive vertices at a time from
+// Passes (plain launches on the context stream).  Both oriented lists of a
+// vertex are subsets of its adjacency list, so each is stored at the
+// vertex's own CSR position (olist / ilist[nindex[v] + k]) and one pass per
+// list builds it - no count pass, no scan:
+//   1. out-select : N+ lists, order-preserving (so still sorted); lists
+//                  < kMidDeg by the in-wave balancer, longer ones binned
+//                  into mid (one wave each) and huge (kHugeChunk-entry
+//                  items, one workgroup each, compacted per item); also
+//                  flags unsorted lists and self loops;  records
+//                  (deg, |N+|) per vertex;
+//   2. pack       : one workgroup per huge list closes the gaps between its
+//                  items;
+//   3-4.           the same for the in-lists N-(x), reading each
+//                  neighbour's (deg, |N+|) in one 8-byte load;
+//   5. count, wave pass: a wave grabs 64 consecutive vertices at a time from
 //      a global counter (a static stride would be badly unbalanced: on R-MAT
 //      graphs a vertex's degree follows the bit pattern of its id); a task
 //      with 2 <= |N+(x)| <= kWaveList and at most kWaveWork probes
@@ -36,8 +41,8 @@
 //      table of N+(x) in LDS (load factor <= 1/4, Fibonacci hashing, linear
 //      probing) and walks the (list, element) pairs 64 per step with the
 //      in-wave balancer, one probe each.  Every other task is cut into parts
-//      of kPartItems items (out-list, then in-list), queued for pass 8;
-//   8. count, workgroup passes: a workgroup grabs a part, hashes N+(x) and
+//      of kPartItems items (out-list, then in-list), queued for pass 6;
+//   6. count, workgroup passes: a workgroup grabs a part, hashes N+(x) and
 //      spreads the part's items over its waves (512 threads and a 32 KiB
 //      table up to kMidList entries, 1024 threads and a 128 KiB table
 //      beyond).  Parts bound the work of one grab, so a few heavy vertices
@@ -46,7 +51,6 @@
 #include "tc.hpp"
 
 #include "frontier.hpp"
-#include "scan.hpp"
 
 namespace indigo {
 
@@ -61,6 +65,7 @@ constexpr int kMidSlots = 8192;
 constexpr int kBigBlock = 1024;
 constexpr int kBigSlots = 32768;
 static_assert(kMidBlock == 512 && kMidSlots == 8192, "the main-kernel name below spells these out");
+static_assert(kHugeChunk % kTcBlock == 0, "tc_pack_huge moves whole items");
 constexpr int kWaveWork = 1 << 15;  // probe bound of a wave-pass task
 constexpr int kPartItems = 1024;    // items per workgroup-pass part
 constexpr int kUnroll = 4;          // windows of 64 per step over a long list
@@ -77,86 +82,130 @@ struct TcControl {
 
 IG_DEV bool outranks(int du, int u, int dv, int v) { return du > dv || (du == dv && u > v); }
 
-IG_DEV int deg(const int* __restrict__ nindex, int v) { return nindex[v + 1] - nindex[v]; }
+// Orientation state of a vertex, written by the out-selection: its degree and
+// |N+(v)|.  The in-selection reads it for every neighbour in one 8-byte load.
+struct alignas(8) TcVert {
+  int deg, odeg;
+};
 
 // Out-list of v: the neighbours that outrank v.
 struct OutSel {
-  IG_DEV bool operator()(const int* __restrict__ nindex, int v, int dv, int u) const
-  {
-    return outranks(deg(nindex, u), u, dv, v);
-  }
+  const int* __restrict__ nindex;
+  IG_DEV int own_odeg(int) const { return 0; }
+  IG_DEV bool operator()(int v, int dv, int, int u) const { return outranks(nindex[u + 1] - nindex[u], u, dv, v); }
 };
 
 // In-list of v: the neighbours u that v outranks (u -> v) whose out-list is
 // shorter than v's.
 struct InSel {
-  const int* oindex;
-  IG_DEV bool operator()(const int* __restrict__ nindex, int v, int dv, int u) const
+  const TcVert* __restrict__ vert;
+  IG_DEV int own_odeg(int v) const { return vert[v].odeg; }
+  IG_DEV bool operator()(int v, int dv, int odv, int u) const
   {
-    if (outranks(deg(nindex, u), u, dv, v)) return false;
-    return oindex[u + 1] - oindex[u] < oindex[v + 1] - oindex[v];
+    const TcVert k = vert[u];
+    return !outranks(k.deg, u, dv, v) && k.odeg < odv;
   }
 };
 
-// Selection over short lists (< kMidDeg), one lane per vertex in id order.
-// Count pass (!kFill): count[v] = entries kept (0 for long lists, which the
-// heavy pass adds); with kBin it also bins the long lists into `heavy` and
-// flags unsorted lists and self loops.  Fill pass: list[index[v] + k].
-template <bool kFill, bool kBin, typename Sel>
+// Where a finished selection is recorded.  Lists live at their vertex's own
+// CSR position (list[nindex[v] + k]), so no count pass or scan is needed.
+struct OutRec {
+  TcVert* vert;
+  int2* orng;  // (start, |N+(v)|)
+  IG_DEV void operator()(int v, int dv, int beg, int kept) const
+  {
+    vert[v] = TcVert{dv, kept};
+    orng[v] = make_int2(beg, kept);
+  }
+};
+struct InRec {
+  int* ilen;  // |N-(v)|
+  IG_DEV void operator()(int v, int, int, int kept) const { ilen[v] = kept; }
+};
+
+// Selection over short lists (< kMidDeg): a wave takes 64 consecutive
+// vertices and walks their CSR entries 64 per step with the in-wave balancer;
+// kept entries are compacted per list in entry order (ballot ranks within
+// the owner's lanes plus the owner's running count), so selected lists stay
+// sorted.  The out pass (kOut) also bins the long lists into the mid / huge
+// queues and flags unsorted lists and self loops.
+template <bool kOut, typename Sel, typename Rec>
 __global__ __launch_bounds__(kTcBlock) void tc_select(int nodes, const int* __restrict__ nindex,
-                                                      const int* __restrict__ nlist, Sel sel, int* count,
-                                                      const int* __restrict__ index, int* list, FrontierQueues heavy,
-                                                      TcControl* ctl)
+                                                      const int* __restrict__ nlist, Sel sel, Rec rec, int* list,
+                                                      FrontierQueues heavy, TcControl* ctl)
 {
   __shared__ FrontierEmitter s_emit;
+  const int lane = lane_id();
   const int wpb = kTcBlock / kWave;
   const int nchunks = (nodes + kWave - 1) / kWave;
-  if (kBin) fe_reset(s_emit);
+  if (kOut) fe_reset(s_emit);
+  bool bad = false;
   for (int c = blockIdx.x * wpb + threadIdx.x / kWave; c < nchunks; c += gridDim.x * wpb) {
-    const int v = c * kWave + lane_id();
+    const int v = c * kWave + lane;
     const bool valid = v < nodes;
     const int beg = valid ? nindex[v] : 0;
-    const int end = valid ? nindex[v + 1] : 0;
-    const int dv = end - beg;
-    if (valid && dv < kMidDeg) {
-      int k = 0;
-      int bad = 0;
-      const int base = kFill ? index[v] : 0;
-      for (int e = beg; e < end; e++) {
-        const int u = nlist[e];
-        if (kBin) bad |= (u == v) | (e > beg && nlist[e - 1] >= u);
-        if (sel(nindex, v, dv, u)) {
-          if (kFill) list[base + k] = u;
-          k++;
-        }
+    const int dv = valid ? nindex[v + 1] - beg : 0;
+    const bool long_list = dv >= kMidDeg;
+    if (kOut) frontier_emit_block(s_emit, long_list, v, dv, heavy, &ctl->heavy);
+    const bool mine = valid && !long_list;
+    const int odv = mine ? sel.own_odeg(v) : 0;
+    const int slen = mine ? dv : 0;
+    const int incl = wave_inclusive_scan(slen);
+    const int excl = incl - slen;
+    const int total = wave_bcast(incl, kWave - 1);
+    int kept = 0;  // as an owner lane: entries of its list kept so far
+    for (int base = 0; base < total; base += kWave) {
+      const int k = base + lane;
+      const int o = min(wave_owner(incl, k), kWave - 1);
+      const int obeg = wave_shfl(beg, o);
+      const int oexcl = wave_shfl(excl, o);
+      const int odeg = wave_shfl(dv, o);
+      const int oodv = wave_shfl(odv, o);
+      const int okept = wave_shfl(kept, o);
+      const int ov = c * kWave + o;
+      bool keep = false;
+      int u = 0;
+      if (k < total) {
+        const int e = obeg + (k - oexcl);
+        u = nlist[e];
+        if (kOut) bad |= (u == ov) || (e > obeg && nlist[e - 1] >= u);
+        keep = sel(ov, odeg, oodv, u);
       }
-      if (!kFill) count[v] = k;
-      if (kBin && bad) st_relaxed(&ctl->error, 1);
+      const unsigned long long m = __ballot(keep);
+      // rank among the kept entries of the same owner in this step
+      const int seg0 = min(max(oexcl - base, 0), kWave - 1);
+      const unsigned long long mine_below = m & ((1ull << lane) - 1ull) & ~((1ull << seg0) - 1ull);
+      if (keep) list[obeg + okept + __popcll(mine_below)] = u;
+      // each owner adds its lanes' kept entries of this step
+      const int s0 = min(max(excl - base, 0), kWave), s1 = min(max(incl - base, 0), kWave);
+      const int w = s1 - s0;
+      const unsigned long long seg = w <= 0 ? 0ull : (w >= kWave ? ~0ull : ((1ull << w) - 1ull) << s0);
+      kept += __popcll(m & seg);
     }
-    if (!kFill) {
-      const bool long_list = valid && dv >= kMidDeg;
-      if (long_list) count[v] = 0;
-      if (kBin) frontier_emit_block(s_emit, long_list, v, dv, heavy, &ctl->heavy);
-    }
+    if (mine) rec(v, dv, beg, kept);
   }
-  if (kBin) fe_flush(s_emit, heavy, &ctl->heavy);
+  if (kOut) {
+    fe_flush(s_emit, heavy, &ctl->heavy);
+    if (__ballot(bad) != 0 && lane == 0) st_relaxed(&ctl->error, 1);
+  }
 }
 
-// Selection over the binned long lists: huge ones in kHugeChunk-entry items
-// (one workgroup each, per-item counts in hcnt), mid ones one wave each;
-// order is kept with ballot prefixes so selected lists stay sorted.
-template <bool kFill, bool kCheck, typename Sel>
+// Selection over the binned long lists.  Mid lists: one wave each, in order,
+// written at the list's own position and recorded.  Huge lists: one
+// workgroup per kHugeChunk-entry item, compacted in order at the item's own
+// position with the item's count in hcnt; tc_pack_huge then closes the gaps.
+template <bool kOut, typename Sel, typename Rec>
 __global__ __launch_bounds__(kTcBlock) void tc_select_heavy(const int* __restrict__ nindex,
-                                                            const int* __restrict__ nlist, Sel sel, int* count,
-                                                            const int* __restrict__ index, int* list,
-                                                            FrontierQueues heavy, int* hcnt, TcControl* ctl)
+                                                            const int* __restrict__ nlist, Sel sel, Rec rec,
+                                                            int* list, FrontierQueues heavy, int* hcnt, TcControl* ctl)
 {
-  __shared__ int s_owner, s_off;
+  __shared__ int s_owner;
   __shared__ int s_wcnt[kTcBlock / kWave];
   const int lane = lane_id();
   const int wid = threadIdx.x / kWave;
   const unsigned long long hp = ctl->heavy.hpack;
   const unsigned nh = huge_count(hp), nch = huge_chunks(hp);
+  bool bad = false;
   for (unsigned item = blockIdx.x; item < nch; item += gridDim.x) {
     if (threadIdx.x == 0) s_owner = huge_owner(heavy.hoff, static_cast<int>(nh), item);
     __syncthreads();
@@ -164,24 +213,18 @@ __global__ __launch_bounds__(kTcBlock) void tc_select_heavy(const int* __restric
     const int v = heavy.hq[j];
     const int vb = nindex[v], ve = nindex[v + 1];
     const int dv = ve - vb;
+    const int odv = sel.own_odeg(v);
     const int beg = vb + static_cast<int>(item - heavy.hoff[j]) * kHugeChunk;
     const int end = min(ve, beg + kHugeChunk);
-    if (kFill && threadIdx.x == 0) {
-      int off = index[v];
-      for (unsigned k = heavy.hoff[j]; k < item; k++) off += hcnt[k];
-      s_off = off;
-    }
-    __syncthreads();
-    int run = kFill ? s_off : 0;
-    bool bad = false;
+    int run = 0;
     for (int base = beg; base < end; base += kTcBlock) {
       const int e = base + static_cast<int>(threadIdx.x);
       bool keep = false;
       int u = 0;
       if (e < end) {
         u = nlist[e];
-        if (kCheck) bad |= (u == v) || (e > vb && nlist[e - 1] >= u);
-        keep = sel(nindex, v, dv, u);
+        if (kOut) bad |= (u == v) || (e > vb && nlist[e - 1] >= u);
+        keep = sel(v, dv, odv, u);
       }
       const unsigned long long m = __ballot(keep);
       if (lane == 0) s_wcnt[wid] = __popcll(m);
@@ -191,39 +234,76 @@ __global__ __launch_bounds__(kTcBlock) void tc_select_heavy(const int* __restric
         before += w < wid ? s_wcnt[w] : 0;
         total += s_wcnt[w];
       }
-      if (kFill && keep) list[run + before + lanes_below(m)] = u;
+      if (keep) list[beg + run + before + lanes_below(m)] = u;
       run += total;
       __syncthreads();
     }
-    if (!kFill && threadIdx.x == 0) {
-      hcnt[item] = run;
-      atomic_add_relaxed(&count[v], run);
-    }
-    if (kCheck && bad) st_relaxed(&ctl->error, 1);
-    __syncthreads();
+    if (threadIdx.x == 0) hcnt[item] = run;
   }
   const unsigned nm = ctl->heavy.nm;
   for (unsigned i = blockIdx.x * (kTcBlock / kWave) + wid; i < nm; i += gridDim.x * (kTcBlock / kWave)) {
     const int v = heavy.mq[i];
     const int vb = nindex[v], ve = nindex[v + 1];
     const int dv = ve - vb;
-    int k = kFill ? index[v] : 0;
-    bool bad = false;
+    const int odv = sel.own_odeg(v);
+    int k = 0;
     for (int base = vb; base < ve; base += kWave) {
       const int e = base + lane;
       bool keep = false;
       int u = 0;
       if (e < ve) {
         u = nlist[e];
-        if (kCheck) bad |= (u == v) || (e > vb && nlist[e - 1] >= u);
-        keep = sel(nindex, v, dv, u);
+        if (kOut) bad |= (u == v) || (e > vb && nlist[e - 1] >= u);
+        keep = sel(v, dv, odv, u);
       }
       const unsigned long long m = __ballot(keep);
-      if (kFill && keep) list[k + lanes_below(m)] = u;
+      if (keep) list[vb + k + lanes_below(m)] = u;
       k += __popcll(m);
     }
-    if (!kFill && lane == 0) count[v] = k;
-    if (kCheck && __ballot(bad) != 0 && lane == 0) st_relaxed(&ctl->error, 1);
+    if (lane == 0) rec(v, dv, vb, k);
+  }
+  if (kOut && __ballot(bad) != 0 && lane == 0) st_relaxed(&ctl->error, 1);
+}
+
+// One workgroup per huge list: moves each item's kept entries down to follow
+// the previous items' (in item order, so the list stays sorted) and records
+// the list.  An item's entries are all read before any is written, and its
+// destination never reaches past its own start, so nothing unread is
+// overwritten.
+template <typename Rec>
+__global__ __launch_bounds__(kTcBlock) void tc_pack_huge(const int* __restrict__ nindex, int* list,
+                                                         FrontierQueues heavy, const int* __restrict__ hcnt, Rec rec,
+                                                         const TcControl* ctl)
+{
+  constexpr int kPer = kHugeChunk / kTcBlock;
+  const unsigned nh = huge_count(ctl->heavy.hpack);
+  for (unsigned j = blockIdx.x; j < nh; j += gridDim.x) {
+    const int v = heavy.hq[j];
+    const int vb = nindex[v], dv = nindex[v + 1] - vb;
+    const unsigned c0 = heavy.hoff[j];
+    const int nc = (dv + kHugeChunk - 1) / kHugeChunk;
+    int run = 0;
+    for (int c = 0; c < nc; c++) {
+      const int k = hcnt[c0 + c];
+      const int src = vb + c * kHugeChunk;
+      int buf[kPer];
+#pragma unroll
+      for (int q = 0; q < kPer; q++) {
+        const int i = q * kTcBlock + static_cast<int>(threadIdx.x);
+        buf[q] = i < k ? list[src + i] : 0;
+      }
+      __syncthreads();
+      if (run != c * kHugeChunk) {
+#pragma unroll
+        for (int q = 0; q < kPer; q++) {
+          const int i = q * kTcBlock + static_cast<int>(threadIdx.x);
+          if (i < k) list[vb + run + i] = buf[q];
+        }
+      }
+      __syncthreads();
+      run += k;
+    }
+    if (threadIdx.x == 0) rec(v, dv, vb, run);
   }
 }
 
@@ -289,7 +369,7 @@ struct TcTask {
 // entries (97% of the probes on R-MAT) are walked by the whole wave, one at a
 // time, kUnroll windows of 64 per step so that their loads overlap; shorter
 // ones share windows of 64 pairs through the in-wave balancer.
-IG_DEV unsigned long long count_pairs(const int* __restrict__ oindex, const int* __restrict__ olist,
+IG_DEV unsigned long long count_pairs(const int2* __restrict__ orng, const int* __restrict__ olist,
                                       const int* __restrict__ items, int n, int limit, int ub0, int ustep,
                                       const int* tab, const TcTable& t, const int* nx, int d)
 {
@@ -298,9 +378,9 @@ IG_DEV unsigned long long count_pairs(const int* __restrict__ oindex, const int*
   auto probe = [&](int w) { return tab != nullptr ? table_find(tab, t, w) : search(nx, d, w); };
   for (int ub = ub0; ub < n; ub += ustep) {
     const int i = ub + lane;
-    const int u = i < n ? items[i] : 0;
-    const int b = i < n ? oindex[u] : 0;
-    const int l = i < n ? oindex[u + 1] - b : 0;
+    const int2 r = i < n ? orng[items[i]] : make_int2(0, 0);
+    const int b = r.x;
+    const int l = r.y;
     const int len = l <= limit ? l : 0;
     unsigned long long todo = __ballot(len >= kLongList);
     while (todo != 0) {
@@ -334,16 +414,16 @@ IG_DEV unsigned long long count_pairs(const int* __restrict__ oindex, const int*
 }
 
 // Items [lo, hi) of the task's item sequence (N+(x), then N-(x)).
-IG_DEV unsigned long long task_pairs(const int* __restrict__ oindex, const int* __restrict__ olist,
+IG_DEV unsigned long long task_pairs(const int2* __restrict__ orng, const int* __restrict__ olist,
                                      const int* __restrict__ ilist, const TcTask& x, int lo, int hi, int ub0,
                                      int ustep, const int* tab, const TcTable& t)
 {
   const int* nx = olist + x.ob;
   unsigned long long cnt = 0;
   const int a1 = min(hi, x.d);
-  if (lo < a1) cnt += count_pairs(oindex, olist, nx + lo, a1 - lo, x.d, ub0, ustep, tab, t, nx, x.d);
+  if (lo < a1) cnt += count_pairs(orng, olist, nx + lo, a1 - lo, x.d, ub0, ustep, tab, t, nx, x.d);
   const int b0 = max(lo, x.d) - x.d, b1 = hi - x.d;
-  if (b0 < b1) cnt += count_pairs(oindex, olist, ilist + x.ib + b0, b1 - b0, x.d, ub0, ustep, tab, t, nx, x.d);
+  if (b0 < b1) cnt += count_pairs(orng, olist, ilist + x.ib + b0, b1 - b0, x.d, ub0, ustep, tab, t, nx, x.d);
   return cnt;
 }
 
@@ -361,8 +441,8 @@ IG_DEV void append_parts(int parts, int x, unsigned long long* queue, unsigned* 
         (static_cast<unsigned long long>(x) << 32) | static_cast<unsigned>(q);
 }
 
-__global__ __launch_bounds__(kTcBlock) void tc_count(int nodes, const int* __restrict__ oindex,
-                                                     const int* __restrict__ olist, const int* __restrict__ iindex,
+__global__ __launch_bounds__(kTcBlock) void tc_count(int nodes, const int2* __restrict__ orng,
+                                                     const int* __restrict__ olist, const int* __restrict__ ilen,
                                                      const int* __restrict__ ilist, unsigned long long* mid,
                                                      unsigned long long* big, TcControl* ctl)
 {
@@ -378,8 +458,9 @@ __global__ __launch_bounds__(kTcBlock) void tc_count(int nodes, const int* __res
     // 64 consecutive vertices per grab; heavy tasks go to the workgroup
     // passes in parts (|N+(x)| < 2: no triangle can be found through N+(x))
     const int vl = static_cast<int>(v0) + lane;
-    const int dl = vl < nodes ? oindex[vl + 1] - oindex[vl] : 0;
-    const int il = vl < nodes ? iindex[vl + 1] - iindex[vl] : 0;
+    const int2 rl = vl < nodes ? orng[vl] : make_int2(0, 0);
+    const int dl = rl.y;
+    const int il = vl < nodes ? ilen[vl] : 0;
     const bool here = dl >= 2 && dl <= kWaveList && (dl + il) * dl <= kWaveWork;
     const int parts = dl >= 2 && !here ? (dl + il + kPartItems - 1) / kPartItems : 0;
     append_parts(dl <= kMidList ? parts : 0, vl, mid, &ctl->nmid);
@@ -388,14 +469,14 @@ __global__ __launch_bounds__(kTcBlock) void tc_count(int nodes, const int* __res
     while (todo != 0) {
       const int j = __ffsll(static_cast<long long>(todo)) - 1;
       todo &= todo - 1;
-      const int v = static_cast<int>(v0) + j;
-      const TcTask x{oindex[v], wave_shfl(dl, j), iindex[v], wave_shfl(il, j)};
+      const int ob = wave_shfl(rl.x, j);  // in-lists share the out-lists' positions
+      const TcTask x{ob, wave_shfl(dl, j), ob, wave_shfl(il, j)};
       const TcTable t = table_for(x.d);
       for (int i = lane; i < t.size; i += kWave) tab[i] = -1;
       wave_lds_sync();
       for (int i = lane; i < x.d; i += kWave) table_insert(tab, t, olist[x.ob + i]);
       wave_lds_sync();
-      cnt += task_pairs(oindex, olist, ilist, x, 0, x.d + x.di, 0, kWave, tab, t);
+      cnt += task_pairs(orng, olist, ilist, x, 0, x.d + x.di, 0, kWave, tab, t);
       wave_lds_sync();
     }
   }
@@ -406,9 +487,9 @@ __global__ __launch_bounds__(kTcBlock) void tc_count(int nodes, const int* __res
 // One workgroup per part from `parts` (grabbed through *next); lists up to
 // lds_list entries (<= kSlots / 4) are hashed into LDS.
 template <int kBlock, int kSlots>
-__global__ __launch_bounds__(kBlock) void tc_count_group(const int* __restrict__ oindex,
+__global__ __launch_bounds__(kBlock) void tc_count_group(const int2* __restrict__ orng,
                                                          const int* __restrict__ olist,
-                                                         const int* __restrict__ iindex,
+                                                         const int* __restrict__ ilen,
                                                          const int* __restrict__ ilist,
                                                          const unsigned long long* __restrict__ parts,
                                                          const unsigned* nparts, unsigned* next, int lds_list,
@@ -428,8 +509,8 @@ __global__ __launch_bounds__(kBlock) void tc_count_group(const int* __restrict__
     const unsigned long long p = parts[k];
     const int v = static_cast<int>(p >> 32);
     const int lo = static_cast<int>(p & 0xffffffffull) * kPartItems;
-    const int ob = oindex[v], ib = iindex[v];
-    const TcTask x{ob, oindex[v + 1] - ob, ib, iindex[v + 1] - ib};
+    const int2 r = orng[v];
+    const TcTask x{r.x, r.y, r.x, ilen[v]};
     const bool hashed = x.d <= lds_list;
     const TcTable t = table_for(hashed ? x.d : 1);
     if (hashed) {
@@ -439,7 +520,7 @@ __global__ __launch_bounds__(kBlock) void tc_count_group(const int* __restrict__
       __syncthreads();
     }
     const int hi = min(lo + kPartItems, x.d + x.di);
-    cnt += task_pairs(oindex, olist, ilist, x, lo, hi, wid * kWave, kWaves * kWave, hashed ? s_tab : nullptr, t);
+    cnt += task_pairs(orng, olist, ilist, x, lo, hi, wid * kWave, kWaves * kWave, hashed ? s_tab : nullptr, t);
     __syncthreads();
   }
   cnt = wave_sum(cnt);
@@ -471,55 +552,51 @@ size_t hcnt_words(int edges)
 size_t tc_workspace_bytes(int nodes, int edges)
 {
   const size_t v = static_cast<size_t>(nodes);
-  return Carver::round(sizeof(TcControl)) + 7 * Carver::round((v + 1) * 4) + 2 * Carver::round(list_words(edges) * 4) +
-         2 * Carver::round(part_words(nodes, edges) * 8) +
-         Carver::round(hcnt_words(edges) * 4) + Carver::round(scan_temp_bytes(nodes + 1)) + 4096;
+  return Carver::round(sizeof(TcControl)) + 2 * Carver::round(v * sizeof(int2)) + Carver::round(v * 4) +
+         3 * Carver::round((v + 1) * 4) + 2 * Carver::round(list_words(edges) * 4) +
+         2 * Carver::round(part_words(nodes, edges) * 8) + Carver::round(hcnt_words(edges) * 4) + 4096;
 }
 
 int tc_run(indigo_ctx* ctx, int nodes, int edges, const int* nindex, const int* nlist, int* launches)
 {
   Carver cv(ctx->ws);
-  const size_t vp = static_cast<size_t>(nodes) + 1;
+  const size_t vn = static_cast<size_t>(nodes), vp = vn + 1;
   TcControl* ctl = cv.take<TcControl>(1);
-  int* ocount = cv.take<int>(vp);
-  int* oindex = cv.take<int>(vp);
-  int* icount = cv.take<int>(vp);
-  int* iindex = cv.take<int>(vp);
+  TcVert* vert = cv.take<TcVert>(vn);
+  int2* orng = cv.take<int2>(vn);
+  int* ilen = cv.take<int>(vn);
   FrontierQueues heavy{nullptr, cv.take<int>(vp), cv.take<int>(vp), cv.take<unsigned>(vp)};
   unsigned long long* mid = cv.take<unsigned long long>(part_words(nodes, edges));
   unsigned long long* big = cv.take<unsigned long long>(part_words(nodes, edges));
   int* olist = cv.take<int>(list_words(edges));
   int* ilist = cv.take<int>(list_words(edges));
   int* hcnt = cv.take<int>(hcnt_words(edges));
-  int* temp = cv.take<int>(scan_temp_bytes(nodes + 1) / 4 + 1);
   IG_TRY(zero_async(ctx, ctl, sizeof(TcControl)));
   hipStream_t s = ctx->stream;
   const int gw = grid_for((nodes + kWave - 1) / kWave, kTcBlock / kWave, 1 << 16);
   const int gh = ctx->compute_units * 8;
-  const OutSel out{};
-  tc_select<false, true><<<gw, kTcBlock, 0, s>>>(nodes, nindex, nlist, out, ocount, oindex, olist, heavy, ctl);
-  tc_select_heavy<false, true><<<gh, kTcBlock, 0, s>>>(nindex, nlist, out, ocount, oindex, olist, heavy, hcnt, ctl);
-  IG_TRY(exclusive_scan(s, ocount, oindex, nodes, temp));
-  tc_select<true, false><<<gw, kTcBlock, 0, s>>>(nodes, nindex, nlist, out, ocount, oindex, olist, heavy, ctl);
-  tc_select_heavy<true, false><<<gh, kTcBlock, 0, s>>>(nindex, nlist, out, ocount, oindex, olist, heavy, hcnt, ctl);
-  const InSel in{oindex};
-  tc_select<false, false><<<gw, kTcBlock, 0, s>>>(nodes, nindex, nlist, in, icount, iindex, ilist, heavy, ctl);
-  tc_select_heavy<false, false><<<gh, kTcBlock, 0, s>>>(nindex, nlist, in, icount, iindex, ilist, heavy, hcnt, ctl);
-  IG_TRY(exclusive_scan(s, icount, iindex, nodes, temp));
-  tc_select<true, false><<<gw, kTcBlock, 0, s>>>(nodes, nindex, nlist, in, icount, iindex, ilist, heavy, ctl);
-  tc_select_heavy<true, false><<<gh, kTcBlock, 0, s>>>(nindex, nlist, in, icount, iindex, ilist, heavy, hcnt, ctl);
-  tc_count<<<ctx->compute_units * 8, kTcBlock, 0, s>>>(nodes, oindex, olist, iindex, ilist, mid, big, ctl);
+  const OutSel out{nindex};
+  const OutRec orec{vert, orng};
+  tc_select<true><<<gw, kTcBlock, 0, s>>>(nodes, nindex, nlist, out, orec, olist, heavy, ctl);
+  tc_select_heavy<true><<<gh, kTcBlock, 0, s>>>(nindex, nlist, out, orec, olist, heavy, hcnt, ctl);
+  tc_pack_huge<<<gh, kTcBlock, 0, s>>>(nindex, olist, heavy, hcnt, orec, ctl);
+  const InSel in{vert};
+  const InRec irec{ilen};
+  tc_select<false><<<gw, kTcBlock, 0, s>>>(nodes, nindex, nlist, in, irec, ilist, heavy, ctl);
+  tc_select_heavy<false><<<gh, kTcBlock, 0, s>>>(nindex, nlist, in, irec, ilist, heavy, hcnt, ctl);
+  tc_pack_huge<<<gh, kTcBlock, 0, s>>>(nindex, ilist, heavy, hcnt, irec, ctl);
+  tc_count<<<ctx->compute_units * 8, kTcBlock, 0, s>>>(nodes, orng, olist, ilen, ilist, mid, big, ctl);
   const int hint = ctx->hints[INDIGO_HINT_TC_LDS_LIST];
   const int mid_list = hint > 0 && hint < kMidSlots / 4 ? hint : kMidSlots / 4;
   const int big_list = hint > 0 && hint < kBigSlots / 4 ? hint : kBigSlots / 4;
   IG_TRY(kernel_begin(ctx, "tc_count_group<512, 8192>"));
   tc_count_group<kMidBlock, kMidSlots><<<ctx->compute_units * 4, kMidBlock, 0, s>>>(
-      oindex, olist, iindex, ilist, mid, &ctl->nmid, &ctl->next_mid, mid_list, ctl);
+      orng, olist, ilen, ilist, mid, &ctl->nmid, &ctl->next_mid, mid_list, ctl);
   IG_TRY(kernel_end(ctx));
   tc_count_group<kBigBlock, kBigSlots><<<ctx->compute_units, kBigBlock, 0, s>>>(
-      oindex, olist, iindex, ilist, big, &ctl->nbig, &ctl->next_big, big_list, ctl);
+      orng, olist, ilen, ilist, big, &ctl->nbig, &ctl->next_big, big_list, ctl);
   IG_HIP_TRY(hipGetLastError());
-  if (launches) *launches = 18;
+  if (launches) *launches = 10;
   return INDIGO_OK;
 }
 
