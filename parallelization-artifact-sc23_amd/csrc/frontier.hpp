@@ -135,14 +135,21 @@ IG_DEV void frontier_emit(bool pred, int u, int deg, const FrontierQueues& fq, F
 constexpr int kEmitSmallCap = 4096;
 constexpr int kEmitMidCap = 512;
 
-struct FrontierEmitter {
-  BlockQueue<kEmitSmallCap> small;
-  BlockQueue<kEmitMidCap> mid;
+template <int kSmallCap, int kMidCap>
+struct FrontierEmitterT {
+  BlockQueue<kSmallCap> small;
+  BlockQueue<kMidCap> mid;
   BlockSum m;
 };
+using FrontierEmitter = FrontierEmitterT<kEmitSmallCap, kEmitMidCap>;
+// For collect_heavy over a whole graph with a low threshold: no small queue,
+// room for a workgroup's whole share of mid lists (an append beyond the
+// buffer costs a same-address global atomic, about 10 ns each, serialised).
+using HeavyCollector = FrontierEmitterT<64, 2048>;
 
 // Workgroup-collective.
-IG_DEV void fe_reset(FrontierEmitter& E)
+template <int S, int M>
+IG_DEV void fe_reset(FrontierEmitterT<S, M>& E)
 {
   if (threadIdx.x == 0) {
     E.small.count = 0;
@@ -153,8 +160,8 @@ IG_DEV void fe_reset(FrontierEmitter& E)
 }
 
 // Wave-collective.
-template <int kMid = kMidDeg>
-IG_DEV void frontier_emit_block(FrontierEmitter& E, bool pred, int u, int deg, const FrontierQueues& fq,
+template <int kMid = kMidDeg, int S, int M>
+IG_DEV void frontier_emit_block(FrontierEmitterT<S, M>& E, bool pred, int u, int deg, const FrontierQueues& fq,
                                 FrontierCounters* c)
 {
   const bool huge = pred && deg >= kHugeDeg;
@@ -182,7 +189,8 @@ IG_DEV void frontier_emit_block(FrontierEmitter& E, bool pred, int u, int deg, c
 }
 
 // Workgroup-collective.
-IG_DEV void fe_flush(FrontierEmitter& E, const FrontierQueues& fq, FrontierCounters* c)
+template <int S, int M>
+IG_DEV void fe_flush(FrontierEmitterT<S, M>& E, const FrontierQueues& fq, FrontierCounters* c)
 {
   bq_flush(E.small, fq.q, &c->n);
   bq_flush(E.mid, fq.mq, &c->nm);
@@ -205,9 +213,9 @@ IG_DEV int huge_owner(const unsigned* hoff, int nh, unsigned item)
 // Workgroup-collective grid-stride pass over vertices 0..nodes-1 (64 per
 // wave) that appends every vertex with a list of >= kMid entries to the mid /
 // huge queues, aggregated per workgroup.
-template <int kMid = kMidDeg>
-IG_DEV void collect_heavy(FrontierEmitter& E, int nodes, const int* __restrict__ nindex, const FrontierQueues& fq,
-                          FrontierCounters* c)
+template <int kMid = kMidDeg, int S, int M>
+IG_DEV void collect_heavy(FrontierEmitterT<S, M>& E, int nodes, const int* __restrict__ nindex,
+                          const FrontierQueues& fq, FrontierCounters* c)
 {
   const int lane = lane_id();
   const int wpb = blockDim.x / kWave;

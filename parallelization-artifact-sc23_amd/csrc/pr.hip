@@ -35,6 +35,7 @@
 #include "device_common.hpp"
 #include "csum.hpp"
 #include "frontier.hpp"
+#include "trace.hpp"
 
 namespace indigo {
 
@@ -45,6 +46,7 @@ constexpr int kPrWaveDeg = 16;
 constexpr float kDamp = 0.85f;
 
 constexpr int kPrMaxGrid = 4096;
+constexpr int kPrUnroll = 4;  // gathers in flight per lane
 
 struct PrControl {
   GridBarrier bar;
@@ -54,6 +56,10 @@ struct PrControl {
   double last_diff;
   int grid;
   int pad[25];
+  // workgroup 0's timeline: entries 0-2 the end of the set-up (scores,
+  // heavy lists, first barrier); then per sweep the start and the ends of its
+  // huge, mid and short phases (entry 3 + 4 (it - 1) + phase, mode = phase)
+  LevelTrace trace;
 };
 
 struct PrArgs {
@@ -79,6 +85,28 @@ struct PrArgs {
 IG_DEV int deg_of(const PrArgs& a, int v)
 {
   return a.degree ? a.degree[v] : a.nindex[v + 1] - a.nindex[v];
+}
+
+// acc += cin[nlist[e]] for e = beg + first, beg + first + step, ... < end,
+// in that order, with kPrUnroll independent loads in flight per lane: the
+// gathers are latency-bound chains (index, then contribution), so a plain
+// loop would wait out one round trip per window.
+template <int kStep>
+IG_DEV void gather(CSum& acc, const float* __restrict__ cin, const int* __restrict__ nlist, int beg, int end,
+                   int first)
+{
+  int e = beg + first;
+  for (; e + (kPrUnroll - 1) * kStep < end; e += kPrUnroll * kStep) {
+    int u[kPrUnroll];
+#pragma unroll
+    for (int q = 0; q < kPrUnroll; q++) u[q] = nlist[e + q * kStep];
+    float c[kPrUnroll];
+#pragma unroll
+    for (int q = 0; q < kPrUnroll; q++) c[q] = cin[u[q]];
+#pragma unroll
+    for (int q = 0; q < kPrUnroll; q++) acc.add(c[q]);
+  }
+  for (; e < end; e += kStep) acc.add(cin[nlist[e]]);
 }
 
 // New score of v from its gathered sum; writes score and next contrib and
@@ -128,7 +156,7 @@ __global__ __launch_bounds__(kPrBlock) void pr_kernel(PrArgs a)
   __shared__ double s_red[kPrBlock / kWave];
   __shared__ float s_ps[kPrBlock / kWave], s_pc[kPrBlock / kWave];
   __shared__ int s_owner;
-  __shared__ FrontierEmitter s_emit;
+  __shared__ HeavyCollector s_emit;
   PrControl* ctl = a.ctl;
   if (!residency_barrier(&ctl->bar, a.start_ticks)) return;
   const long long deadline = now_ticks() + a.budget;
@@ -139,17 +167,21 @@ __global__ __launch_bounds__(kPrBlock) void pr_kernel(PrArgs a)
   const float inv_n = 1.0f / static_cast<float>(a.nodes);
   const float base = (1.0f - kDamp) / static_cast<float>(a.nodes);
 
+  if (tid == 0) ctl->trace.start = now_ticks();
   for (int v = tid; v < a.nodes; v += stride) {
     const float s = a.init_uniform ? inv_n : a.scores[v];
     a.scores[v] = s;
     a.contrib[0][v] = s / static_cast<float>(deg_of(a, v));
   }
+  trace_level(&ctl->trace, 0, 0u, 0ull, 0);
   collect_heavy<kPrWaveDeg>(s_emit, a.nodes, a.nindex, a.heavy, &ctl->heavy);
+  trace_level(&ctl->trace, 1, 0u, 0ull, 1);
   if (!grid_barrier(&ctl->bar, deadline)) {
     if (tid == 0) ctl->status = 1;
     if (threadIdx.x == 0) a.exit_it[blockIdx.x] = -1000000;
     return;
   }
+  trace_level(&ctl->trace, 2, 0u, 0ull, 2);
   const unsigned nm = ld_relaxed(&ctl->heavy.nm);
   const unsigned long long hp = ld_relaxed(&ctl->heavy.hpack);
   const unsigned nh = huge_count(hp), nch = huge_chunks(hp);
@@ -161,6 +193,8 @@ __global__ __launch_bounds__(kPrBlock) void pr_kernel(PrArgs a)
     float* cout = a.contrib[it & 1];
     double* slots = a.partial + (it & 1) * (2 * kPrMaxGrid);
     double d = 0.0;
+    const int tl = 3 + 4 * (it - 1);
+    trace_level(&ctl->trace, tl, static_cast<unsigned>(it), 0ull, 0);
 
     // phase A: huge items -> partials
     for (unsigned item = blockIdx.x; item < nch; item += gridDim.x) {
@@ -171,30 +205,33 @@ __global__ __launch_bounds__(kPrBlock) void pr_kernel(PrArgs a)
       const int beg = a.nindex[v] + static_cast<int>(item - a.heavy.hoff[j]) * kHugeChunk;
       const int end = min(a.nindex[v + 1], beg + kHugeChunk);
       CSum acc;
-      for (int e = beg + static_cast<int>(threadIdx.x); e < end; e += kPrBlock) acc.add(cin[a.nlist[e]]);
+      gather<kPrBlock>(acc, cin, a.nlist, beg, end, static_cast<int>(threadIdx.x));
       const CSum blk = block_merge(acc, s_ps, s_pc);
       if (threadIdx.x == 0) a.hpart[item] = make_float2(blk.s, blk.c);
     }
+    trace_level(&ctl->trace, tl + 1, static_cast<unsigned>(it), 0ull, 1);
     // phase A: mid lists, one wave each
     for (unsigned i = blockIdx.x * wpb + threadIdx.x / kWave; i < nm; i += gridDim.x * wpb) {
       const int v = a.heavy.mq[i];
       const int beg = a.nindex[v], end = a.nindex[v + 1];
       CSum acc;
-      for (int e = beg + lane; e < end; e += kWave) acc.add(cin[a.nlist[e]]);
+      gather<kWave>(acc, cin, a.nlist, beg, end, lane);
       acc = wave_merge(acc);
       if (lane == 0) d += finish_vertex(a, v, acc.value(), base, cout);
     }
+    trace_level(&ctl->trace, tl + 2, static_cast<unsigned>(it), 0ull, 2);
     // phase A: short lists, one lane each, vertex order
     for (int v = tid; v < a.nodes; v += stride) {
       const int beg = a.nindex[v], end = a.nindex[v + 1];
       if (end - beg >= kPrWaveDeg) continue;
       CSum acc;
-      for (int e = beg; e < end; e++) acc.add(cin[a.nlist[e]]);
+      gather<1>(acc, cin, a.nlist, beg, end, 0);
       d += finish_vertex(a, v, acc.value(), base, cout);
     }
     d = block_sum_f64(d, s_red);
     if (threadIdx.x == 0) slots[2 * blockIdx.x] = d;
     d = 0.0;
+    trace_level(&ctl->trace, tl + 3, static_cast<unsigned>(it), 0ull, 3);
 
     if (nh > 0) {
       if (!grid_barrier(&ctl->bar, deadline)) {
@@ -339,6 +376,7 @@ int pr_collect(indigo_ctx* ctx, int* iterations, double* last_diff)
     return set_error(INDIGO_ERR_TIMEOUT, "pagerank: persistent kernel stopped (status %d); workgroups:%s",
                      h->status, msg.c_str());
   }
+  store_trace(ctx, h->trace);
   if (iterations) *iterations = h->iterations;
   if (last_diff) *last_diff = h->last_diff;
   return INDIGO_OK;
