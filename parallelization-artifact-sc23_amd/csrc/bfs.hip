@@ -262,7 +262,13 @@ IG_DEV int small_phase(const BfsArgs& a, int L, unsigned n, BfsControl* ctl, lon
 
 __global__ __launch_bounds__(kBfsBlock) void bfs_kernel(BfsArgs a)
 {
-  __shared__ FrontierEmitter s_emit;
+  // the set-up's heavy-list collection needs a larger mid buffer than the
+  // level emitter (no spills to the global counter); they share the LDS
+  __shared__ union {
+    FrontierEmitter emit;
+    HeavyCollector collect;
+  } s_lds;
+  FrontierEmitter& s_emit = s_lds.emit;
   BfsControl* ctl = a.ctl;
   if (!residency_barrier(&ctl->bar, a.start_ticks)) return;
   const long long deadline = now_ticks() + a.budget;
@@ -270,7 +276,7 @@ __global__ __launch_bounds__(kBfsBlock) void bfs_kernel(BfsArgs a)
   const int stride = gridDim.x * blockDim.x;
   if (tid == 0) ctl->trace.start = now_ticks();
   for (int i = tid; i < a.nodes; i += stride) st_relaxed(&a.dist[i], (i == a.src) ? 0 : kInf);
-  if (a.allow_bu) collect_heavy(s_emit, a.nodes, a.nindex, a.heavy, &ctl->heavy);
+  if (a.allow_bu) collect_heavy(s_lds.collect, a.nodes, a.nindex, a.heavy, &ctl->heavy);
   if (blockIdx.x == 0 && threadIdx.x < kWave) {
     const int deg = degree_of(a.nindex, a.src);
     frontier_emit(threadIdx.x == 0 && deg > 0, a.src, deg, a.fq[0], &ctl->cnt[0]);
