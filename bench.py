@@ -140,6 +140,10 @@ def kernel_bytes(alg: str, g, pr_iters: int, tc_model: dict) -> float:
     return tc_model["bytes"]
 
 
+MAIN_KERNELS = {"bfs": "bfs_kernel", "sssp": "sssp_kernel", "cc": "cc_link", "mis": "mis_kernel", "pr": "pr_kernel",
+                "tc": "tc_count_group<512, 8192>"}
+
+
 def read_traffic(path: str, workload: str, kernel: str) -> dict | None:
     """HBM bytes per launch of `kernel` measured with rocprofv3 --pmc
     (FETCH_SIZE, WRITE_SIZE) on this workload, scaled by the same counters on
@@ -149,9 +153,9 @@ def read_traffic(path: str, workload: str, kernel: str) -> dict | None:
             rec = json.load(f)
     except (OSError, ValueError):
         return None
-    if rec.get("workload") != workload or rec.get("kernel") != kernel:
+    if rec.get("workload") != workload or kernel not in rec.get("kernels", {}):
         return None
-    return rec
+    return dict(rec["kernels"][kernel], source=rec.get("source"))
 
 
 def _pmc_values(path: str, counter: str, match) -> list[float]:
@@ -162,30 +166,38 @@ def _pmc_values(path: str, counter: str, match) -> list[float]:
                 if r["Counter_Name"] == counter and match(r["Kernel_Name"])]
 
 
-def record_traffic(paths: list[str], workload: str, kernel: str, out: str) -> int:
-    """Writes the traffic record from four rocprofv3 --pmc CSVs: FETCH_SIZE and
-    WRITE_SIZE (KiB per dispatch) of `kernel` on the workload, then of the
-    1 GiB copy of --copy-calibration.  Each counter is scaled by what it reads
-    for the copy, which moves exactly 1 GiB each way."""
+def record_traffic(paths: list[str], workload: str, out: str) -> int:
+    """Writes the traffic record of every algorithm's main kernel from four
+    rocprofv3 --pmc CSVs: FETCH_SIZE and WRITE_SIZE (KiB per dispatch) on the
+    workload, then of the 1 GiB copy of --copy-calibration.  Each counter is
+    scaled by what it reads for the copy, which moves exactly 1 GiB each
+    way."""
     kf, kw, cf, cw = paths
-    short = kernel.split("<")[0]
-    is_kernel = lambda name: short in name and (kernel == short or kernel.split("<")[1].rstrip(">") in name)
     is_copy = lambda name: "copy" in name.lower() or "elementwise" in name.lower()
-    fetch, write = _pmc_values(kf, "FETCH_SIZE", is_kernel), _pmc_values(kw, "WRITE_SIZE", is_kernel)
     cfetch = [x for x in _pmc_values(cf, "FETCH_SIZE", is_copy) if x > 1e5]
     cwrite = [x for x in _pmc_values(cw, "WRITE_SIZE", is_copy) if x > 1e5]
-    if not (fetch and write and cfetch and cwrite):
-        print("record-traffic: missing dispatches in the CSVs", file=sys.stderr)
+    if not (cfetch and cwrite):
+        print("record-traffic: no copy dispatches in the calibration CSVs", file=sys.stderr)
         return 1
     gib_kib = float(1 << 20)
     ffac, wfac = gib_kib / statistics.median(cfetch), gib_kib / statistics.median(cwrite)
-    rb = 1024.0 * statistics.mean(fetch) * ffac
-    wb = 1024.0 * statistics.mean(write) * wfac
-    rec = {"workload": workload, "kernel": kernel, "bytes_per_launch": rb + wb, "read_bytes": rb, "write_bytes": wb,
-           "fetch_kib_per_dispatch": statistics.mean(fetch), "write_kib_per_dispatch": statistics.mean(write),
-           "copy_scale": {"fetch": ffac, "write": wfac}, "dispatches": [len(fetch), len(write)],
-           "source": "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, python3 bench.py --algs tc --steps 2 --warmup 0 "
-                     "--no-cpu-baseline, scaled by a 1 GiB copy (bench.py --copy-calibration)"}
+    kernels = {}
+    for alg, kernel in MAIN_KERNELS.items():
+        match = lambda name, k=kernel: f"{k}(" in name
+        fetch, write = _pmc_values(kf, "FETCH_SIZE", match), _pmc_values(kw, "WRITE_SIZE", match)
+        if not (fetch and write):
+            continue
+        rb = 1024.0 * statistics.mean(fetch) * ffac
+        wb = 1024.0 * statistics.mean(write) * wfac
+        kernels[kernel] = {"algorithm": alg, "bytes_per_launch": rb + wb, "read_bytes": rb, "write_bytes": wb,
+                           "fetch_kib_per_dispatch": statistics.mean(fetch),
+                           "write_kib_per_dispatch": statistics.mean(write), "dispatches": [len(fetch), len(write)]}
+    if not kernels:
+        print("record-traffic: no main-kernel dispatches in the CSVs", file=sys.stderr)
+        return 1
+    rec = {"workload": workload, "copy_scale": {"fetch": ffac, "write": wfac}, "kernels": kernels,
+           "source": "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, python3 bench.py --steps 2 --warmup 0 "
+                     "--no-cpu-baseline --profile-only, scaled by a 1 GiB copy (bench.py --copy-calibration)"}
     with open(out, "w") as f:
         json.dump(rec, f, indent=1)
     print(json.dumps(rec))
@@ -504,11 +516,11 @@ def main() -> int:
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device-to-device copy 5 times (the PMC scale reference) and exit")
     ap.add_argument("--record-traffic", nargs=4, metavar=("KFETCH", "KWRITE", "CFETCH", "CWRITE"),
-                    help="write profiles/traffic_<workload>.json for the TC main kernel from four PMC CSVs")
+                    help="write profiles/traffic_<workload>.json for the main kernels from four PMC CSVs")
     args = ap.parse_args()
     if args.record_traffic:
         workload = f"rmat{args.scale} ef{args.edge_factor}"
-        return record_traffic(args.record_traffic, workload, "tc_count_group<512, 8192>",
+        return record_traffic(args.record_traffic, workload,
                               os.path.join(REPO, "profiles", f"traffic_{workload.replace(' ', '_')}.json"))
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: every kernel here runs on a single MI355X "
@@ -619,8 +631,11 @@ def main() -> int:
                    "call_frac": round(b_call / (med[a] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
     dom = max(ALGS, key=lambda a: med[a])
     workload = f"rmat{args.scale} ef{args.edge_factor}"
-    traffic = read_traffic(os.path.join(REPO, "profiles", f"traffic_{workload.replace(' ', '_')}.json"), workload,
-                           kname[dom])
+    traffic_path = os.path.join(REPO, "profiles", f"traffic_{workload.replace(' ', '_')}.json")
+    for a in ALGS:  # measured HBM bytes of each main kernel, where recorded
+        t = read_traffic(traffic_path, workload, kname[a])
+        roof[a]["traffic"] = t["bytes_per_launch"] if t else None
+    traffic = read_traffic(traffic_path, workload, kname[dom])
 
     cpu = None
     if not args.no_cpu_baseline:
