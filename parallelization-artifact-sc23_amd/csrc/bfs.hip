@@ -144,89 +144,124 @@ IG_DEV void bottom_up(const BfsArgs& a, int L, unsigned heavy_mid, unsigned heav
   }
 }
 
+// LDS of the one-workgroup phase (shares the kernel's LDS with the level
+// emitter, which is idle meanwhile).
+struct SmallLds {
+  int q[kSmallCap];      // frontier of the current level, refilled with the next one
+  int incl[kSmallCap];   // inclusive prefix of the current frontier's degrees
+  int shift[kSmallCap];  // nindex[v] - exclusive prefix: entry k of frontier slot j is nlist[shift[j] + k]
+  int wsum[kBfsBlock / kWave];
+  int cnt;
+  int total;
+  int abort_flag;
+};
+
 // Workgroup 0 advances levels alone while the frontier's degree sum fits the
-// LDS queue, paying workgroup barriers only.  Entered at level L with the
-// frontier in the small global queue (n entries, degree sum m <= kSmallCap);
+// LDS arrays, paying workgroup barriers only.  Entered at level L with the
+// frontier in the small global queue (n entries, degree sum <= kSmallCap);
 // on leaving (frontier empty, or its degree sum too large) it re-emits the
 // frontier of the level it reached into that level's global queues and
 // counters, cleared first.  The counters may only be touched once every other
 // workgroup has read level L's counters: each of them bumps small_ack
 // (release) after deciding on this phase, and workgroup 0 waits for all of
 // them (bounded) before the hand-back.  Returns the level reached.
-IG_DEV int small_phase(const BfsArgs& a, int L, unsigned n, BfsControl* ctl, long long deadline, bool* aborted)
+//
+// A level is three dependent rounds of global accesses: the frontier's
+// index pairs (all at once, one run of slots per thread), then after a
+// workgroup prefix sum the list entries spread evenly over all 512 threads,
+// then one CAS per entry (INT_MAX -> L + 1, so each vertex is claimed once).
+// Claimed vertices join the next frontier whatever their degree; those
+// without neighbours contribute no entries and are not handed back.
+IG_DEV int small_phase(const BfsArgs& a, int L, unsigned n, BfsControl* ctl, long long deadline, bool* aborted,
+                       SmallLds& S)
 {
-  __shared__ int s_q[2][kSmallCap];
-  __shared__ int s_cnt[2];
-  __shared__ int s_m[2];
-  __shared__ int s_abort;
+  constexpr int kPer = kSmallCap / kBfsBlock;  // frontier slots per thread
   const int lane = lane_id();
+  const int wid = threadIdx.x / kWave;
   const int* gin = a.fq[L & 1].q;
-  for (unsigned i = threadIdx.x; i < n; i += blockDim.x) s_q[0][i] = ld_relaxed(&gin[i]);
+  for (unsigned i = threadIdx.x; i < n; i += blockDim.x) S.q[i] = ld_relaxed(&gin[i]);
   if (threadIdx.x == 0) {
-    s_cnt[0] = static_cast<int>(n);
-    s_cnt[1] = 0;
-    s_m[0] = 0;
-    s_m[1] = 0;
-    s_abort = 0;
+    S.cnt = static_cast<int>(n);
+    S.abort_flag = 0;
   }
   __syncthreads();
-  int cur = 0;
   *aborted = false;
+  int cn = S.cnt;
   for (;;) {
-    const int cn = s_cnt[cur];
-    if (cn == 0 || s_m[cur] > kSmallCap) break;
-    if ((L & 63) == 0 && threadIdx.x == 0) s_abort = kernel_aborted(&ctl->bar, deadline) ? 1 : 0;
+    if (cn == 0) break;
+    if ((L & 63) == 0 && threadIdx.x == 0) S.abort_flag = kernel_aborted(&ctl->bar, deadline) ? 1 : 0;
+    // round 1: index pairs of this thread's run of frontier slots
+    const int i0 = min(cn, static_cast<int>(threadIdx.x) * kPer);
+    const int i1 = min(cn, i0 + kPer);
+    int beg[kPer], deg[kPer];
+    int mine = 0;
+#pragma unroll
+    for (int j = 0; j < kPer; j++) {
+      const bool ok = i0 + j < i1;
+      const int v = ok ? S.q[i0 + j] : 0;
+      beg[j] = ok ? a.nindex[v] : 0;
+      deg[j] = ok ? a.nindex[v + 1] - beg[j] : 0;
+      mine += deg[j];
+    }
+    const int winc = wave_inclusive_scan(mine);
+    if (lane == kWave - 1) S.wsum[wid] = winc;
     __syncthreads();
-    if (s_abort) {
+    if (S.abort_flag) {
       *aborted = true;
       return L;
     }
-    const int nxt = cur ^ 1;
+    int before = 0, total = 0;
+    for (int w = 0; w < kBfsBlock / kWave; w++) {
+      before += w < wid ? S.wsum[w] : 0;
+      total += S.wsum[w];
+    }
+    if (total > kSmallCap) break;  // S.q still holds level L's frontier
+    int run = before + winc - mine;
+#pragma unroll
+    for (int j = 0; j < kPer; j++) {
+      if (i0 + j < i1) {
+        S.incl[i0 + j] = run + deg[j];
+        S.shift[i0 + j] = beg[j] - run;
+        run += deg[j];
+      }
+    }
+    if (threadIdx.x == 0) S.cnt = 0;
+    __syncthreads();
+    // rounds 2 and 3: list entries, then claims; the next frontier refills S.q
     const int next = L + 1;
-    const int wpb = blockDim.x / kWave;
-    for (int c = threadIdx.x / kWave; c * kWave < cn; c += wpb) {
-      const int idx = c * kWave + lane;
-      const bool valid = idx < cn;
-      const int v = valid ? s_q[cur][idx] : 0;
-      const int beg = valid ? a.nindex[v] : 0;
-      const int deg = valid ? a.nindex[v + 1] - beg : 0;
-      const int incl = wave_inclusive_scan(deg);
-      const int total = wave_shfl(incl, kWave - 1);
-      for (int base = 0; base < total; base += kWave) {
-        const int k = base + lane;
-        const int o = min(wave_owner(incl, k), kWave - 1);
-        const int obeg = wave_shfl(beg, o);
-        const int oexcl = wave_shfl(incl - deg, o);
-        bool won = false;
-        int u = 0, udeg = 0;
-        if (k < total) {
-          u = a.nlist[obeg + (k - oexcl)];
-          if (claim(a.dist, u, next)) {
-            udeg = degree_of(a.nindex, u);
-            won = udeg > 0;
-          }
+#pragma unroll
+    for (int j = 0; j < kPer; j++) {
+      const int k = j * kBfsBlock + static_cast<int>(threadIdx.x);
+      const bool act = k < total;
+      int u = 0;
+      if (act) {
+        int lo = 0, hi = cn - 1;  // owner: the first slot with incl > k
+        while (lo < hi) {
+          const int mid = (lo + hi) >> 1;
+          if (S.incl[mid] > k) hi = mid;
+          else lo = mid + 1;
         }
-        const unsigned long long m = __ballot(won);
-        if (m != 0) {
-          const int leader = __ffsll(static_cast<long long>(m)) - 1;
-          const int dsum = wave_sum(won ? udeg : 0);
-          int b = 0;
-          if (lane == leader) {
-            b = atomicAdd(&s_cnt[nxt], __popcll(m));
-            atomicAdd(&s_m[nxt], dsum);
-          }
-          b = wave_shfl(b, leader);
-          // the previous level's degree sum bounds this count by kSmallCap
-          if (won) s_q[nxt][b + lanes_below(m)] = u;
-        }
+        u = a.nlist[S.shift[lo] + k];
+      }
+      beg[j] = u;  // reuse: the entry's neighbour
+      deg[j] = act ? 1 : 0;
+    }
+#pragma unroll
+    for (int j = 0; j < kPer; j++) deg[j] = deg[j] && cas_relaxed(&a.dist[beg[j]], kInf, next);
+#pragma unroll
+    for (int j = 0; j < kPer; j++) {
+      const unsigned long long m = __ballot(deg[j] != 0);
+      if (m != 0) {
+        const int leader = __ffsll(static_cast<long long>(m)) - 1;
+        int b = 0;
+        if (lane == leader) b = atomicAdd(&S.cnt, __popcll(m));
+        b = wave_shfl(b, leader);
+        // at most `total` <= kSmallCap claims in all
+        if (deg[j]) S.q[b + lanes_below(m)] = beg[j];
       }
     }
     __syncthreads();
-    if (threadIdx.x == 0) {
-      s_cnt[cur] = 0;
-      s_m[cur] = 0;
-    }
-    cur = nxt;
+    cn = S.cnt;
     L = next;
     __syncthreads();
   }
@@ -236,26 +271,25 @@ IG_DEV int small_phase(const BfsArgs& a, int L, unsigned n, BfsControl* ctl, lon
     const unsigned others = gridDim.x - 1u;
     while (__hip_atomic_load(&ctl->small_ack.v, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < others) {
       if (kernel_aborted(&ctl->bar, deadline)) {
-        s_abort = 1;
+        S.abort_flag = 1;
         break;
       }
       __builtin_amdgcn_s_sleep(1);
     }
     __hip_atomic_store(&ctl->small_ack.v, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    if (!s_abort)
+    if (!S.abort_flag)
       for (int k = 0; k < 3; k++) counters_clear(&ctl->cnt[k]);
   }
   __syncthreads();
-  if (s_abort) {
+  if (S.abort_flag) {
     *aborted = true;
     return L;
   }
-  const int cn = s_cnt[cur];
   for (int base = 0; base < cn; base += blockDim.x) {
     const int i = base + static_cast<int>(threadIdx.x);
-    const bool valid = i < cn;
-    const int v = valid ? s_q[cur][i] : 0;
-    frontier_emit(valid, v, valid ? degree_of(a.nindex, v) : 0, a.fq[L & 1], &ctl->cnt[L % 3]);
+    const int v = i < cn ? S.q[i] : 0;
+    const int d = i < cn ? degree_of(a.nindex, v) : 0;
+    frontier_emit(d > 0, v, d, a.fq[L & 1], &ctl->cnt[L % 3]);
   }
   return L;
 }
@@ -267,6 +301,7 @@ __global__ __launch_bounds__(kBfsBlock) void bfs_kernel(BfsArgs a)
   __shared__ union {
     FrontierEmitter emit;
     HeavyCollector collect;
+    SmallLds small;
   } s_lds;
   FrontierEmitter& s_emit = s_lds.emit;
   BfsControl* ctl = a.ctl;
@@ -315,7 +350,7 @@ __global__ __launch_bounds__(kBfsBlock) void bfs_kernel(BfsArgs a)
         __hip_atomic_fetch_add(&ctl->small_ack.v, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
       if (blockIdx.x == 0) {
         bool aborted = false;
-        const int L2 = small_phase(a, L, ic.n, ctl, deadline, &aborted);
+        const int L2 = small_phase(a, L, ic.n, ctl, deadline, &aborted, s_lds.small);
         if (threadIdx.x == 0) {
           st_relaxed(&ctl->level_base, L2);
           ctl->small_levels += L2 - L;
