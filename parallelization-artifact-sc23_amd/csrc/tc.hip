@@ -70,7 +70,6 @@ constexpr int kWaveWork = 1 << 15;  // probe bound of a wave-pass task
 constexpr int kPartItems = 1024;    // items per workgroup-pass part
 constexpr int kUnroll = 4;          // windows of 64 per step over a long list
 constexpr int kLongList = 32;       // lists walked by the whole wave
-constexpr int kTcGrab = 4;          // 64-vertex chunks per wave-pass grab
 
 struct TcControl {
   FrontierCounters heavy;  // lists of >= kMidDeg entries
@@ -452,39 +451,35 @@ __global__ __launch_bounds__(kTcBlock) void tc_count(int nodes, const int2* __re
   int* tab = s_tab[threadIdx.x / kWave];
   unsigned long long cnt = 0;
   for (;;) {
-    // kTcGrab x 64 consecutive vertices per grab (one same-address atomic
-    // each; those serialise at about 10 ns apiece)
-    unsigned g0 = 0;
-    if (lane == 0) g0 = atomic_add_relaxed(&ctl->next, static_cast<unsigned>(kTcGrab * kWave));
-    g0 = wave_shfl(g0, 0);
-    if (g0 >= static_cast<unsigned>(nodes)) break;
-    for (int sub = 0; sub < kTcGrab; sub++) {
-      const unsigned v0 = g0 + static_cast<unsigned>(sub * kWave);
-      if (v0 >= static_cast<unsigned>(nodes)) break;
-      // heavy tasks go to the workgroup passes in parts (|N+(x)| < 2: no
-      // triangle can be found through N+(x))
-      const int vl = static_cast<int>(v0) + lane;
-      const int2 rl = vl < nodes ? orng[vl] : make_int2(0, 0);
-      const int dl = rl.y;
-      const int il = vl < nodes ? ilen[vl] : 0;
-      const bool here = dl >= 2 && dl <= kWaveList && (dl + il) * dl <= kWaveWork;
-      const int parts = dl >= 2 && !here ? (dl + il + kPartItems - 1) / kPartItems : 0;
-      append_parts(dl <= kMidList ? parts : 0, vl, mid, &ctl->nmid);
-      append_parts(dl > kMidList ? parts : 0, vl, big, &ctl->nbig);
-      unsigned long long todo = __ballot(here);
-      while (todo != 0) {
-        const int j = __ffsll(static_cast<long long>(todo)) - 1;
-        todo &= todo - 1;
-        const int ob = wave_shfl(rl.x, j);  // in-lists share the out-lists' positions
-        const TcTask x{ob, wave_shfl(dl, j), ob, wave_shfl(il, j)};
-        const TcTable t = table_for(x.d);
-        for (int i = lane; i < t.size; i += kWave) tab[i] = -1;
-        wave_lds_sync();
-        for (int i = lane; i < x.d; i += kWave) table_insert(tab, t, olist[x.ob + i]);
-        wave_lds_sync();
-        cnt += task_pairs(orng, olist, ilist, x, 0, x.d + x.di, 0, kWave, tab, t);
-        wave_lds_sync();
-      }
+    // 64 consecutive vertices per grab (8 grabs per wave on average at
+    // R-MAT 22: coarser grabs leave a tail)
+    unsigned v0 = 0;
+    if (lane == 0) v0 = atomic_add_relaxed(&ctl->next, static_cast<unsigned>(kWave));
+    v0 = wave_shfl(v0, 0);
+    if (v0 >= static_cast<unsigned>(nodes)) break;
+    // heavy tasks go to the workgroup passes in parts (|N+(x)| < 2: no
+    // triangle can be found through N+(x))
+    const int vl = static_cast<int>(v0) + lane;
+    const int2 rl = vl < nodes ? orng[vl] : make_int2(0, 0);
+    const int dl = rl.y;
+    const int il = vl < nodes ? ilen[vl] : 0;
+    const bool here = dl >= 2 && dl <= kWaveList && (dl + il) * dl <= kWaveWork;
+    const int parts = dl >= 2 && !here ? (dl + il + kPartItems - 1) / kPartItems : 0;
+    append_parts(dl <= kMidList ? parts : 0, vl, mid, &ctl->nmid);
+    append_parts(dl > kMidList ? parts : 0, vl, big, &ctl->nbig);
+    unsigned long long todo = __ballot(here);
+    while (todo != 0) {
+      const int j = __ffsll(static_cast<long long>(todo)) - 1;
+      todo &= todo - 1;
+      const int ob = wave_shfl(rl.x, j);  // in-lists share the out-lists' positions
+      const TcTask x{ob, wave_shfl(dl, j), ob, wave_shfl(il, j)};
+      const TcTable t = table_for(x.d);
+      for (int i = lane; i < t.size; i += kWave) tab[i] = -1;
+      wave_lds_sync();
+      for (int i = lane; i < x.d; i += kWave) table_insert(tab, t, olist[x.ob + i]);
+      wave_lds_sync();
+      cnt += task_pairs(orng, olist, ilist, x, 0, x.d + x.di, 0, kWave, tab, t);
+      wave_lds_sync();
     }
   }
   cnt = wave_sum(cnt);
