@@ -45,6 +45,24 @@ def test_cc_large(ig, orc, ctx):
         assert np.array_equal(ctx.cc(ig.DeviceGraph(g)), orc.cc(g)), g.name
 
 
+def test_cc_dense_components(ig, orc, ctx):
+    """Average degree >= 8 takes the Afforest order (sampled links, giant
+    root, skipped lists).  Here the largest component does not hold vertex 0,
+    ids of several components interleave, and isolated vertices sit between
+    them, so the skipped tree is neither the first nor the only one."""
+    rng = np.random.default_rng(11)
+    perm = rng.permutation(6000)
+    src, dst = [], []
+    for lo, hi, m in ((0, 1000, 9000), (1000, 5000, 40000), (5000, 5900, 7000)):
+        a = rng.integers(lo, hi, m)
+        b = rng.integers(lo, hi, m)
+        src.append(perm[a])
+        dst.append(perm[b])
+    g = ig.from_edges(6000, np.concatenate(src), np.concatenate(dst))
+    assert g.edges >= 8 * g.nodes
+    assert np.array_equal(ctx.cc(ig.DeviceGraph(g)), orc.cc(g))
+
+
 @pytest.mark.parametrize("signed", [False, True])
 def test_mis_catalog(ig, orc, ctx, signed):
     for g in catalog(ig):
