@@ -52,7 +52,8 @@
 //         relaxations by atomicMin, a vertex is queued for the next round
 //         when its distance for some source drops.
 //   CC  : union-find hooking the larger root under the smaller (labels =
-//         component minima), then flattening.
+//         component minima), then flattening; graphs of average degree 8 or
+//         more link in Afforest order and skip the giant component's lists.
 //   MIS : the greedy fixed point in descending (hash(v + 712313887), v)
 //         order; per round a decide phase (a vertex joins when no neighbour
 //         is included and no undecided neighbour has a larger key) and an
@@ -88,6 +89,9 @@ constexpr int kMaxGang = INDIGO_MAX_GANG;
 constexpr int kGangShift = 18;   // CSR entries per gang member
 constexpr int kHeavyDeg = 1024;  // lists this long are split into items
 constexpr int kItem = 1024;      // entries per item
+constexpr int kCcSample = 2;      // CC: neighbours linked per vertex before the giant root is chosen
+constexpr int kCcSamples = 256;   // CC: vertices sampled to choose it
+constexpr int kCcAfforestMin = 4096;  // CC: smallest graph that takes the Afforest order
 constexpr int kPrWaveList = 32;  // PR: lists walked by a whole wave
 constexpr float kDamp = 0.85f;
 
@@ -522,6 +526,17 @@ IG_DEV int find_root(int* parent, int x)
   return x;
 }
 
+// Root of x without path halving.  The final flatten writes roots into the
+// same array the finds walk: a halving store that read parent[x] before it
+// was flattened could put an intermediate ancestor back over the root, so
+// that pass stores roots only.
+IG_DEV int root_of(const int* parent, int x)
+{
+  int p;
+  while ((p = ld_relaxed(&parent[x])) != x) x = p;
+  return x;
+}
+
 IG_DEV void link(int* parent, int a, int b)
 {
   int ra = find_root(parent, a);
@@ -538,37 +553,97 @@ IG_DEV void link(int* parent, int a, int b)
   }
 }
 
-IG_DEV bool run_cc(const BatchTask& T, const Team& tm)
+// Workgroup-collective: the most frequent root among kCcSamples vertices
+// spread over the id range (ties to the smaller root).  Every gang member
+// reads the same compressed parents, so all choose the same root.
+IG_DEV int giant_root(const BatchTask& T, int* parent, unsigned long long (*s_acc)[kWave])
+{
+  int* s_root = reinterpret_cast<int*>(&s_acc[0][0]);  // kCcSamples ints of the task scratch LDS
+  const int V = T.nodes;
+  for (int i = threadIdx.x; i < kCcSamples; i += blockDim.x)
+    s_root[i] = find_root(parent, static_cast<int>((static_cast<unsigned long long>(i) * 2654435761ull) %
+                                                   static_cast<unsigned>(V)));
+  __syncthreads();
+  unsigned long long key = 0;  // (count << 32) | ~root: the largest wins
+  for (int i = threadIdx.x; i < kCcSamples; i += blockDim.x) {
+    unsigned c = 0;
+    for (int j = 0; j < kCcSamples; j++) c += s_root[j] == s_root[i] ? 1u : 0u;
+    const unsigned long long k = (static_cast<unsigned long long>(c) << 32) | (~static_cast<unsigned>(s_root[i]));
+    key = k > key ? k : key;
+  }
+  __syncthreads();
+  key = wave_max(key);
+  if (lane_id() == 0) s_acc[threadIdx.x / kWave][0] = key;
+  __syncthreads();
+  unsigned long long best = 0;
+  for (int w = 0; w < kBatchWaves; w++) best = s_acc[w][0] > best ? s_acc[w][0] : best;
+  __syncthreads();
+  return static_cast<int>(~static_cast<unsigned>(best & 0xffffffffull));
+}
+
+// CC as the single-query kernel: union-find hooking the larger root under
+// the smaller (labels = component minima).  Graphs with an average degree of
+// at least 8 (and kCcAfforestMin vertices) take the Afforest order - link
+// each vertex with its first kCcSample neighbours, compress, choose the giant
+// root, and let only vertices outside its tree link their lists (all
+// entries); the others link every undirected edge once, from its larger
+// endpoint.
+IG_DEV bool run_cc(const BatchTask& T, const Team& tm, unsigned long long (*s_acc)[kWave])
 {
   const int V = T.nodes;
   const int lane = lane_id();
   int* parent = T.out;
+  const bool aff = static_cast<long long>(T.edges) >= 8LL * V && V >= kCcAfforestMin;
   const Chunks ch = team_chunks(tm, V);
   for (int c = ch.first; c < ch.count; c += ch.step) {
     const int v = c * kWave + lane;
-    if (v < V) st_relaxed(&parent[v], v);
+    if (v < V) {
+      int p = v;
+      if (aff && T.nindex[v] < T.nindex[v + 1]) p = min(v, T.nlist[T.nindex[v]]);
+      st_relaxed(&parent[v], p);
+    }
   }
   int nitems = 0;
-  if (!build_items(T, tm, &nitems)) return false;
+  if (!build_items(T, tm, &nitems)) return false;  // (ends with a team barrier)
+  int giant = -1;
+  if (aff) {
+    for (int c = ch.first; c < ch.count; c += ch.step) {
+      const int v = c * kWave + lane;
+      if (v < V) {
+        const int b = T.nindex[v], e = min(T.nindex[v + 1], b + kCcSample);
+        for (int k = b; k < e; k++) link(parent, v, T.nlist[k]);
+      }
+    }
+    if (!team_sync(tm)) return false;
+    for (int c = ch.first; c < ch.count; c += ch.step) {
+      const int v = c * kWave + lane;
+      if (v < V) st_relaxed(&parent[v], find_root(parent, v));
+    }
+    if (!team_sync(tm)) return false;
+    giant = giant_root(T, parent, s_acc);
+  }
   for (int c = ch.first; c < ch.count; c += ch.step) {
     const int v = c * kWave + lane;
-    wave_entries_of(v, v < V, T.nindex, [&](int o, int e, bool valid) {
+    bool todo = v < V;
+    if (aff && todo) todo = T.nindex[v + 1] - T.nindex[v] > kCcSample && find_root(parent, v) != giant;
+    wave_entries_of(v, todo, T.nindex, [&](int o, int e, bool valid) {
       if (!valid) return;
       const int ov = c * kWave + o;
       const int u = T.nlist[e];
-      if (u < ov) link(parent, ov, u);
+      if (aff || u < ov) link(parent, ov, u);
     });
   }
   for_items(T, tm, nitems, [&](int v, int eb, int ee, int) {
+    if (aff && find_root(parent, v) == giant) return;
     for (int e = eb + lane; e < ee; e += kWave) {
       const int u = T.nlist[e];
-      if (u < v) link(parent, v, u);
+      if (aff || u < v) link(parent, v, u);
     }
   });
   if (!team_sync(tm)) return false;
   for (int c = ch.first; c < ch.count; c += ch.step) {
     const int v = c * kWave + lane;
-    if (v < V) st_relaxed(&parent[v], find_root(parent, v));
+    if (v < V) st_relaxed(&parent[v], root_of(parent, v));
   }
   return true;
 }
@@ -803,7 +878,7 @@ __global__ __launch_bounds__(kBatchBlock) void batch_kernel(const BatchTask* __r
     bool ok = true;
     if (T.alg == INDIGO_QUERY_BFS) ok = run_bfs(T, tm, s_acc);
     else if (T.alg == INDIGO_QUERY_SSSP) ok = run_sssp(T, tm);
-    else if (T.alg == INDIGO_QUERY_CC) ok = run_cc(T, tm);
+    else if (T.alg == INDIGO_QUERY_CC) ok = run_cc(T, tm, s_acc);
     else if (T.alg == INDIGO_QUERY_PR) ok = run_pr(T, tm, s_red);
     else ok = run_mis(T, tm, s_flag);
     if (!ok) break;

@@ -94,6 +94,17 @@ IG_DEV T wave_sum(T x)
 }
 
 template <typename T>
+IG_DEV T wave_max(T x)
+{
+#pragma unroll
+  for (int d = kWave / 2; d > 0; d >>= 1) {
+    const T y = __shfl_xor(x, d, kWave);
+    x = y > x ? y : x;
+  }
+  return x;
+}
+
+template <typename T>
 IG_DEV T wave_min(T x)
 {
 #pragma unroll

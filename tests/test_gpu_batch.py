@@ -148,3 +148,30 @@ def test_batch_big_gangs(ig, orc, ctx, cap):
     _check_against_plain(ctx, [g], [dg], qs[:4])
     scores, iters = qs[4].result()
     _pr_check(ig, orc, g, scores, iters)
+
+
+@pytest.mark.parametrize("gang", [1, 3, 37])
+def test_batch_explicit_gangs(ig, ctx, gang):
+    """Per-query gang sizes (the autotuner's knob, indigo_query.gang): any
+    size from 1 to the grid, odd ones included, gives the same results."""
+    g = ig.rmat(16, 8, seed=21, wmin=1, wmax=255)
+    road = ig.road(120, 100, keep_p=0.8, seed=4, wmin=1, wmax=1000)
+    gs = [g, road]
+    dgs = [ig.DeviceGraph(x) for x in gs]
+    rng = np.random.default_rng(gang)
+    qs = []
+    for gi, x in enumerate(gs):
+        qs += [ig.Query("bfs", gi, dgs[gi], rng.integers(0, x.nodes, 32), gang=gang),
+               ig.Query("sssp", gi, dgs[gi], rng.integers(0, x.nodes, 8), gang=gang),
+               ig.Query("cc", gi, dgs[gi], gang=gang), ig.Query("mis", gi, dgs[gi], gang=gang)]
+    print(f"batch with gangs of {gang}", flush=True)
+    ctx.batch(dgs, qs)
+    _check_against_plain(ctx, gs, dgs, qs)
+
+
+def test_batch_rejects_bad_gang(ig, ctx):
+    g = ig.grid(4, 4)
+    dg = ig.DeviceGraph(g)
+    for bad in (-1, ig._lib.MAX_GANG + 1):
+        with pytest.raises(ig.IndigoError):
+            ctx.batch([dg], [ig.Query("cc", 0, dg, gang=bad)])
