@@ -114,7 +114,9 @@ IG_DEV void relax(const SsspArgs& a, int dv, int e, bool active, int T, int it_s
   if (active) {
     u = a.nlist[e];
     nd = dv + a.eweight[e];
-    if (nd < a.dist[u]) {
+    // device-scope read: a plain load may see a stale L2 copy of a distance
+    // another XCD has lowered, and let a useless atomic through
+    if (nd < ld_relaxed(&a.dist[u])) {
       const int old = atomic_min_relaxed(&a.dist[u], nd);
       if (nd < old) {
         udeg = degree_of(a.nindex, u);
