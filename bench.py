@@ -140,7 +140,9 @@ def kernel_bytes(alg: str, g, pr_iters: int, tc_model: dict) -> float:
     return tc_model["bytes"]
 
 
-MAIN_KERNELS = {"bfs": "bfs_kernel", "sssp": "sssp_kernel", "cc": "cc_link", "mis": "mis_kernel", "pr": "pr_kernel",
+# one main kernel per algorithm for the traffic record (CC's timed span is a
+# sequence of kernels: its traffic is not recorded)
+MAIN_KERNELS = {"bfs": "bfs_kernel", "sssp": "sssp_kernel", "mis": "mis_kernel", "pr": "pr_kernel",
                 "tc": "tc_count_group<512, 8192>"}
 
 
@@ -183,7 +185,8 @@ def record_traffic(paths: list[str], workload: str, out: str) -> int:
     ffac, wfac = gib_kib / statistics.median(cfetch), gib_kib / statistics.median(cwrite)
     kernels = {}
     for alg, kernel in MAIN_KERNELS.items():
-        match = lambda name, k=kernel: f"{k}(" in name
+        # plain or templated symbol (the kernel may be named without its template arguments)
+        match = lambda name, k=kernel: f"::{k}(" in name or f"::{k}<" in name
         fetch, write = _pmc_values(kf, "FETCH_SIZE", match), _pmc_values(kw, "WRITE_SIZE", match)
         if not (fetch and write):
             continue
