@@ -36,6 +36,7 @@ namespace {
 constexpr int kMisBlock = 512;
 
 constexpr int kUndecided = 0, kIncluded = 1, kExcluded = 2;
+constexpr int kMisLaneUnroll = 4;  // entries per step of a one-lane decision
 
 struct MisControl {
   GridBarrier bar;
@@ -67,22 +68,38 @@ IG_DEV unsigned long long mis_key(int v, unsigned flip)
 // One lane decides vertex v (short list).  Returns the new state.
 IG_DEV int decide_lane(const MisArgs& a, int v, int beg, int end)
 {
+  // Only neighbours with a larger key can decide v: a smaller one cannot be
+  // included while v is undecided (it would need v excluded first), so their
+  // states are not read.  Four entries' loads are in flight at a time and
+  // evaluated in entry order.
   const unsigned long long kv = mis_key(v, a.flip);
-  for (int e = beg; e < end; e++) {
-    const int u = a.nlist[e];
-    if (u == v) continue;
-    const int su = ld_relaxed(&a.status[u]);
-    if (su == kIncluded) return kExcluded;
-    if (su == kUndecided && mis_key(u, a.flip) > kv) return kUndecided;
+  for (int e = beg; e < end; e += kMisLaneUnroll) {
+    int su[kMisLaneUnroll];
+#pragma unroll
+    for (int q = 0; q < kMisLaneUnroll; q++) {
+      const int u = e + q < end ? a.nlist[e + q] : v;
+      const bool larger = u != v && mis_key(u, a.flip) > kv;
+      su[q] = larger ? ld_relaxed(&a.status[u]) : kExcluded;
+    }
+#pragma unroll
+    for (int q = 0; q < kMisLaneUnroll; q++) {
+      if (su[q] == kIncluded) return kExcluded;
+      if (su[q] == kUndecided) return kUndecided;
+    }
   }
   return kIncluded;
 }
-
+// v has just been included: its larger-key neighbours are all excluded
+// already, so only the smaller-key ones are read, and the undecided among
+// them excluded.  (Storing unconditionally is slower: device-scope stores
+// to already excluded hub neighbours cost more than the reads they save.)
 IG_DEV void exclude_neighbours_lane(const MisArgs& a, int v, int beg, int end)
 {
+  const unsigned long long kv = mis_key(v, a.flip);
   for (int e = beg; e < end; e++) {
     const int u = a.nlist[e];
-    if (u != v && ld_relaxed(&a.status[u]) == kUndecided) st_relaxed(&a.status[u], kExcluded);
+    if (u != v && mis_key(u, a.flip) < kv && ld_relaxed(&a.status[u]) == kUndecided)
+      st_relaxed(&a.status[u], kExcluded);
   }
 }
 
@@ -99,10 +116,10 @@ IG_DEV int decide_wave(const MisArgs& a, int v, int beg, int end)
       const int e = base + k * kWave + lane;
       if (e < end) {
         const int u = a.nlist[e];
-        if (u != v) {
+        if (u != v && mis_key(u, a.flip) > kv) {  // smaller keys cannot decide v (see decide_lane)
           const int su = ld_relaxed(&a.status[u]);
           inc |= su == kIncluded;
-          hi |= su == kUndecided && mis_key(u, a.flip) > kv;
+          hi |= su == kUndecided;
         }
       }
     }
@@ -114,9 +131,11 @@ IG_DEV int decide_wave(const MisArgs& a, int v, int beg, int end)
 
 IG_DEV void exclude_neighbours_wave(const MisArgs& a, int v, int beg, int end)
 {
+  const unsigned long long kv = mis_key(v, a.flip);
   for (int e = beg + lane_id(); e < end; e += kWave) {
     const int u = a.nlist[e];
-    if (u != v && ld_relaxed(&a.status[u]) == kUndecided) st_relaxed(&a.status[u], kExcluded);
+    if (u != v && mis_key(u, a.flip) < kv && ld_relaxed(&a.status[u]) == kUndecided)  // as exclude_neighbours_lane
+      st_relaxed(&a.status[u], kExcluded);
   }
 }
 
