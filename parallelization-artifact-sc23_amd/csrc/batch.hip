@@ -688,8 +688,9 @@ IG_DEV bool run_mis(const BatchTask& T, const Team& tm, int (*s_flag)[kWave])
         if (!valid) return;
         const int ov = c * kWave + o;
         const int u = T.nlist[e];
-        const int su = ld_relaxed(&status[u]);
-        if (su == 1 || (su == 0 && u != ov && mis_key(u) > mis_key(ov))) s_flag[wid][o] = 1;
+        // only larger keys can block (or be included next to) an undecided
+        // vertex: a smaller one cannot be included before it is decided
+        if (u != ov && mis_key(u) > mis_key(ov) && ld_relaxed(&status[u]) != 2) s_flag[wid][o] = 1;
       });
       wave_lds_sync();
       if (und && s_flag[wid][lane] == 0) st_relaxed(&status[v], 1);
@@ -700,9 +701,10 @@ IG_DEV bool run_mis(const BatchTask& T, const Team& tm, int (*s_flag)[kWave])
       bool blk = false, inc = false;
       for (int e = eb + lane; e < ee; e += kWave) {
         const int u = T.nlist[e];
+        if (u == v || mis_key(u) < kv) continue;  // smaller keys cannot decide v
         const int su = ld_relaxed(&status[u]);
         inc |= su == 1;
-        blk |= su == 1 || (su == 0 && u != v && mis_key(u) > kv);
+        blk |= su != 2;
       }
       const bool any_blk = __ballot(blk) != 0ull, any_inc = __ballot(inc) != 0ull;
       if (lane == 0 && any_blk) st_relaxed(&blocked[v], 1u);
@@ -719,7 +721,10 @@ IG_DEV bool run_mis(const BatchTask& T, const Team& tm, int (*s_flag)[kWave])
       s_flag[wid][lane] = 0;
       wave_lds_sync();
       wave_entries_of(v, und && !hv, T.nindex, [&](int o, int e, bool valid) {
-        if (valid && ld_relaxed(&status[T.nlist[e]]) == 1) s_flag[wid][o] = 1;
+        if (!valid) return;
+        const int u = T.nlist[e];
+        // an included neighbour of an undecided vertex has a larger key
+        if (mis_key(u) > mis_key(c * kWave + o) && ld_relaxed(&status[u]) == 1) s_flag[wid][o] = 1;
       });
       wave_lds_sync();
       bool still = false;
