@@ -247,18 +247,24 @@ __global__ __launch_bounds__(kTcBlock) void tc_select_heavy(const int* __restric
     const int dv = ve - vb;
     const int odv = sel.own_odeg(v);
     int k = 0;
-    for (int base = vb; base < ve; base += kWave) {
-      const int e = base + lane;
-      bool keep = false;
-      int u = 0;
-      if (e < ve) {
-        u = nlist[e];
-        if (kOut) bad |= (u == v) || (e > vb && nlist[e - 1] >= u);
-        keep = sel(v, dv, odv, u);
+    // kUnroll windows per step: their list and neighbour-state loads overlap
+    for (int base = vb; base < ve; base += kUnroll * kWave) {
+      int u[kUnroll];
+#pragma unroll
+      for (int q = 0; q < kUnroll; q++) {
+        const int e = base + q * kWave + lane;
+        u[q] = e < ve ? nlist[e] : -1;
+        if (kOut && e < ve) bad |= (u[q] == v) || (e > vb && nlist[e - 1] >= u[q]);
       }
-      const unsigned long long m = __ballot(keep);
-      if (keep) list[vb + k + lanes_below(m)] = u;
-      k += __popcll(m);
+      bool keep[kUnroll];
+#pragma unroll
+      for (int q = 0; q < kUnroll; q++) keep[q] = u[q] >= 0 && sel(v, dv, odv, u[q]);
+#pragma unroll
+      for (int q = 0; q < kUnroll; q++) {
+        const unsigned long long m = __ballot(keep[q]);
+        if (keep[q]) list[vb + k + lanes_below(m)] = u[q];
+        k += __popcll(m);
+      }
     }
     if (lane == 0) rec(v, dv, vb, k);
   }
