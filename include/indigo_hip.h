@@ -166,6 +166,8 @@ int indigo_ctx_set_hint(indigo_ctx* ctx, int key, int value);
 #define INDIGO_HINT_TC_LDS_LIST 6    /* longest oriented list TC hashes in LDS (0: 16384); longer ones use binary search */
 #define INDIGO_HINT_RESIDENCY_MS 7   /* how long a persistent grid may take to assemble (0: 100 ms); longer when
                                         the GPU is shared with other work */
+#define INDIGO_HINT_MAX_WORKGROUPS 8 /* cap on the workgroups of any persistent grid of the context (0: the
+                                        whole GPU); for contexts that share the GPU */
 
 /* All resident entries take the device CSR as (nodes, edges, d_nindex,
  * d_nlist[, d_eweight]) and return after the result is in device memory. */

@@ -93,6 +93,7 @@ int coop_launch(indigo_ctx* ctx, const void* kernel, int block, size_t shmem, vo
   if (ctx->hints[INDIGO_HINT_BLOCKS_PER_CU] > 0) per_cu = std::min(per_cu, ctx->hints[INDIGO_HINT_BLOCKS_PER_CU]);
   int grid = per_cu * ctx->compute_units;
   if (max_blocks > 0) grid = std::min(grid, max_blocks);
+  if (ctx->hints[INDIGO_HINT_MAX_WORKGROUPS] > 0) grid = std::min(grid, ctx->hints[INDIGO_HINT_MAX_WORKGROUPS]);
   grid = std::max(grid, 1);
   if (name != nullptr) IG_TRY(kernel_begin(ctx, name));
   IG_HIP_TRY(hipLaunchKernel(kernel, dim3(grid), dim3(block), args, static_cast<unsigned>(shmem), ctx->stream));

@@ -409,8 +409,6 @@ int serve_run(const indigo_graph_ref* graphs, int ngraphs, indigo_request* reqs,
     if (rc != INDIGO_OK) break;
     // the GPU is shared: give persistent grids time to assemble
     s.ctx->hints[INDIGO_HINT_RESIDENCY_MS] = 2000;
-    // single-query persistent kernels take a share of the CUs, not the GPU
-    s.ctx->hints[INDIGO_HINT_BLOCKS_PER_CU] = 1;
   }
   // workspaces sized up front for the largest triangle count, so that no
   // slot reallocates (hipFree waits for the device) while others run
@@ -426,6 +424,10 @@ int serve_run(const indigo_graph_ref* graphs, int ngraphs, indigo_request* reqs,
   int limit = 0;
   if (rc == INDIGO_OK) rc = batch_grid_limit(sv.slots[0].ctx, &limit);
   sv.grid_per_slot = std::max(1, limit / cfg->slots);
+  // single-query persistent kernels take the slot's share of the GPU too: a
+  // grid sized for the whole GPU cannot assemble beside the other slots
+  for (Slot& s : sv.slots)
+    if (s.ctx != nullptr) s.ctx->hints[INDIGO_HINT_MAX_WORKGROUPS] = sv.grid_per_slot;
   if (rc == INDIGO_OK) rc = sv.run();
   if (rc != INDIGO_OK) {
     const std::string msg = indigo_last_error();

@@ -30,6 +30,7 @@ HINT_BUDGET_MS = 4
 HINT_BLOCKS_PER_CU = 5
 HINT_TC_LDS_LIST = 6
 HINT_RESIDENCY_MS = 7
+HINT_MAX_WORKGROUPS = 8
 
 
 class IndigoError(RuntimeError):

@@ -173,3 +173,23 @@ def test_copy_ms_copies_and_times(ig, ctx):
     ms = ctx.copy_ms(b, a, src.nbytes)
     assert ms > 0
     assert np.array_equal(b.to_array(np.int32, src.size), src)
+
+
+@pytest.mark.parametrize("cap", [1, 7])
+def test_persistent_grid_cap(ig, orc, ctx, cap):
+    """INDIGO_HINT_MAX_WORKGROUPS (contexts that share the GPU, as the serving
+    slots do): the persistent kernels give the same results on a grid of one
+    workgroup and on an odd grid smaller than the eight barrier groups."""
+    g = ig.rmat(13, 16, seed=5, wmin=1, wmax=255)
+    road = ig.road(80, 60, keep_p=0.8, seed=2, wmin=1, wmax=1000)
+    ctx.set_hint(ig._lib.HINT_MAX_WORKGROUPS, cap)
+    try:
+        for h in (g, road):
+            dg = ig.DeviceGraph(h)
+            assert np.array_equal(ctx.bfs(dg, 0), orc.bfs(h, 0)), (h.name, "bfs")
+            assert np.array_equal(ctx.sssp(dg, 0), orc.sssp(h, 0)), (h.name, "sssp")
+            assert np.array_equal(ctx.mis(dg), orc.mis(h)), (h.name, "mis")
+            scores, iters = ctx.pagerank(dg)
+            _pr_check(ig, orc, h, scores, iters)
+    finally:
+        ctx.set_hint(ig._lib.HINT_MAX_WORKGROUPS, 0)
