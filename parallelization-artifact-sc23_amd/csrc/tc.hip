@@ -70,6 +70,10 @@ constexpr int kWaveWork = 1 << 15;  // probe bound of a wave-pass task
 constexpr int kPartItems = 1024;    // items per workgroup-pass part
 constexpr int kUnroll = 4;          // windows of 64 per step over a long list
 constexpr int kLongList = 32;       // lists walked by the whole wave
+constexpr int kTinyList = 16;       // wave-pass tasks this small are counted kTinyGroup at a time
+constexpr int kTinyGroup = 16;      // 16 tables of 64 slots fill the wave's LDS region
+static_assert(4 * kTinyList <= 64 && kTinyGroup * 64 <= kWaveSlots && kTinyList < kLongList,
+              "tiny-task tables: 64 slots each, all in the wave's region, no long lists");
 
 struct TcControl {
   FrontierCounters heavy;  // lists of >= kMidDeg entries
@@ -433,6 +437,83 @@ IG_DEV unsigned long long task_pairs(const int2* __restrict__ orng, const int* _
   return cnt;
 }
 
+// Wave-collective: the tiny tasks (|N+(x)| <= kTinyList) of the lanes in
+// `mask` (at most kTinyGroup of them), counted together: each gets a 64-slot
+// table in the wave's LDS region, and their items (N+(x), then N-(x)) and
+// then the items' list entries are spread over the wave with the in-wave
+// balancer.  A counted item's list is no longer than its task's N+(x) (the
+// shorter-list rule), so at most kTinyList entries: no long-list walk.  One
+// task at a time would pay its own chain of round trips for a handful of
+// probes.
+IG_DEV unsigned long long tiny_tasks(unsigned long long mask, const int2 rl, int dl, int il,
+                                     const int2* __restrict__ orng, const int* __restrict__ olist,
+                                     const int* __restrict__ ilist, int* tab)
+{
+  const int lane = lane_id();
+  // lane j takes the task of the j-th lane of `mask`; lanes past the last
+  // task hold an empty one
+  int src = kWave;
+  {
+    unsigned long long m = mask;
+    for (int j = 0; m != 0; j++) {
+      const int sl = __ffsll(static_cast<long long>(m)) - 1;
+      m &= m - 1;
+      if (lane == j) src = sl;
+    }
+  }
+  const int ntask = __popcll(mask);
+  const bool has = src < kWave;
+  const int tob = wave_shfl(rl.x, has ? src : 0);  // in-lists share the out-lists' positions
+  const int td0 = wave_shfl(dl, has ? src : 0);
+  const int ti0 = wave_shfl(il, has ? src : 0);
+  const int td = has ? td0 : 0;
+  const int tn = has ? td0 + ti0 : 0;  // items: N+(x), then N-(x)
+  const TcTable t = table_for(kTinyList);  // 64 slots for every task
+  for (int i = lane; i < ntask * t.size; i += kWave) tab[i] = -1;
+  wave_lds_sync();
+  {
+    const int incl = wave_inclusive_scan(td);
+    const int total = wave_bcast(incl, kWave - 1);
+    const int shift = tob - (incl - td);  // entry k of the concatenation is olist[shift[j] + k]
+    for (int base = 0; base < total; base += kWave) {
+      const int k = base + lane;
+      const int j = min(wave_owner(incl, k), kWave - 1);
+      const int at = wave_shfl(shift, j) + k;
+      if (k < total) table_insert(tab + j * t.size, t, olist[at]);
+    }
+  }
+  wave_lds_sync();
+  unsigned long long cnt = 0;
+  const int incl_n = wave_inclusive_scan(tn);
+  const int total_n = wave_bcast(incl_n, kWave - 1);
+  for (int ibase = 0; ibase < total_n; ibase += kWave) {
+    const int k = ibase + lane;
+    const int j = min(wave_owner(incl_n, k), kWave - 1);
+    const int i = k - wave_shfl(incl_n - tn, j);
+    const int jd = wave_shfl(td, j);
+    const int job = wave_shfl(tob, j);
+    int b = 0, len = 0;
+    if (k < total_n) {
+      const int u = i < jd ? olist[job + i] : ilist[job + (i - jd)];
+      const int2 r = orng[u];
+      b = r.x;
+      len = r.y <= jd ? r.y : 0;  // the shorter-list rule of count_pairs
+    }
+    const int incl = wave_inclusive_scan(len);
+    const int total = wave_bcast(incl, kWave - 1);
+    const int shift = b - (incl - len);
+    for (int pbase = 0; pbase < total; pbase += kWave) {
+      const int kk = pbase + lane;
+      const int o = min(wave_owner(incl, kk), kWave - 1);
+      const int at = wave_shfl(shift, o) + kk;
+      const int tj = wave_shfl(j, o);
+      if (kk < total) cnt += table_find(tab + tj * t.size, t, olist[at]) ? 1ull : 0ull;
+    }
+  }
+  wave_lds_sync();
+  return cnt;
+}
+
 // Wave-collective: appends the parts (x << 32 | part) of each lane's task.
 IG_DEV void append_parts(int parts, int x, unsigned long long* queue, unsigned* count)
 {
@@ -474,6 +555,18 @@ __global__ __launch_bounds__(kTcBlock) void tc_count(int nodes, const int2* __re
     append_parts(dl <= kMidList ? parts : 0, vl, mid, &ctl->nmid);
     append_parts(dl > kMidList ? parts : 0, vl, big, &ctl->nbig);
     unsigned long long todo = __ballot(here);
+    // tiny tasks first, kTinyGroup at a time
+    unsigned long long tiny = __ballot(here && dl <= kTinyList);
+    todo &= ~tiny;
+    while (tiny != 0) {
+      unsigned long long grp = 0;
+      for (int q = 0; q < kTinyGroup && tiny != 0; q++) {
+        const unsigned long long low = tiny & (~tiny + 1ull);
+        grp |= low;
+        tiny &= tiny - 1;
+      }
+      cnt += tiny_tasks(grp, rl, dl, il, orng, olist, ilist, tab);
+    }
     while (todo != 0) {
       const int j = __ffsll(static_cast<long long>(todo)) - 1;
       todo &= todo - 1;
