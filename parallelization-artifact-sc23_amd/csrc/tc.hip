@@ -58,7 +58,7 @@ namespace {
 
 constexpr int kTcBlock = 256;
 constexpr int kWaveList = 256;    // longest N+(x) the wave pass takes
-constexpr int kWaveSlots = 1024;  // its LDS table (4 slots per entry)
+constexpr int kWaveSlots = 1024;  // its largest LDS table (4 slots per entry)
 constexpr int kMidBlock = 512;
 constexpr int kMidList = 2048;
 constexpr int kMidSlots = 8192;
@@ -71,8 +71,11 @@ constexpr int kPartItems = 1024;    // items per workgroup-pass part
 constexpr int kUnroll = 4;          // windows of 64 per step over a long list
 constexpr int kLongList = 32;       // lists walked by the whole wave
 constexpr int kTinyList = 16;       // wave-pass tasks this small are counted kTinyGroup at a time
-constexpr int kTinyGroup = 16;      // 16 tables of 64 slots fill the wave's LDS region
-static_assert(4 * kTinyList <= 64 && kTinyGroup * 64 <= kWaveSlots && kTinyList < kLongList,
+constexpr int kTinyGroup = 16;      // tables of kTinySlots (+ mirror) side by side in the wave's region
+constexpr int kTinySlots = 64;
+constexpr int kWaveRegion = kWaveSlots + 16;  // a wave's LDS words
+static_assert(4 * kTinyList <= kTinySlots && kTinyGroup * (kTinySlots + 1) <= kWaveRegion &&
+                  kWaveSlots + 1 <= kWaveRegion && kTinyList < kLongList,
               "tiny-task tables: 64 slots each, all in the wave's region, no long lists");
 
 struct TcControl {
@@ -317,6 +320,16 @@ __global__ __launch_bounds__(kTcBlock) void tc_pack_huge(const int* __restrict__
   }
 }
 
+// list[k] with a 32-bit byte offset from a wave-uniform base (k < 2^30):
+// the load takes the base from scalar registers.
+IG_DEV int load_at(const int* list, int k)
+{
+  return *reinterpret_cast<const int*>(reinterpret_cast<const char*>(list) + static_cast<unsigned>(k) * 4u);
+}
+
+// No vertex id and not the empty-slot marker: never found in a table.
+constexpr int kNoEntry = -2;
+
 IG_DEV bool search(const int* list, int n, int key)
 {
   int lo = 0, hi = n;
@@ -328,36 +341,54 @@ IG_DEV bool search(const int* list, int n, int key)
   return lo < n && list[lo] == key;
 }
 
-// Hash table geometry for a list of d entries: a power of two >= 4d slots
-// (at least 64); the slot of w is the top log2(size) bits of w * 2654435761
-// (Fibonacci hashing).  Empty slots hold -1.
+// Hash table geometry for a list of d entries: a power of two >= 8d slots
+// (load <= 1/8) when that fits in `cap` slots, else >= 4d (callers keep 4d
+// within cap), and at least 64.  The slot of w is the top log2(size) bits of
+// the low 32 bits of (w mod 2^24) * 0x9E3779 (multiplicative hashing with a
+// full-rate 24-bit multiply; ids above 2^24 share home slots, which costs
+// probes, never results).  Empty slots hold -1, and slot `size` repeats slot
+// 0, so that a probe reads two consecutive slots with one ds_read2 and never
+// wraps within a pair: a table takes size + 1 words.
 struct TcTable {
   int size;
   unsigned mask;
   int shift;
 };
 
-IG_DEV TcTable table_for(int d)
+IG_DEV TcTable table_for(int d, int cap)
 {
-  const int size = max(64, 1 << (32 - __clz(4 * d - 1)));
+  int size = max(64, 1 << (32 - __clz(8 * d - 1)));
+  if (size > cap) size >>= 1;
   return TcTable{size, static_cast<unsigned>(size - 1), __clz(size) + 1};
 }
 
-IG_DEV unsigned slot_of(int w, int shift) { return (static_cast<unsigned>(w) * 2654435761u) >> shift; }
+IG_DEV unsigned slot_of(int w, int shift)
+{
+  return static_cast<unsigned>(__umul24(static_cast<unsigned>(w), 0x9E3779u)) >> shift;
+}
 
 IG_DEV void table_insert(int* tab, const TcTable& t, int w)
 {
   unsigned h = slot_of(w, t.shift);
   while (atomicCAS(&tab[h], -1, w) != -1) h = (h + 1) & t.mask;
+  if (h == 0) tab[t.size] = w;  // the mirror of slot 0
 }
 
-IG_DEV bool table_find(const int* tab, const TcTable& t, int w)
+// Linear probing two slots at a time from slot h: w was inserted before the
+// first empty slot of its probe sequence.
+IG_DEV bool table_find_from(const int* tab, const TcTable& t, int w, unsigned h)
 {
-  unsigned h = slot_of(w, t.shift);
-  int x;
-  while ((x = tab[h]) != -1 && x != w) h = (h + 1) & t.mask;
-  return x == w;
+  // w is never in the second slot of a pair whose first is empty
+  int a = tab[h], b = tab[h + 1];
+  while ((a != w) & (b != w) & (a != -1) & (b != -1)) {
+    h = (h + 2) & t.mask;
+    a = tab[h];
+    b = tab[h + 1];
+  }
+  return (a == w) | (b == w);
 }
+
+IG_DEV bool table_find(const int* tab, const TcTable& t, int w) { return table_find_from(tab, t, w, slot_of(w, t.shift)); }
 
 IG_DEV void wave_lds_sync()
 {
@@ -379,13 +410,19 @@ struct TcTask {
 // entries (97% of the probes on R-MAT) are walked by the whole wave, one at a
 // time, kUnroll windows of 64 per step so that their loads overlap; shorter
 // ones share windows of 64 pairs through the in-wave balancer.
+template <bool kHashed>
 IG_DEV unsigned long long count_pairs(const int2* __restrict__ orng, const int* __restrict__ olist,
                                       const int* __restrict__ items, int n, int limit, int ub0, int ustep,
                                       const int* tab, const TcTable& t, const int* nx, int d)
 {
   const int lane = lane_id();
-  unsigned long long cnt = 0;
-  auto probe = [&](int w) { return tab != nullptr ? table_find(tab, t, w) : search(nx, d, w); };
+  // a lane probes at most n * (limit / 64 + 1) entries here: 32 bits suffice
+  unsigned cnt = 0;
+  unsigned long long wcnt = 0;  // the long-list hits of the whole wave
+  auto probe = [&](int w) {
+    if constexpr (kHashed) return table_find(tab, t, w);
+    else return search(nx, d, w);
+  };
   for (int ub = ub0; ub < n; ub += ustep) {
     const int i = ub + lane;
     const int2 r = i < n ? orng[items[i]] : make_int2(0, 0);
@@ -396,17 +433,43 @@ IG_DEV unsigned long long count_pairs(const int2* __restrict__ orng, const int* 
     while (todo != 0) {
       const int o = __ffsll(static_cast<long long>(todo)) - 1;
       todo &= todo - 1;
-      const int ob2 = wave_bcast(b, o), ol = wave_bcast(len, o);
+      const int ol = wave_bcast(len, o);
+      const int* row = olist + wave_bcast(b, o);
       for (int base = 0; base < ol; base += kUnroll * kWave) {
+        // every lane loads (past the end: the last entry again, a cache
+        // hit) so that the loads issue back to back; kNoEntry marks the
+        // lanes past the end
         int w[kUnroll];
 #pragma unroll
         for (int q = 0; q < kUnroll; q++) {
           const int k = base + q * kWave + lane;
-          w[q] = k < ol ? olist[ob2 + k] : -1;
+          const int v = load_at(row, min(k, ol - 1));
+          w[q] = k < ol ? v : kNoEntry;
         }
+        if constexpr (kHashed) {
+          // the first pair of slots of every window at once; the few lanes
+          // whose probe runs on continue two slots at a time.  w is never in
+          // the second slot when the first is empty: it was inserted before
+          // the first empty slot of its probe sequence.
+          unsigned h[kUnroll];
+          int s0[kUnroll], s1[kUnroll];
 #pragma unroll
-        for (int q = 0; q < kUnroll; q++)
-          if (w[q] >= 0) cnt += probe(w[q]) ? 1ull : 0ull;
+          for (int q = 0; q < kUnroll; q++) {
+            h[q] = slot_of(w[q], t.shift);
+            s0[q] = tab[h[q]];
+            s1[q] = tab[h[q] + 1];
+          }
+#pragma unroll
+          for (int q = 0; q < kUnroll; q++) {
+            bool hit = (s0[q] == w[q]) | (s1[q] == w[q]);
+            if (!hit & (s0[q] != -1) & (s1[q] != -1)) hit = table_find_from(tab, t, w[q], (h[q] + 2) & t.mask);
+            wcnt += static_cast<unsigned long long>(__popcll(__ballot(hit)));
+          }
+        } else {
+#pragma unroll
+          for (int q = 0; q < kUnroll; q++)
+            if (w[q] >= 0) cnt += search(nx, d, w[q]) ? 1u : 0u;
+        }
       }
     }
     const int slen = len < kLongList ? len : 0;
@@ -417,13 +480,14 @@ IG_DEV unsigned long long count_pairs(const int2* __restrict__ orng, const int* 
       const int k = base + lane;
       const int o = min(wave_owner(incl, k), kWave - 1);
       const int at = wave_shfl(shift, o) + k;
-      if (k < total) cnt += probe(olist[at]) ? 1ull : 0ull;
+      if (k < total) cnt += probe(olist[at]) ? 1u : 0u;
     }
   }
-  return cnt;
+  return cnt + (lane == 0 ? wcnt : 0ull);
 }
 
 // Items [lo, hi) of the task's item sequence (N+(x), then N-(x)).
+template <bool kHashed>
 IG_DEV unsigned long long task_pairs(const int2* __restrict__ orng, const int* __restrict__ olist,
                                      const int* __restrict__ ilist, const TcTask& x, int lo, int hi, int ub0,
                                      int ustep, const int* tab, const TcTable& t)
@@ -431,9 +495,10 @@ IG_DEV unsigned long long task_pairs(const int2* __restrict__ orng, const int* _
   const int* nx = olist + x.ob;
   unsigned long long cnt = 0;
   const int a1 = min(hi, x.d);
-  if (lo < a1) cnt += count_pairs(orng, olist, nx + lo, a1 - lo, x.d, ub0, ustep, tab, t, nx, x.d);
+  if (lo < a1) cnt += count_pairs<kHashed>(orng, olist, nx + lo, a1 - lo, x.d, ub0, ustep, tab, t, nx, x.d);
   const int b0 = max(lo, x.d) - x.d, b1 = hi - x.d;
-  if (b0 < b1) cnt += count_pairs(orng, olist, ilist + x.ib + b0, b1 - b0, x.d, ub0, ustep, tab, t, nx, x.d);
+  if (b0 < b1)
+    cnt += count_pairs<kHashed>(orng, olist, ilist + x.ib + b0, b1 - b0, x.d, ub0, ustep, tab, t, nx, x.d);
   return cnt;
 }
 
@@ -468,8 +533,9 @@ IG_DEV unsigned long long tiny_tasks(unsigned long long mask, const int2 rl, int
   const int ti0 = wave_shfl(il, has ? src : 0);
   const int td = has ? td0 : 0;
   const int tn = has ? td0 + ti0 : 0;  // items: N+(x), then N-(x)
-  const TcTable t = table_for(kTinyList);  // 64 slots for every task
-  for (int i = lane; i < ntask * t.size; i += kWave) tab[i] = -1;
+  const TcTable t = table_for(kTinyList, kTinySlots);  // 64 slots for every task
+  const int stride = t.size + 1;                      // and the mirror slot
+  for (int i = lane; i < ntask * stride; i += kWave) tab[i] = -1;
   wave_lds_sync();
   {
     const int incl = wave_inclusive_scan(td);
@@ -479,7 +545,7 @@ IG_DEV unsigned long long tiny_tasks(unsigned long long mask, const int2 rl, int
       const int k = base + lane;
       const int j = min(wave_owner(incl, k), kWave - 1);
       const int at = wave_shfl(shift, j) + k;
-      if (k < total) table_insert(tab + j * t.size, t, olist[at]);
+      if (k < total) table_insert(tab + j * stride, t, olist[at]);
     }
   }
   wave_lds_sync();
@@ -507,7 +573,7 @@ IG_DEV unsigned long long tiny_tasks(unsigned long long mask, const int2 rl, int
       const int o = min(wave_owner(incl, kk), kWave - 1);
       const int at = wave_shfl(shift, o) + kk;
       const int tj = wave_shfl(j, o);
-      if (kk < total) cnt += table_find(tab + tj * t.size, t, olist[at]) ? 1ull : 0ull;
+      if (kk < total) cnt += table_find(tab + tj * stride, t, olist[at]) ? 1ull : 0ull;
     }
   }
   wave_lds_sync();
@@ -533,7 +599,7 @@ __global__ __launch_bounds__(kTcBlock) void tc_count(int nodes, const int2* __re
                                                      const int* __restrict__ ilist, unsigned long long* mid,
                                                      unsigned long long* big, TcControl* ctl)
 {
-  __shared__ int s_tab[kTcBlock / kWave][kWaveSlots];
+  __shared__ int s_tab[kTcBlock / kWave][kWaveRegion];
   const int lane = lane_id();
   int* tab = s_tab[threadIdx.x / kWave];
   unsigned long long cnt = 0;
@@ -572,12 +638,12 @@ __global__ __launch_bounds__(kTcBlock) void tc_count(int nodes, const int2* __re
       todo &= todo - 1;
       const int ob = wave_shfl(rl.x, j);  // in-lists share the out-lists' positions
       const TcTask x{ob, wave_shfl(dl, j), ob, wave_shfl(il, j)};
-      const TcTable t = table_for(x.d);
-      for (int i = lane; i < t.size; i += kWave) tab[i] = -1;
+      const TcTable t = table_for(x.d, kWaveSlots);
+      for (int i = lane; i <= t.size; i += kWave) tab[i] = -1;
       wave_lds_sync();
       for (int i = lane; i < x.d; i += kWave) table_insert(tab, t, olist[x.ob + i]);
       wave_lds_sync();
-      cnt += task_pairs(orng, olist, ilist, x, 0, x.d + x.di, 0, kWave, tab, t);
+      cnt += task_pairs<true>(orng, olist, ilist, x, 0, x.d + x.di, 0, kWave, tab, t);
       wave_lds_sync();
     }
   }
@@ -596,7 +662,7 @@ __global__ __launch_bounds__(kBlock) void tc_count_group(const int2* __restrict_
                                                          const unsigned* nparts, unsigned* next, int lds_list,
                                                          TcControl* ctl)
 {
-  __shared__ int s_tab[kSlots];
+  __shared__ int s_tab[kSlots + 1];  // a table and its mirror slot
   __shared__ unsigned s_k;
   constexpr int kWaves = kBlock / kWave;
   const int wid = threadIdx.x / kWave;
@@ -613,15 +679,16 @@ __global__ __launch_bounds__(kBlock) void tc_count_group(const int2* __restrict_
     const int2 r = orng[v];
     const TcTask x{r.x, r.y, r.x, ilen[v]};
     const bool hashed = x.d <= lds_list;
-    const TcTable t = table_for(hashed ? x.d : 1);
+    const TcTable t = table_for(hashed ? x.d : 1, kSlots);
     if (hashed) {
-      for (int i = threadIdx.x; i < t.size; i += kBlock) s_tab[i] = -1;
+      for (int i = threadIdx.x; i <= t.size; i += kBlock) s_tab[i] = -1;
       __syncthreads();
       for (int i = threadIdx.x; i < x.d; i += kBlock) table_insert(s_tab, t, olist[x.ob + i]);
       __syncthreads();
     }
     const int hi = min(lo + kPartItems, x.d + x.di);
-    cnt += task_pairs(orng, olist, ilist, x, lo, hi, wid * kWave, kWaves * kWave, hashed ? s_tab : nullptr, t);
+    if (hashed) cnt += task_pairs<true>(orng, olist, ilist, x, lo, hi, wid * kWave, kWaves * kWave, s_tab, t);
+    else cnt += task_pairs<false>(orng, olist, ilist, x, lo, hi, wid * kWave, kWaves * kWave, nullptr, t);
     __syncthreads();
   }
   cnt = wave_sum(cnt);
