@@ -69,7 +69,8 @@ static_assert(kHugeChunk % kTcBlock == 0, "tc_pack_huge moves whole items");
 constexpr int kWaveWork = 1 << 15;  // probe bound of a wave-pass task
 constexpr int kPartItems = 1024;    // items per workgroup-pass part
 constexpr int kUnroll = 4;          // windows of 64 per step over a long list
-constexpr int kLongList = 32;       // lists walked by the whole wave
+constexpr int kLongList = 64;       // lists walked by the whole wave (32 and 128: slower)
+constexpr int kWalk = 4;            // windows of 64 in flight per step of that walk (2 and 8: slower)
 constexpr int kTinyList = 16;       // wave-pass tasks this small are counted kTinyGroup at a time
 constexpr int kTinyGroup = 16;      // tables of kTinySlots (+ mirror) side by side in the wave's region
 constexpr int kTinySlots = 64;
@@ -403,13 +404,56 @@ struct TcTask {
   int ob, d, ib, di;
 };
 
+// Wave-collective: kN windows of 64 entries of row[0, ol) from `base`
+// (windows past base + 64*kN are not touched; within them, every lane loads -
+// past the end: the last entry again, a cache hit - so that the loads issue
+// back to back, and kNoEntry marks the lanes past the end).  Hashed: the first
+// pair of slots of every window at once, and the few lanes whose probe runs
+// on continue two slots at a time; w is never in the second slot when the
+// first is empty, since it was inserted before the first empty slot of its
+// probe sequence.  Hits are added to the wave's `wcnt` (hashed) or the lane's
+// `cnt` (binary search of nx[0, d)).
+template <bool kHashed, int kN>
+IG_DEV void walk_step(const int* row, int base, int ol, int lane, const int* tab, const TcTable& t, const int* nx,
+                      int d, unsigned& cnt, unsigned long long& wcnt)
+{
+  static_assert(kN >= 1 && kN <= kWalk, "window count");
+  int w[kN];
+#pragma unroll
+  for (int q = 0; q < kN; q++) {
+    const int k = base + q * kWave + lane;
+    const int v = load_at(row, min(k, ol - 1));
+    w[q] = k < ol ? v : kNoEntry;
+  }
+  if constexpr (kHashed) {
+    unsigned h[kN];
+    int s0[kN], s1[kN];
+#pragma unroll
+    for (int q = 0; q < kN; q++) {
+      h[q] = slot_of(w[q], t.shift);
+      s0[q] = tab[h[q]];
+      s1[q] = tab[h[q] + 1];
+    }
+#pragma unroll
+    for (int q = 0; q < kN; q++) {
+      bool hit = (s0[q] == w[q]) | (s1[q] == w[q]);
+      if (!hit & (s0[q] != -1) & (s1[q] != -1)) hit = table_find_from(tab, t, w[q], (h[q] + 2) & t.mask);
+      wcnt += static_cast<unsigned long long>(__popcll(__ballot(hit)));
+    }
+  } else {
+#pragma unroll
+    for (int q = 0; q < kN; q++)
+      if (w[q] >= 0) cnt += search(nx, d, w[q]) ? 1u : 0u;
+  }
+}
+
 // Wave-collective: over items[ub0 + 64k + lane] (k = 0, 1, ... stepping by
 // ustep), counts the w in N+(item) found in N+(x) - through `tab`, or by
 // binary search of the sorted nx[0, d) when tab is null - for the items
 // whose list has at most `limit` entries.  Lists of kLongList or more
-// entries (97% of the probes on R-MAT) are walked by the whole wave, one at a
-// time, kUnroll windows of 64 per step so that their loads overlap; shorter
-// ones share windows of 64 pairs through the in-wave balancer.
+// entries are walked by the whole wave, one at a time, kWalk windows of 64 per
+// step so that their loads overlap; shorter ones share windows of 64 pairs
+// through the in-wave balancer.
 template <bool kHashed>
 IG_DEV unsigned long long count_pairs(const int2* __restrict__ orng, const int* __restrict__ olist,
                                       const int* __restrict__ items, int n, int limit, int ub0, int ustep,
@@ -435,41 +479,18 @@ IG_DEV unsigned long long count_pairs(const int2* __restrict__ orng, const int* 
       todo &= todo - 1;
       const int ol = wave_bcast(len, o);
       const int* row = olist + wave_bcast(b, o);
-      for (int base = 0; base < ol; base += kUnroll * kWave) {
-        // every lane loads (past the end: the last entry again, a cache
-        // hit) so that the loads issue back to back; kNoEntry marks the
-        // lanes past the end
-        int w[kUnroll];
-#pragma unroll
-        for (int q = 0; q < kUnroll; q++) {
-          const int k = base + q * kWave + lane;
-          const int v = load_at(row, min(k, ol - 1));
-          w[q] = k < ol ? v : kNoEntry;
-        }
-        if constexpr (kHashed) {
-          // the first pair of slots of every window at once; the few lanes
-          // whose probe runs on continue two slots at a time.  w is never in
-          // the second slot when the first is empty: it was inserted before
-          // the first empty slot of its probe sequence.
-          unsigned h[kUnroll];
-          int s0[kUnroll], s1[kUnroll];
-#pragma unroll
-          for (int q = 0; q < kUnroll; q++) {
-            h[q] = slot_of(w[q], t.shift);
-            s0[q] = tab[h[q]];
-            s1[q] = tab[h[q] + 1];
-          }
-#pragma unroll
-          for (int q = 0; q < kUnroll; q++) {
-            bool hit = (s0[q] == w[q]) | (s1[q] == w[q]);
-            if (!hit & (s0[q] != -1) & (s1[q] != -1)) hit = table_find_from(tab, t, w[q], (h[q] + 2) & t.mask);
-            wcnt += static_cast<unsigned long long>(__popcll(__ballot(hit)));
-          }
-        } else {
-#pragma unroll
-          for (int q = 0; q < kUnroll; q++)
-            if (w[q] >= 0) cnt += search(nx, d, w[q]) ? 1u : 0u;
-        }
+      // steps of kWalk windows while more than kWalk - 1 windows are left
+      // (the last of them may be partial), then one step of just the 0 to
+      // kWalk - 1 windows left
+      static_assert(kWalk == 4, "the switch below covers 1 to kWalk - 1 windows");
+      int base = 0;
+      for (; ol - base > (kWalk - 1) * kWave; base += kWalk * kWave)
+        walk_step<kHashed, kWalk>(row, base, ol, lane, tab, t, nx, d, cnt, wcnt);
+      switch ((ol - base + kWave - 1) / kWave) {
+        case 1: walk_step<kHashed, 1>(row, base, ol, lane, tab, t, nx, d, cnt, wcnt); break;
+        case 2: walk_step<kHashed, 2>(row, base, ol, lane, tab, t, nx, d, cnt, wcnt); break;
+        case 3: walk_step<kHashed, 3>(row, base, ol, lane, tab, t, nx, d, cnt, wcnt); break;
+        default: break;
       }
     }
     const int slen = len < kLongList ? len : 0;
