@@ -72,11 +72,10 @@ constexpr int kUnroll = 4;          // windows of 64 per step over a long list
 constexpr int kLongList = 64;       // lists walked by the whole wave (32 and 128: slower)
 constexpr int kWalk = 4;            // windows of 64 in flight per step of that walk (2 and 8: slower)
 constexpr int kTinyList = 16;       // wave-pass tasks this small are counted kTinyGroup at a time
-constexpr int kTinyGroup = 16;      // tables of kTinySlots (+ mirror) side by side in the wave's region
+constexpr int kTinyGroup = 16;      // tables of kTinySlots side by side in the wave's region
 constexpr int kTinySlots = 64;
-constexpr int kWaveRegion = kWaveSlots + 16;  // a wave's LDS words
-static_assert(4 * kTinyList <= kTinySlots && kTinyGroup * (kTinySlots + 1) <= kWaveRegion &&
-                  kWaveSlots + 1 <= kWaveRegion && kTinyList < kLongList,
+constexpr int kWaveRegion = kWaveSlots;  // a wave's LDS words
+static_assert(4 * kTinyList <= kTinySlots && kTinyGroup * kTinySlots <= kWaveRegion && kTinyList < kLongList,
               "tiny-task tables: 64 slots each, all in the wave's region, no long lists");
 
 struct TcControl {
@@ -342,54 +341,68 @@ IG_DEV bool search(const int* list, int n, int key)
   return lo < n && list[lo] == key;
 }
 
-// Hash table geometry for a list of d entries: a power of two >= 8d slots
-// (load <= 1/8) when that fits in `cap` slots, else >= 4d (callers keep 4d
-// within cap), and at least 64.  The slot of w is the top log2(size) bits of
-// the low 32 bits of (w mod 2^24) * 0x9E3779 (multiplicative hashing with a
-// full-rate 24-bit multiply; ids above 2^24 share home slots, which costs
-// probes, never results).  Empty slots hold -1, and slot `size` repeats slot
-// 0, so that a probe reads two consecutive slots with one ds_read2 and never
-// wraps within a pair: a table takes size + 1 words.
+// Hash tables in LDS: buckets of 4 slots (16 bytes, read with one
+// ds_read_b128), a power of two >= 8d slots (load <= 1/8) when that fits in
+// `cap` slots, else >= 4d (callers keep 4d within cap), and at least 64.
+// The home bucket of w is the top log2(buckets) bits of the low 32 bits of
+// (w mod 2^24) * 0x9E3779 (multiplicative hashing with a full-rate 24-bit
+// multiply; ids above 2^24 share home buckets, which costs probes, never
+// results).  Empty slots hold -1.  An insertion takes the first empty slot of
+// its bucket, in slot order, and moves on to the next bucket (cyclically)
+// only when all four are taken, so the taken slots of a bucket are always a
+// prefix: a bucket whose last slot is empty ends every probe sequence through
+// it.  At load 1/8 a bucket is full with probability ~0.2%.
 struct TcTable {
-  int size;
-  unsigned mask;
-  int shift;
+  int size;       // slots
+  unsigned mask;  // buckets - 1
+  int shift;      // 32 - log2(buckets)
 };
 
 IG_DEV TcTable table_for(int d, int cap)
 {
   int size = max(64, 1 << (32 - __clz(8 * d - 1)));
   if (size > cap) size >>= 1;
-  return TcTable{size, static_cast<unsigned>(size - 1), __clz(size) + 1};
+  const int buckets = size / 4;
+  return TcTable{size, static_cast<unsigned>(buckets - 1), __clz(buckets) + 1};
 }
 
-IG_DEV unsigned slot_of(int w, int shift)
+IG_DEV unsigned bucket_of(int w, int shift)
 {
   return static_cast<unsigned>(__umul24(static_cast<unsigned>(w), 0x9E3779u)) >> shift;
 }
 
+IG_DEV int4 read_bucket(const int* tab, unsigned b) { return reinterpret_cast<const int4*>(tab)[b]; }
+
+// (xor-min form: compares of the four elements would be packed into bit
+// vectors by the compiler, at three times the instructions)
+IG_DEV bool in_bucket(const int4& s, int w)
+{
+  const unsigned u = static_cast<unsigned>(w);
+  return min(min(static_cast<unsigned>(s.x) ^ u, static_cast<unsigned>(s.y) ^ u),
+             min(static_cast<unsigned>(s.z) ^ u, static_cast<unsigned>(s.w) ^ u)) == 0u;
+}
+
 IG_DEV void table_insert(int* tab, const TcTable& t, int w)
 {
-  unsigned h = slot_of(w, t.shift);
-  while (atomicCAS(&tab[h], -1, w) != -1) h = (h + 1) & t.mask;
-  if (h == 0) tab[t.size] = w;  // the mirror of slot 0
-}
-
-// Linear probing two slots at a time from slot h: w was inserted before the
-// first empty slot of its probe sequence.
-IG_DEV bool table_find_from(const int* tab, const TcTable& t, int w, unsigned h)
-{
-  // w is never in the second slot of a pair whose first is empty
-  int a = tab[h], b = tab[h + 1];
-  while ((a != w) & (b != w) & (a != -1) & (b != -1)) {
-    h = (h + 2) & t.mask;
-    a = tab[h];
-    b = tab[h + 1];
+  for (unsigned b = bucket_of(w, t.shift);; b = (b + 1) & t.mask) {
+    int* s = tab + 4 * b;
+    for (int j = 0; j < 4; j++)
+      if (atomicCAS(&s[j], -1, w) == -1) return;
   }
-  return (a == w) | (b == w);
 }
 
-IG_DEV bool table_find(const int* tab, const TcTable& t, int w) { return table_find_from(tab, t, w, slot_of(w, t.shift)); }
+// Probe from bucket b on, one bucket per LDS read.
+IG_DEV bool table_find_from(const int* tab, const TcTable& t, int w, unsigned b)
+{
+  int4 s = read_bucket(tab, b);
+  while (!in_bucket(s, w) && s.w != -1) {
+    b = (b + 1) & t.mask;
+    s = read_bucket(tab, b);
+  }
+  return in_bucket(s, w);
+}
+
+IG_DEV bool table_find(const int* tab, const TcTable& t, int w) { return table_find_from(tab, t, w, bucket_of(w, t.shift)); }
 
 IG_DEV void wave_lds_sync()
 {
@@ -407,12 +420,10 @@ struct TcTask {
 // Wave-collective: kN windows of 64 entries of row[0, ol) from `base`
 // (windows past base + 64*kN are not touched; within them, every lane loads -
 // past the end: the last entry again, a cache hit - so that the loads issue
-// back to back, and kNoEntry marks the lanes past the end).  Hashed: the first
-// pair of slots of every window at once, and the few lanes whose probe runs
-// on continue two slots at a time; w is never in the second slot when the
-// first is empty, since it was inserted before the first empty slot of its
-// probe sequence.  Hits are added to the wave's `wcnt` (hashed) or the lane's
-// `cnt` (binary search of nx[0, d)).
+// back to back, and kNoEntry marks the lanes past the end).  Hashed: the home
+// buckets of every window at once, and the few lanes whose bucket is full
+// and holds no match go on through the next ones.  Hits are added to the
+// wave's `wcnt` (hashed) or the lane's `cnt` (binary search of nx[0, d)).
 template <bool kHashed, int kN>
 IG_DEV void walk_step(const int* row, int base, int ol, int lane, const int* tab, const TcTable& t, const int* nx,
                       int d, unsigned& cnt, unsigned long long& wcnt)
@@ -427,18 +438,23 @@ IG_DEV void walk_step(const int* row, int base, int ol, int lane, const int* tab
   }
   if constexpr (kHashed) {
     unsigned h[kN];
-    int s0[kN], s1[kN];
+    int4 sl[kN];
 #pragma unroll
     for (int q = 0; q < kN; q++) {
-      h[q] = slot_of(w[q], t.shift);
-      s0[q] = tab[h[q]];
-      s1[q] = tab[h[q] + 1];
+      h[q] = bucket_of(w[q], t.shift);
+      sl[q] = read_bucket(tab, h[q]);
     }
+    // hit masks straight from one compare per slot, on the scalar unit
 #pragma unroll
     for (int q = 0; q < kN; q++) {
-      bool hit = (s0[q] == w[q]) | (s1[q] == w[q]);
-      if (!hit & (s0[q] != -1) & (s1[q] != -1)) hit = table_find_from(tab, t, w[q], (h[q] + 2) & t.mask);
-      wcnt += static_cast<unsigned long long>(__popcll(__ballot(hit)));
+      unsigned long long hm = __ballot(sl[q].x == w[q]) | __ballot(sl[q].y == w[q]) |
+                              __ballot(sl[q].z == w[q]) | __ballot(sl[q].w == w[q]);
+      const unsigned long long go = __ballot(sl[q].w != -1) & ~hm;  // full bucket, no match
+      if (go != 0) {
+        const bool mine = ((go >> lane) & 1ull) != 0;
+        hm |= __ballot(mine && table_find_from(tab, t, w[q], (h[q] + 1) & t.mask));
+      }
+      wcnt += static_cast<unsigned long long>(__popcll(hm));
     }
   } else {
 #pragma unroll
@@ -555,7 +571,7 @@ IG_DEV unsigned long long tiny_tasks(unsigned long long mask, const int2 rl, int
   const int td = has ? td0 : 0;
   const int tn = has ? td0 + ti0 : 0;  // items: N+(x), then N-(x)
   const TcTable t = table_for(kTinyList, kTinySlots);  // 64 slots for every task
-  const int stride = t.size + 1;                      // and the mirror slot
+  const int stride = t.size;
   for (int i = lane; i < ntask * stride; i += kWave) tab[i] = -1;
   wave_lds_sync();
   {
@@ -620,7 +636,7 @@ __global__ __launch_bounds__(kTcBlock) void tc_count(int nodes, const int2* __re
                                                      const int* __restrict__ ilist, unsigned long long* mid,
                                                      unsigned long long* big, TcControl* ctl)
 {
-  __shared__ int s_tab[kTcBlock / kWave][kWaveRegion];
+  __shared__ __align__(16) int s_tab[kTcBlock / kWave][kWaveRegion];
   const int lane = lane_id();
   int* tab = s_tab[threadIdx.x / kWave];
   unsigned long long cnt = 0;
@@ -660,7 +676,7 @@ __global__ __launch_bounds__(kTcBlock) void tc_count(int nodes, const int2* __re
       const int ob = wave_shfl(rl.x, j);  // in-lists share the out-lists' positions
       const TcTask x{ob, wave_shfl(dl, j), ob, wave_shfl(il, j)};
       const TcTable t = table_for(x.d, kWaveSlots);
-      for (int i = lane; i <= t.size; i += kWave) tab[i] = -1;
+      for (int i = lane; i < t.size; i += kWave) tab[i] = -1;
       wave_lds_sync();
       for (int i = lane; i < x.d; i += kWave) table_insert(tab, t, olist[x.ob + i]);
       wave_lds_sync();
@@ -683,7 +699,7 @@ __global__ __launch_bounds__(kBlock) void tc_count_group(const int2* __restrict_
                                                          const unsigned* nparts, unsigned* next, int lds_list,
                                                          TcControl* ctl)
 {
-  __shared__ int s_tab[kSlots + 1];  // a table and its mirror slot
+  __shared__ __align__(16) int s_tab[kSlots];
   __shared__ unsigned s_k;
   constexpr int kWaves = kBlock / kWave;
   const int wid = threadIdx.x / kWave;
@@ -702,7 +718,7 @@ __global__ __launch_bounds__(kBlock) void tc_count_group(const int2* __restrict_
     const bool hashed = x.d <= lds_list;
     const TcTable t = table_for(hashed ? x.d : 1, kSlots);
     if (hashed) {
-      for (int i = threadIdx.x; i <= t.size; i += kBlock) s_tab[i] = -1;
+      for (int i = threadIdx.x; i < t.size; i += kBlock) s_tab[i] = -1;
       __syncthreads();
       for (int i = threadIdx.x; i < x.d; i += kBlock) table_insert(s_tab, t, olist[x.ob + i]);
       __syncthreads();
