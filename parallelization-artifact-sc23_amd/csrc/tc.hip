@@ -344,10 +344,11 @@ IG_DEV bool search(const int* list, int n, int key)
 // Hash tables in LDS: buckets of 4 slots (16 bytes, read with one
 // ds_read_b128), a power of two >= 8d slots (load <= 1/8) when that fits in
 // `cap` slots, else >= 4d (callers keep 4d within cap), and at least 64.
-// The home bucket of w is the top log2(buckets) bits of the low 32 bits of
-// (w mod 2^24) * 0x9E3779 (multiplicative hashing with a full-rate 24-bit
-// multiply; ids above 2^24 share home buckets, which costs probes, never
-// results).  Empty slots hold -1.  An insertion takes the first empty slot of
+// The home bucket of w is bits [24 - log2(buckets), 24) of (w mod 2^24) *
+// 0x9E3779: Fibonacci hashing in a 24-bit word (0x9E3779 / 2^24 ~ 0.618, so
+// consecutive ids land ~0.618 of the table apart) with a full-rate 24-bit
+// multiply.  Ids above 2^24 share home buckets, which costs probes, never
+// results.  Empty slots hold -1.  An insertion takes the first empty slot of
 // its bucket, in slot order, and moves on to the next bucket (cyclically)
 // only when all four are taken, so the taken slots of a bucket are always a
 // prefix: a bucket whose last slot is empty ends every probe sequence through
@@ -355,7 +356,8 @@ IG_DEV bool search(const int* list, int n, int key)
 struct TcTable {
   int size;       // slots
   unsigned mask;  // buckets - 1
-  int shift;      // 32 - log2(buckets)
+  int shift;      // 24 - log2(buckets)
+  int bits;       // log2(buckets)
 };
 
 IG_DEV TcTable table_for(int d, int cap)
@@ -363,12 +365,14 @@ IG_DEV TcTable table_for(int d, int cap)
   int size = max(64, 1 << (32 - __clz(8 * d - 1)));
   if (size > cap) size >>= 1;
   const int buckets = size / 4;
-  return TcTable{size, static_cast<unsigned>(buckets - 1), __clz(buckets) + 1};
+  return TcTable{size, static_cast<unsigned>(buckets - 1), __clz(buckets) - 7, 31 - __clz(buckets)};
 }
 
-IG_DEV unsigned bucket_of(int w, int shift)
+IG_DEV unsigned bucket_of(int w, const TcTable& t)
 {
-  return static_cast<unsigned>(__umul24(static_cast<unsigned>(w), 0x9E3779u)) >> shift;
+  // one v_bfe_u32 takes the bits
+  return __builtin_amdgcn_ubfe(static_cast<unsigned>(__umul24(static_cast<unsigned>(w), 0x9E3779u)),
+                               static_cast<unsigned>(t.shift), static_cast<unsigned>(t.bits));
 }
 
 IG_DEV int4 read_bucket(const int* tab, unsigned b) { return reinterpret_cast<const int4*>(tab)[b]; }
@@ -384,7 +388,7 @@ IG_DEV bool in_bucket(const int4& s, int w)
 
 IG_DEV void table_insert(int* tab, const TcTable& t, int w)
 {
-  for (unsigned b = bucket_of(w, t.shift);; b = (b + 1) & t.mask) {
+  for (unsigned b = bucket_of(w, t);; b = (b + 1) & t.mask) {
     int* s = tab + 4 * b;
     for (int j = 0; j < 4; j++)
       if (atomicCAS(&s[j], -1, w) == -1) return;
@@ -402,7 +406,7 @@ IG_DEV bool table_find_from(const int* tab, const TcTable& t, int w, unsigned b)
   return in_bucket(s, w);
 }
 
-IG_DEV bool table_find(const int* tab, const TcTable& t, int w) { return table_find_from(tab, t, w, bucket_of(w, t.shift)); }
+IG_DEV bool table_find(const int* tab, const TcTable& t, int w) { return table_find_from(tab, t, w, bucket_of(w, t)); }
 
 IG_DEV void wave_lds_sync()
 {
@@ -441,7 +445,7 @@ IG_DEV void walk_step(const int* row, int base, int ol, int lane, const int* tab
     int4 sl[kN];
 #pragma unroll
     for (int q = 0; q < kN; q++) {
-      h[q] = bucket_of(w[q], t.shift);
+      h[q] = bucket_of(w[q], t);
       sl[q] = read_bucket(tab, h[q]);
     }
     // hit masks straight from one compare per slot, on the scalar unit
