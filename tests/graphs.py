@@ -16,6 +16,22 @@ def clique(ig, n):
     return ig.from_edges(n, s, d, name=f"K{n}")
 
 
+def cliques(ig, sizes, seed=None):
+    """Disjoint cliques of the given sizes on consecutive ids, or on ids
+    shuffled by a seeded permutation."""
+    src, dst, base = [], [], 0
+    for k in sizes:
+        s, d = np.triu_indices(k, 1)
+        src.append(s + base)
+        dst.append(d + base)
+        base += k
+    s, d = np.concatenate(src), np.concatenate(dst)
+    if seed is not None:
+        perm = np.random.default_rng(seed).permutation(base)
+        s, d = perm[s], perm[d]
+    return ig.from_edges(base, s.astype(np.int32), d.astype(np.int32), name=f"cliques{len(sizes)}")
+
+
 def disjoint(ig):
     # two triangles, a path of 3, and isolated vertices 9..11
     s = np.array([0, 1, 2, 3, 4, 5, 6, 7], np.int32)

@@ -4,7 +4,7 @@ and L1 <= 1e-4 against the FP64 oracle run for the same iteration count."""
 import numpy as np
 import pytest
 
-from graphs import catalog, clique, weighted
+from graphs import catalog, clique, cliques, weighted
 
 pytestmark = pytest.mark.gpu
 
@@ -102,6 +102,21 @@ def test_tc_catalog(ig, orc, ctx):
     for g in catalog(ig):
         assert ctx.tc(ig.DeviceGraph(g)) == orc.tc(g), g.name
     assert ctx.tc(ig.DeviceGraph(clique(ig, 40))) == 40 * 39 * 38 // 6
+
+
+def test_tc_clique_unions(ig, ctx):
+    """Closed-form counts (the sum of C(k, 3)) on disjoint cliques, on
+    consecutive and on shuffled ids.  In a clique the out-list of the j-th
+    smallest id holds the k - 1 - j larger ones, so: K3..K17 give every task
+    of up to 16 entries (counted 16 tasks at a time); K33..K200 wave-pass
+    tasks whose probed lists reach 64 entries and more (the whole-wave walk,
+    ending on 1 to 4 windows); K300 and K2049 workgroup-pass tasks, the
+    latter's largest table at load 1/4 (2048 entries in 8192 slots)."""
+    sizes = [3, 4, 5, 9, 16, 17] * 12 + [33, 64, 65, 66, 129, 200, 300, 2049]
+    want = sum(k * (k - 1) * (k - 2) // 6 for k in sizes)
+    for seed in (None, 4):
+        g = cliques(ig, sizes, seed=seed)
+        assert ctx.tc(ig.DeviceGraph(g)) == want, seed
 
 
 def test_tc_rmat16(ig, orc, ctx):
