@@ -627,7 +627,8 @@ def main() -> int:
     for a in ALGS:
         b = kernel_bytes(a, g, pr_iters, tcm)
         gbs = b / (kmed[a] * 1e-3) / 1e9
-        # the survey's per-algorithm model over the whole call (all launches)
+        # the one-pass model per algorithm over the whole call (all launches); TC: the
+        # merge model of the reference's vertex form (tc_merge_bytes)
         b_call = tc_merge_bytes(g) if a == "tc" else b
         roof[a] = {"kernel": kname[a], "kernel_ms": round(kmed[a], 4), "bytes": b, "achieved_gbs": round(gbs, 1),
                    "frac": round(gbs / HBM_PEAK_GBS, 4), "call_ms": round(med[a], 4), "call_model_bytes": b_call,

@@ -1,5 +1,5 @@
 // Shared helpers of the wrapper translation units that compile the
-// reference's own C++ sources (from /root/reference, never copied) into
+// reference's own C++ sources (compiled where they lie, never copied) into
 // oracle/_ref/libref_<alg>.so.  Test infrastructure only.
 #pragma once
 #include <cstdio>

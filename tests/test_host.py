@@ -102,7 +102,7 @@ def test_golden_egr_readable_by_eclgraph_layout(ig):
 def test_context_fails_cleanly_without_gpu(ig):
     import subprocess
     import sys
-    # in a child process, so a GPU box still runs the rest of the suite normally
+    # in a child process, so a machine with a GPU still runs the rest of the suite normally
     code = ("import sys; sys.path.insert(0, %r); import indigo2 as ig, ctypes\n"
             "h = ctypes.c_void_p(); rc = ig.load().indigo_ctx_create(0, ctypes.byref(h))\n"
             "print(rc)") % os.path.join(REPO, "parallelization-artifact-sc23_amd")
