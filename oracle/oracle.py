@@ -135,6 +135,9 @@ def _ref_lib(alg: str):
         elif alg == "mis":
             lib.ref_mis_threads.argtypes = [c_int, c_int, _PI, _PI, POINTER(c_uint), _PI, c_int]
             lib.ref_mis_threads.restype = c_double
+        elif alg == "mis_omp":
+            lib.ref_mis_omp.argtypes = [c_int, c_int, _PI, _PI, POINTER(c_uint), _PI, c_int]
+            lib.ref_mis_omp.restype = c_double
         elif alg == "pr":
             lib.ref_pr_threads.argtypes = [c_int, c_int, _PI, _PI, POINTER(c_float), _PI, c_int]
             lib.ref_pr_threads.restype = c_double
@@ -182,6 +185,16 @@ def ref_mis(g, threads: int = 1):
     st = np.empty(g.nodes, np.int32)
     pr = np.empty(g.nodes, np.uint32)
     rt = _ref_lib("mis").ref_mis_threads(g.nodes, g.edges, _p(ni), _p(nl), pr.ctypes.data_as(POINTER(c_uint)),
+                                         _p(st), threads)
+    return st, pr, rt
+
+
+def ref_mis_omp(g, threads: int = 1):
+    """The reference's best published OpenMP MIS variant (oracle/_ref)."""
+    ni, nl = _csr(g)
+    st = np.empty(g.nodes, np.int32)
+    pr = np.empty(g.nodes, np.uint32)
+    rt = _ref_lib("mis_omp").ref_mis_omp(g.nodes, g.edges, _p(ni), _p(nl), pr.ctypes.data_as(POINTER(c_uint)),
                                          _p(st), threads)
     return st, pr, rt
 

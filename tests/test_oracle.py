@@ -71,6 +71,9 @@ def test_oracle_matches_compiled_reference(ig, orc, seed):
         st, pri, _ = orc.ref_mis(g, 4)
         assert np.array_equal(orc.mis(g), st), g.name
         assert pri[3] == orc.mis_hash(3 + 712313887)
+        if orc.ref_available("mis_omp"):
+            st, pri, _ = orc.ref_mis_omp(g, 4)
+            assert np.array_equal(orc.mis(g), st), g.name
         sc, it, _ = orc.ref_pr(g, 4)
         mine, it2 = orc.pr_ref32(g)
         assert it == it2 and np.array_equal(sc, mine), g.name
