@@ -26,7 +26,16 @@
 //     below the old split, already expanded at that distance) are dropped,
 //     the rest stay parked;
 //   * vertices are expanded with their current distance (re-read at
-//     expansion), so stale duplicates only cost work, never correctness.
+//     expansion), so stale duplicates only cost work, never correctness;
+//   * while the near frontier is small (only short lists, degree sum <=
+//     kSsspSmallCap), workgroup 0 runs the near iterations alone with the
+//     frontier in LDS and workgroup barriers only (small_near), parking
+//     entries at or above the split in the far pile as the grid does, and
+//     also the splits while the far pile has at most kSsspSmallCap entries;
+//     it hands the frontier back when it outgrows the LDS, or when the
+//     frontier empties with a larger (or no) far pile.  Deep sparse graphs spend thousands of iterations on
+//     frontiers of tens of vertices, where the grid barrier was most of an
+//     iteration's cost.
 // delta defaults to a sampled mean weight x 32 / mean degree (at least 1).
 #include "sssp.hpp"
 
@@ -38,9 +47,11 @@ namespace indigo {
 namespace {
 
 constexpr int kSsspBlock = 512;
+constexpr int kSsspSmallCap = 2048;  // list entries of a one-workgroup iteration
 
 struct SsspControl {
   GridBarrier bar;
+  PaddedU32 small_ack;  // workgroups that have read the counters before a one-workgroup phase
   FrontierCounters cnt[3];
   PaddedU32 far_cnt[3];
   PaddedU32 far_min[3];
@@ -48,7 +59,12 @@ struct SsspControl {
   int epochs;
   int status;
   int delta;
-  int pad[28];
+  int small_iters;  // iterations (near and split) run by workgroup 0 alone
+  int level_base;   // counter rotation index at which the grid resumes after a one-workgroup phase
+  int it_base;      // iteration (stamp) count at which it resumes
+  int split_base;   // split T at which it resumes
+  int epoch_base;   // far-pile epoch at which it resumes
+  int pad[23];
   LevelTrace trace;
 };
 
@@ -133,9 +149,254 @@ IG_DEV void relax(const SsspArgs& a, int dv, int e, bool active, int T, int it_s
   se_park(S, park, u, nd, far, far_cnt);
 }
 
-__global__ __launch_bounds__(kSsspBlock) void sssp_kernel(SsspArgs a)
+// LDS of the one-workgroup phase (shares the kernel's LDS with the emitter,
+// which is idle meanwhile).
+struct SsspSmallLds {
+  int q[kSsspSmallCap];      // near frontier of the current iteration, refilled with the next one
+  int incl[kSsspSmallCap];   // inclusive prefix of the frontier's degrees
+  int shift[kSsspSmallCap];  // nindex[v] - exclusive prefix: entry k of slot j is nlist[shift[j] + k]
+  int dv[kSsspSmallCap];     // distance of slot j's vertex at expansion
+  int wsum[kSsspBlock / kWave];
+  int cnt;
+  int abort_flag;
+  unsigned far_min;
+};
+
+// Workgroup 0 runs near iterations alone while the frontier's degree sum
+// fits the LDS arrays.  Entered at rotation index L with the near frontier in
+// the small global queue (n entries); each iteration relaxes every entry as
+// relax() does (distance check, atomicMin, then the stamp for an improvement
+// below T, which joins the next frontier in LDS, or the far-pile stamp for
+// one at or above T, parked in the current epoch's global pile).  When the
+// frontier empties and the far pile holds at most kSsspSmallCap entries, it
+// also splits as the grid does: T moves to max(T, far minimum) + delta, the
+// pile is re-filed (below the new T into the frontier, still above it into
+// the next epoch's pile, settled entries dropped) and the piles rotate.  On
+// leaving (frontier too large, or empty with a larger or no far pile) it
+// re-emits the frontier into the global queues and counters of the rotation
+// index reached, cleared first, once every other workgroup has acknowledged
+// reading the entry counters (small_ack, as in the BFS kernel's
+// one-workgroup phase).  L and it advance by one per iteration and split, as
+// in the grid; T and epoch are updated in place.
+IG_DEV void small_near(const SsspArgs& a, int& L, int& it, unsigned n, int& T, int& epoch, int delta,
+                       SsspControl* ctl, long long deadline, bool* aborted, SsspSmallLds& S)
 {
-  __shared__ SsspEmit s_emit;
+  constexpr int kPer = kSsspSmallCap / kSsspBlock;  // frontier slots per thread
+  const int lane = lane_id();
+  const int wid = threadIdx.x / kWave;
+  int* far = a.far[epoch % 3];
+  unsigned* far_cnt = &ctl->far_cnt[epoch % 3].v;
+  const int* gin = a.fq[L & 1].q;
+  for (unsigned i = threadIdx.x; i < n; i += blockDim.x) S.q[i] = ld_relaxed(&gin[i]);
+  if (threadIdx.x == 0) {
+    S.cnt = static_cast<int>(n);
+    S.abort_flag = 0;
+    S.far_min = static_cast<unsigned>(kInf);
+  }
+  __syncthreads();
+  *aborted = false;
+  int cn = S.cnt;
+  for (;;) {
+    if ((it & 63) == 0 && threadIdx.x == 0) S.abort_flag = kernel_aborted(&ctl->bar, deadline) ? 1 : 0;
+    if (cn == 0) {
+      // near frontier empty: split here while the far pile is small
+      const unsigned nf = ld_relaxed(far_cnt);
+      if (nf == 0 || nf > static_cast<unsigned>(kSsspSmallCap)) break;
+      const unsigned fm = min(ld_relaxed(&ctl->far_min[epoch % 3].v), S.far_min);
+      const int fmin = static_cast<int>(fm);
+      trace_level(&ctl->trace, L, nf, 0ull, 3);
+      __syncthreads();  // every thread has read S.far_min and the abort flag
+      if (S.abort_flag) {
+        *aborted = true;
+        return;
+      }
+      const int T_old = T;
+      const long long Tn = static_cast<long long>(T > fmin ? T : fmin) + delta;
+      T = Tn > kInf ? kInf : static_cast<int>(Tn);
+      it++;
+      const int nepoch = epoch + 1;
+      int* nfar = a.far[nepoch % 3];
+      unsigned* nfar_cnt = &ctl->far_cnt[nepoch % 3].v;
+      if (threadIdx.x == 0) {
+        // pile (epoch + 2) % 3 was consumed by the previous split
+        st_relaxed(&ctl->far_cnt[(epoch + 2) % 3].v, 0u);
+        st_relaxed(&ctl->far_min[(epoch + 2) % 3].v, static_cast<unsigned>(kInf));
+        S.far_min = static_cast<unsigned>(kInf);
+        S.cnt = 0;
+        ctl->epochs++;
+      }
+      __syncthreads();
+      for (unsigned base = 0; base < nf; base += blockDim.x) {
+        const unsigned i = base + threadIdx.x;
+        bool to_near = false, keep = false;
+        int u = 0, du = 0;
+        if (i < nf) {
+          u = ld_relaxed(&far[i]);
+          du = ld_relaxed(&a.dist[u]);
+          if (du >= T_old) {
+            if (du < T)
+              to_near = __hip_atomic_exchange(&a.stamp[u], it, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != it;
+            else
+              keep = __hip_atomic_exchange(&a.farstamp[u], nepoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) !=
+                     nepoch;
+          }
+        }
+        const unsigned long long m = __ballot(to_near);
+        if (m != 0) {
+          const int leader = __ffsll(static_cast<long long>(m)) - 1;
+          int b = 0;
+          if (lane == leader) b = atomicAdd(&S.cnt, __popcll(m));
+          b = wave_shfl(b, leader);
+          // at most nf <= kSsspSmallCap entries in all
+          if (to_near) S.q[b + lanes_below(m)] = u;
+        }
+        wave_append(keep, u, nfar, nfar_cnt);
+        const int km = wave_min(keep ? du : kInf);
+        if (lane == 0 && km != kInf) atomicMin(&S.far_min, static_cast<unsigned>(km));
+      }
+      epoch = nepoch;
+      far = nfar;
+      far_cnt = nfar_cnt;
+      __syncthreads();
+      cn = S.cnt;
+      L++;
+      __syncthreads();
+      continue;
+    }
+    // round 1: index pairs and distances of this thread's run of slots
+    const int i0 = min(cn, static_cast<int>(threadIdx.x) * kPer);
+    const int i1 = min(cn, i0 + kPer);
+    int beg[kPer], deg[kPer], dvs[kPer];
+    int mine = 0;
+#pragma unroll
+    for (int j = 0; j < kPer; j++) {
+      const bool ok = i0 + j < i1;
+      const int v = ok ? S.q[i0 + j] : 0;
+      beg[j] = ok ? a.nindex[v] : 0;
+      deg[j] = ok ? a.nindex[v + 1] - beg[j] : 0;
+      dvs[j] = ok ? ld_relaxed(&a.dist[v]) : 0;
+      mine += deg[j];
+    }
+    const int winc = wave_inclusive_scan(mine);
+    if (lane == kWave - 1) S.wsum[wid] = winc;
+    __syncthreads();
+    if (S.abort_flag) {
+      *aborted = true;
+      return;
+    }
+    int before = 0, total = 0;
+    for (int w = 0; w < kSsspBlock / kWave; w++) {
+      before += w < wid ? S.wsum[w] : 0;
+      total += S.wsum[w];
+    }
+    if (total > kSsspSmallCap) break;  // S.q still holds this iteration's frontier
+    trace_level(&ctl->trace, L, static_cast<unsigned>(cn), static_cast<unsigned long long>(total), 2);
+    int run = before + winc - mine;
+#pragma unroll
+    for (int j = 0; j < kPer; j++) {
+      if (i0 + j < i1) {
+        S.incl[i0 + j] = run + deg[j];
+        S.shift[i0 + j] = beg[j] - run;
+        S.dv[i0 + j] = dvs[j];
+        run += deg[j];
+      }
+    }
+    if (threadIdx.x == 0) S.cnt = 0;
+    __syncthreads();
+    it++;
+    // round 2: the list entries, spread evenly over the workgroup
+    int u[kPer], nd[kPer];
+    bool act[kPer];
+#pragma unroll
+    for (int j = 0; j < kPer; j++) {
+      const int k = j * kSsspBlock + static_cast<int>(threadIdx.x);
+      act[j] = k < total;
+      u[j] = 0;
+      nd[j] = kInf;
+      if (act[j]) {
+        int lo = 0, hi = cn - 1;  // owner: the first slot with incl > k
+        while (lo < hi) {
+          const int mid = (lo + hi) >> 1;
+          if (S.incl[mid] > k) hi = mid;
+          else lo = mid + 1;
+        }
+        const int e = S.shift[lo] + k;
+        u[j] = a.nlist[e];
+        nd[j] = S.dv[lo] + a.eweight[e];
+      }
+    }
+    // round 3: relaxations (device-scope reads, as relax())
+    bool imp[kPer];
+#pragma unroll
+    for (int j = 0; j < kPer; j++) imp[j] = act[j] && nd[j] < ld_relaxed(&a.dist[u[j]]);
+#pragma unroll
+    for (int j = 0; j < kPer; j++) imp[j] = imp[j] && nd[j] < atomic_min_relaxed(&a.dist[u[j]], nd[j]);
+    bool near[kPer], park[kPer];
+#pragma unroll
+    for (int j = 0; j < kPer; j++) {
+      near[j] = imp[j] && nd[j] < T &&
+                __hip_atomic_exchange(&a.stamp[u[j]], it, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != it;
+      park[j] = imp[j] && nd[j] >= T &&
+                __hip_atomic_exchange(&a.farstamp[u[j]], epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != epoch;
+    }
+#pragma unroll
+    for (int j = 0; j < kPer; j++) {
+      const unsigned long long m = __ballot(near[j]);
+      if (m != 0) {
+        const int leader = __ffsll(static_cast<long long>(m)) - 1;
+        int b = 0;
+        if (lane == leader) b = atomicAdd(&S.cnt, __popcll(m));
+        b = wave_shfl(b, leader);
+        // stamps admit each vertex once per iteration: at most total <= kSsspSmallCap
+        if (near[j]) S.q[b + lanes_below(m)] = u[j];
+      }
+      wave_append(park[j], u[j], far, far_cnt);
+      const int pm = wave_min(park[j] ? nd[j] : kInf);
+      if (lane == 0 && pm != kInf) atomicMin(&S.far_min, static_cast<unsigned>(pm));
+    }
+    __syncthreads();
+    cn = S.cnt;
+    L++;
+    __syncthreads();
+  }
+  // hand the frontier back to the grid, once all others have read the entry
+  // iteration's counters
+  if (threadIdx.x == 0) {
+    if (S.far_min != static_cast<unsigned>(kInf)) atomic_min_relaxed(&ctl->far_min[epoch % 3].v, S.far_min);
+    const unsigned others = gridDim.x - 1u;
+    while (__hip_atomic_load(&ctl->small_ack.v, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < others) {
+      if (kernel_aborted(&ctl->bar, deadline)) {
+        S.abort_flag = 1;
+        break;
+      }
+      __builtin_amdgcn_s_sleep(1);
+    }
+    __hip_atomic_store(&ctl->small_ack.v, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (!S.abort_flag)
+      for (int k = 0; k < 3; k++) counters_clear(&ctl->cnt[k]);
+  }
+  __syncthreads();
+  if (S.abort_flag) {
+    *aborted = true;
+    return;
+  }
+  for (int base = 0; base < cn; base += blockDim.x) {
+    const int i = base + static_cast<int>(threadIdx.x);
+    const int v = i < cn ? S.q[i] : 0;
+    const int d = i < cn ? degree_of(a.nindex, v) : 0;
+    frontier_emit(d > 0, v, d, a.fq[L & 1], &ctl->cnt[L % 3]);
+  }
+}
+
+// 6 waves per SIMD: three 512-thread workgroups per CU, the persistent grid's
+// residency (at most 80 VGPRs)
+__global__ __launch_bounds__(kSsspBlock) __attribute__((amdgpu_waves_per_eu(6, 8))) void sssp_kernel(SsspArgs a)
+{
+  __shared__ union {
+    SsspEmit emit;
+    SsspSmallLds small;
+  } s_lds;
+  SsspEmit& s_emit = s_lds.emit;
   SsspControl* ctl = a.ctl;
   if (!residency_barrier(&ctl->bar, a.start_ticks)) return;
   const long long deadline = now_ticks() + a.budget;
@@ -215,8 +476,42 @@ __global__ __launch_bounds__(kSsspBlock) void sssp_kernel(SsspArgs a)
     unsigned* far_cnt = &ctl->far_cnt[epoch % 3].v;
     unsigned* far_min = &ctl->far_min[epoch % 3].v;
     const bool split = static_cast<unsigned long long>(n) + nm + nh == 0;
-    trace_level(&ctl->trace, L, split ? ld_relaxed(far_cnt) : n + nm + nh, split ? 0ull : ld_relaxed(&ic->m),
-                split ? 3 : 0);
+    const unsigned long long m = ld_relaxed(&ic->m);
+    const bool small = !split && nm == 0 && nh == 0 && m <= static_cast<unsigned long long>(kSsspSmallCap);
+    trace_level(&ctl->trace, L, split ? ld_relaxed(far_cnt) : n + nm + nh, split ? 0ull : m,
+                split ? 3 : (small ? 2 : 0));
+    if (small) {
+      // one-workgroup phase; the other workgroups acknowledge that they have
+      // read this iteration's counters and wait at the next barrier
+      if (blockIdx.x != 0 && threadIdx.x == 0)
+        __hip_atomic_fetch_add(&ctl->small_ack.v, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+      if (blockIdx.x == 0) {
+        bool aborted = false;
+        int L2 = L, it2 = it, T2 = T, epoch2 = epoch;
+        small_near(a, L2, it2, n, T2, epoch2, delta, ctl, deadline, &aborted, s_lds.small);
+        if (threadIdx.x == 0) {
+          st_relaxed(&ctl->level_base, L2);
+          st_relaxed(&ctl->it_base, it2);
+          st_relaxed(&ctl->split_base, T2);
+          st_relaxed(&ctl->epoch_base, epoch2);
+          ctl->small_iters += L2 - L;
+        }
+      }
+      if (!grid_barrier<true>(&ctl->bar, deadline)) {
+        if (tid == 0) ctl->status = 1;
+        return;
+      }
+      const int L2 = ld_relaxed(&ctl->level_base);
+      if (L2 == L) {  // cannot happen: the phase always runs at least one iteration
+        if (tid == 0) ctl->status = 2;
+        return;
+      }
+      L = L2;
+      it = ld_relaxed(&ctl->it_base);
+      T = ld_relaxed(&ctl->split_base);
+      epoch = ld_relaxed(&ctl->epoch_base);
+      continue;
+    }
     if (split) {
       // near frontier empty: move the split and re-file the far pile
       const unsigned nf = ld_relaxed(far_cnt);
