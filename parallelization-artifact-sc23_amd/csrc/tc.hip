@@ -421,25 +421,15 @@ struct TcTask {
   int ob, d, ib, di;
 };
 
-// Wave-collective: kN windows of 64 entries of row[0, ol) from `base`
-// (windows past base + 64*kN are not touched; within them, every lane loads -
-// past the end: the last entry again, a cache hit - so that the loads issue
-// back to back, and kNoEntry marks the lanes past the end).  Hashed: the home
-// buckets of every window at once, and the few lanes whose bucket is full
-// and holds no match go on through the next ones.  Hits are added to the
-// wave's `wcnt` (hashed) or the lane's `cnt` (binary search of nx[0, d)).
+// Wave-collective: counts the entries w[0..kN) (kNoEntry: none) found in
+// N+(x).  Hashed: the home buckets of every window at once, and the few lanes
+// whose bucket is full and holds no match go on through the next ones; hits
+// are added to the wave's `wcnt`.  Otherwise binary search of nx[0, d), hits
+// added to the lane's `cnt`.
 template <bool kHashed, int kN>
-IG_DEV void walk_step(const int* row, int base, int ol, int lane, const int* tab, const TcTable& t, const int* nx,
-                      int d, unsigned& cnt, unsigned long long& wcnt)
+IG_DEV void probe_windows(const int* w, int lane, const int* tab, const TcTable& t, const int* nx, int d,
+                          unsigned& cnt, unsigned long long& wcnt)
 {
-  static_assert(kN >= 1 && kN <= kWalk, "window count");
-  int w[kN];
-#pragma unroll
-  for (int q = 0; q < kN; q++) {
-    const int k = base + q * kWave + lane;
-    const int v = load_at(row, min(k, ol - 1));
-    w[q] = k < ol ? v : kNoEntry;
-  }
   if constexpr (kHashed) {
     unsigned h[kN];
     int4 sl[kN];
@@ -467,13 +457,61 @@ IG_DEV void walk_step(const int* row, int base, int ol, int lane, const int* tab
   }
 }
 
+// Wave-collective: kN windows of 64 entries of row[0, ol) from `base`
+// (windows past base + 64*kN are not touched; within them, every lane loads -
+// past the end: the last entry again, a cache hit - so that the loads issue
+// back to back, and kNoEntry marks the lanes past the end), then counted.
+template <bool kHashed, int kN>
+IG_DEV void walk_step(const int* row, int base, int ol, int lane, const int* tab, const TcTable& t, const int* nx,
+                      int d, unsigned& cnt, unsigned long long& wcnt)
+{
+  static_assert(kN >= 1 && kN <= kWalk, "window count");
+  int w[kN];
+#pragma unroll
+  for (int q = 0; q < kN; q++) {
+    const int k = base + q * kWave + lane;
+    const int v = load_at(row, min(k, ol - 1));
+    w[q] = k < ol ? v : kNoEntry;
+  }
+  probe_windows<kHashed, kN>(w, lane, tab, t, nx, d, cnt, wcnt);
+}
+
+// Wave-collective: kN windows of 64 pairs of the in-wave balancer from
+// `base` (< total): pair k of the lanes' concatenated short lists is
+// olist[shift[o] + k] for its owner lane o (incl: the inclusive prefix of the
+// list lengths).  Owners first, then every lane loads (past the end: window
+// 0's first pair again) so that the loads issue back to back, then counted.
+template <bool kHashed, int kN>
+IG_DEV void balanced_step(const int* __restrict__ olist, int incl, int shift, int total, int base, int lane,
+                          const int* tab, const TcTable& t, const int* nx, int d, unsigned& cnt,
+                          unsigned long long& wcnt)
+{
+  static_assert(kN >= 1 && kN <= kWalk, "window count");
+  int at[kN];
+#pragma unroll
+  for (int q = 0; q < kN; q++) {
+    const int k = base + q * kWave + lane;
+    const int o = min(wave_owner(incl, k), kWave - 1);
+    at[q] = wave_shfl(shift, o) + k;
+  }
+  const int fb = wave_bcast(at[0], 0);  // pair `base`, which exists
+  int w[kN];
+#pragma unroll
+  for (int q = 0; q < kN; q++) {
+    const bool ok = base + q * kWave + lane < total;
+    const int v = olist[ok ? at[q] : fb];
+    w[q] = ok ? v : kNoEntry;
+  }
+  probe_windows<kHashed, kN>(w, lane, tab, t, nx, d, cnt, wcnt);
+}
+
 // Wave-collective: over items[ub0 + 64k + lane] (k = 0, 1, ... stepping by
 // ustep), counts the w in N+(item) found in N+(x) - through `tab`, or by
 // binary search of the sorted nx[0, d) when tab is null - for the items
 // whose list has at most `limit` entries.  Lists of kLongList or more
 // entries are walked by the whole wave, one at a time, kWalk windows of 64 per
 // step so that their loads overlap; shorter ones share windows of 64 pairs
-// through the in-wave balancer.
+// through the in-wave balancer, also kWalk windows per step.
 template <bool kHashed>
 IG_DEV unsigned long long count_pairs(const int2* __restrict__ orng, const int* __restrict__ olist,
                                       const int* __restrict__ items, int n, int limit, int ub0, int ustep,
@@ -483,10 +521,6 @@ IG_DEV unsigned long long count_pairs(const int2* __restrict__ orng, const int* 
   // a lane probes at most n * (limit / 64 + 1) entries here: 32 bits suffice
   unsigned cnt = 0;
   unsigned long long wcnt = 0;  // the long-list hits of the whole wave
-  auto probe = [&](int w) {
-    if constexpr (kHashed) return table_find(tab, t, w);
-    else return search(nx, d, w);
-  };
   for (int ub = ub0; ub < n; ub += ustep) {
     const int i = ub + lane;
     const int2 r = i < n ? orng[items[i]] : make_int2(0, 0);
@@ -517,11 +551,15 @@ IG_DEV unsigned long long count_pairs(const int2* __restrict__ orng, const int* 
     const int incl = wave_inclusive_scan(slen);
     const int total = wave_bcast(incl, kWave - 1);
     const int shift = b - (incl - slen);  // pair k is olist[shift[o] + k] for its owner lane o
-    for (int base = 0; base < total; base += kWave) {
-      const int k = base + lane;
-      const int o = min(wave_owner(incl, k), kWave - 1);
-      const int at = wave_shfl(shift, o) + k;
-      if (k < total) cnt += probe(olist[at]) ? 1u : 0u;
+    // kWalk windows of pairs per step, then one step of just the windows left
+    int base = 0;
+    for (; total - base > (kWalk - 1) * kWave; base += kWalk * kWave)
+      balanced_step<kHashed, kWalk>(olist, incl, shift, total, base, lane, tab, t, nx, d, cnt, wcnt);
+    switch ((total - base + kWave - 1) / kWave) {
+      case 1: balanced_step<kHashed, 1>(olist, incl, shift, total, base, lane, tab, t, nx, d, cnt, wcnt); break;
+      case 2: balanced_step<kHashed, 2>(olist, incl, shift, total, base, lane, tab, t, nx, d, cnt, wcnt); break;
+      case 3: balanced_step<kHashed, 3>(olist, incl, shift, total, base, lane, tab, t, nx, d, cnt, wcnt); break;
+      default: break;
     }
   }
   return cnt + (lane == 0 ? wcnt : 0ull);
