@@ -69,7 +69,7 @@ static_assert(kHugeChunk % kTcBlock == 0, "tc_pack_huge moves whole items");
 constexpr int kWaveWork = 1 << 15;  // probe bound of a wave-pass task
 constexpr int kPartItems = 1024;    // items per workgroup-pass part
 constexpr int kUnroll = 4;          // windows of 64 per step over a long list
-constexpr int kLongList = 64;       // lists walked by the whole wave (32 and 128: slower)
+constexpr int kLongList = 96;       // lists walked by the whole wave (64, 160, 256: slower; 128: the same)
 constexpr int kWalk = 4;            // windows of 64 in flight per step of that walk (2 and 8: slower)
 constexpr int kTinyList = 16;       // wave-pass tasks this small are counted kTinyGroup at a time
 constexpr int kTinyGroup = 16;      // tables of kTinySlots side by side in the wave's region

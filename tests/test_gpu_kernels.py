@@ -124,7 +124,8 @@ def test_tc_clique_unions(ig, ctx):
     consecutive and on shuffled ids.  In a clique the out-list of the j-th
     smallest id holds the k - 1 - j larger ones, so: K3..K17 give every task
     of up to 16 entries (counted 16 tasks at a time); K33..K200 wave-pass
-    tasks whose probed lists reach 64 entries and more (the whole-wave walk,
+    tasks whose probed lists run from short (the in-wave balancer, 1 to 4
+    windows of pairs per step) to 96 entries and more (the whole-wave walk,
     ending on 1 to 4 windows); K300 and K2049 workgroup-pass tasks, the
     latter's largest table at load 1/4 (2048 entries in 8192 slots)."""
     sizes = [3, 4, 5, 9, 16, 17] * 12 + [33, 64, 65, 66, 129, 200, 300, 2049]
