@@ -160,7 +160,7 @@ int indigo_ctx_trim(indigo_ctx* ctx);
 int indigo_ctx_set_hint(indigo_ctx* ctx, int key, int value);
 #define INDIGO_HINT_BFS_DIRECTION 1 /* 0 auto, 1 top-down only, 2 allow bottom-up */
 #define INDIGO_HINT_SSSP_DELTA 2    /* 0 auto, else near-far split increment */
-#define INDIGO_HINT_SMALL_FRONTIER 3 /* 0 auto, 1 off: never use the one-workgroup mode */
+#define INDIGO_HINT_SMALL_FRONTIER 3 /* 0 auto, 1 off: never use the one-workgroup mode (BFS, SSSP) */
 #define INDIGO_HINT_BUDGET_MS 4      /* bound on device-side waits in ms (0: 60000) */
 #define INDIGO_HINT_BLOCKS_PER_CU 5  /* cap on co-resident workgroups per CU of persistent kernels (0: occupancy) */
 #define INDIGO_HINT_TC_LDS_LIST 6    /* longest oriented list TC hashes in LDS (0: 16384); longer ones use binary search */

@@ -73,6 +73,7 @@ struct SsspArgs {
   int edges;
   int src;
   int delta_hint;
+  int allow_small;  // the one-workgroup phase may run (INDIGO_HINT_SMALL_FRONTIER != 1)
   const int* __restrict__ nindex;
   const int* __restrict__ nlist;
   const int* __restrict__ eweight;
@@ -477,7 +478,8 @@ __global__ __launch_bounds__(kSsspBlock) __attribute__((amdgpu_waves_per_eu(6, 8
     unsigned* far_min = &ctl->far_min[epoch % 3].v;
     const bool split = static_cast<unsigned long long>(n) + nm + nh == 0;
     const unsigned long long m = ld_relaxed(&ic->m);
-    const bool small = !split && nm == 0 && nh == 0 && m <= static_cast<unsigned long long>(kSsspSmallCap);
+    const bool small =
+        a.allow_small && !split && nm == 0 && nh == 0 && m <= static_cast<unsigned long long>(kSsspSmallCap);
     trace_level(&ctl->trace, L, split ? ld_relaxed(far_cnt) : n + nm + nh, split ? 0ull : m,
                 split ? 3 : (small ? 2 : 0));
     if (small) {
@@ -601,6 +603,7 @@ int sssp_run(indigo_ctx* ctx, int nodes, int edges, const int* nindex, const int
   a.edges = edges;
   a.src = src;
   a.delta_hint = ctx->hints[INDIGO_HINT_SSSP_DELTA];
+  a.allow_small = ctx->hints[INDIGO_HINT_SMALL_FRONTIER] != 1;
   a.nindex = nindex;
   a.nlist = nlist;
   a.eweight = eweight;

@@ -30,19 +30,23 @@ def test_sssp_delta_hints(ig, orc, ctx, delta):
         ctx.set_hint(ig._lib.HINT_SSSP_DELTA, 0)
 
 
+@pytest.mark.parametrize("small", [0, 1])
 @pytest.mark.parametrize("delta", [0, 1, 300, 100000])
-def test_sssp_deep_graphs(ig, orc, ctx, delta):
-    """Deep sparse graphs run most iterations in the one-workgroup phase,
-    and with it most far-pile splits (delta 1: thousands of them; 100000:
-    almost none); every source's distances equal the oracle's."""
+def test_sssp_deep_graphs(ig, orc, ctx, delta, small):
+    """Deep sparse graphs run most iterations in the one-workgroup phase
+    (small 0), and with it most far-pile splits (delta 1: thousands of them;
+    100000: almost none), or all on the grid (small 1); every source's
+    distances equal the oracle's."""
     ctx.set_hint(ig._lib.HINT_SSSP_DELTA, delta)
+    ctx.set_hint(ig._lib.HINT_SMALL_FRONTIER, small)
     try:
         for g in (ig.road(256, 256, seed=7, wmin=1, wmax=1000), weighted(ig, ig.grid(300, 200), seed=3, wmax=50)):
             dg = ig.DeviceGraph(g)
             for src in (0, g.nodes // 2 + 17, g.nodes - 1):
-                assert np.array_equal(ctx.sssp(dg, src), orc.sssp(g, src)), (g.name, delta, src)
+                assert np.array_equal(ctx.sssp(dg, src), orc.sssp(g, src)), (g.name, delta, small, src)
     finally:
         ctx.set_hint(ig._lib.HINT_SSSP_DELTA, 0)
+        ctx.set_hint(ig._lib.HINT_SMALL_FRONTIER, 0)
 
 
 def test_sssp_rmat18(ig, orc, ctx):
