@@ -27,6 +27,17 @@ def test_library_exports_every_declared_symbol(ig):
     assert not missing, missing
 
 
+def test_python_constants_match_the_header(ig):
+    """Every numeric INDIGO_* constant of include/indigo_hip.h (status codes,
+    hints, query kinds, capacities) has the same value in the binding."""
+    import re
+    text = open(os.path.join(REPO, "include", "indigo_hip.h")).read()
+    defs = re.findall(r"^#define INDIGO_(\w+)\s+\(?(-?\d+)\)?", text, re.M)
+    assert len(defs) >= 20
+    for name, value in defs:
+        assert getattr(ig._lib, name) == int(value), name
+
+
 def test_python_binding_covers_the_abi(ig):
     bound = set(ig._lib._SIGNATURES) | {"indigo_last_error"}
     assert set(declared_functions()) <= bound
