@@ -20,6 +20,14 @@
  *      stream with cached workspace, and the kernel time of the last call is
  *      measured with HIP events on that stream.  Benchmarks, the batched
  *      engine and the serving loop use this layer.
+ *
+ * Graph contents are trusted, as the reference trusts them: nindex[0] = 0,
+ * nindex non-decreasing up to nindex[nodes] = edges, every nlist entry in
+ * [0, nodes), and symmetric lists for CC, MIS and TC.  Sizes, sources and
+ * null pointers are checked; list contents are not, since that would cost a
+ * pass over the edges per call (TC alone reports unsorted lists and self
+ * loops, with INDIGO_ERR_UNSORTED, as the reference's TC does).
+ * indigo_graph_check (indigo_graph.h) verifies a host graph beforehand.
  */
 #ifndef INDIGO_HIP_H
 #define INDIGO_HIP_H
