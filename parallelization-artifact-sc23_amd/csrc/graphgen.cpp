@@ -426,7 +426,7 @@ int indigo_write_egr(const char* path, const indigo_host_graph* g)
 
 int indigo_graph_check(const indigo_host_graph* g)
 {
-  if (g == nullptr || g->nodes < 1 || g->edges < 0 || !g->nindex) return 1;
+  if (g == nullptr || g->nodes < 1 || g->edges < 0 || !g->nindex || (!g->nlist && g->edges > 0)) return 1;
   if (g->nindex[0] != 0 || g->nindex[g->nodes] != g->edges) return 1;
   for (int v = 0; v < g->nodes; v++)
     if (g->nindex[v + 1] < g->nindex[v]) return 1;
