@@ -105,6 +105,21 @@ def tc(g) -> int:
     return int(load().orc_tc(g.nodes, _p(ni), _p(nl)))
 
 
+def synth_weights(g) -> np.ndarray:
+    """Weights the reference harnesses make for a graph stored without them
+    (include/sssp_vertex_cuda.h:127-137, include/mis_vertex_cuda.h:117-127):
+    w[j] = 1 + ((i * nei) % nodes) for entry j = (i, nei) of the CSR, the
+    product in 32-bit int (it wraps on the GPU box's compilers), % truncating
+    toward zero as in C, then negated if negative."""
+    ni = np.asarray(g.nindex, dtype=np.int64)
+    nl = np.asarray(g.nlist, dtype=np.int64)
+    rows = np.repeat(np.arange(g.nodes, dtype=np.int64), np.diff(ni))
+    prod = (rows * nl) & 0xFFFFFFFF
+    prod = np.where(prod >= 1 << 31, prod - (1 << 32), prod)
+    w = 1 + np.fmod(prod, g.nodes)
+    return np.abs(w).astype(np.int32)
+
+
 # ---- the compiled reference (oracle/_ref) ------------------------------------
 def ref_available(alg: str) -> bool:
     return os.path.exists(os.path.join(REF_DIR, f"libref_{alg}.so"))
