@@ -4,6 +4,10 @@ and of the reference's own C++ code compiled into _ref/ (when built).
 TEST INFRASTRUCTURE ONLY.  Imported by tests/, __graft_entry__.smoke() and
 the CPU-baseline leg of bench.py, and only as the checker or the CPU
 baseline - never on the product path.
+
+Besides the bindings, synth_weights restates the reference harnesses'
+weight synthesis in numpy; it is pinned by hand-worked values, two of them
+with a wrapping 32-bit product (tests/test_harness_host.py).
 """
 from __future__ import annotations
 
