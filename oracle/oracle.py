@@ -113,7 +113,7 @@ def synth_weights(g) -> np.ndarray:
     """Weights the reference harnesses make for a graph stored without them
     (include/sssp_vertex_cuda.h:127-137, include/mis_vertex_cuda.h:117-127):
     w[j] = 1 + ((i * nei) % nodes) for entry j = (i, nei) of the CSR, the
-    product in 32-bit int (it wraps on the GPU box's compilers), % truncating
+    product in 32-bit int (wrapping, as gcc and clang compile it), % truncating
     toward zero as in C, then negated if negative."""
     ni = np.asarray(g.nindex, dtype=np.int64)
     nl = np.asarray(g.nlist, dtype=np.int64)
@@ -160,6 +160,9 @@ def _ref_lib(alg: str):
         elif alg == "pr":
             lib.ref_pr_threads.argtypes = [c_int, c_int, _PI, _PI, POINTER(c_float), _PI, c_int]
             lib.ref_pr_threads.restype = c_double
+        elif alg == "tc_omp":
+            lib.ref_tc_omp.argtypes = [c_int, _PI, _PI, POINTER(c_longlong), c_int]
+            lib.ref_tc_omp.restype = c_double
         elif alg == "tc":
             lib.ref_tc_threads.argtypes = [c_int, _PI, _PI, POINTER(c_longlong), c_int]
             lib.ref_tc_threads.restype = c_double
@@ -234,4 +237,12 @@ def ref_tc(g, threads: int = 1, serial: bool = False):
         return int(lib.ref_tc_serial(g.nodes, _p(ni), _p(nl))), 0.0
     c = c_longlong()
     rt = lib.ref_tc_threads(g.nodes, _p(ni), _p(nl), ctypes.byref(c), threads)
+    return int(c.value), rt
+
+
+def ref_tc_omp(g, threads: int = 1):
+    """The reference's best published OpenMP TC variant (oracle/_ref)."""
+    ni, nl = _csr(g)
+    c = c_longlong()
+    rt = _ref_lib("tc_omp").ref_tc_omp(g.nodes, _p(ni), _p(nl), ctypes.byref(c), threads)
     return int(c.value), rt

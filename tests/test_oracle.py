@@ -78,6 +78,8 @@ def test_oracle_matches_compiled_reference(ig, orc, seed):
         mine, it2 = orc.pr_ref32(g)
         assert it == it2 and np.array_equal(sc, mine), g.name
         assert orc.tc(g) == orc.ref_tc(g, 4)[0], g.name
+        if orc.ref_available("tc_omp"):
+            assert orc.tc(g) == orc.ref_tc_omp(g, 4)[0], g.name
 
 
 # ---- (c) independent facts ----------------------------------------------------
