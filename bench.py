@@ -462,9 +462,10 @@ def measure_deep(kind: str, steps: int, warmup: int) -> dict:
 def cpu_baseline(scale: int, threads: int) -> dict | None:
     """The reference's own best published CPU variants (compiled from its
     sources into oracle/_ref) on a bounded sample of the workload: the best
-    C++-threads variant of each algorithm, and for MIS and TC also the best
-    OpenMP variant (MIS: faster in the published data; TC: its dynamic
-    schedule balances R-MAT's skewed lists); the faster of the two counts."""
+    C++-threads variant of each algorithm (data/*_cpp.csv) and an OpenMP
+    variant: for MIS and TC the best published one on rmat (data/mis_omp.csv,
+    data/tc_omp.csv), for the others the winner of a sweep of all variants
+    (oracle/sweep_cpu.py); the faster of the two counts."""
     sys.path.insert(0, os.path.join(REPO, "oracle"))
     try:
         import oracle as orc  # checker / baseline only
@@ -481,17 +482,17 @@ def cpu_baseline(scale: int, threads: int) -> dict | None:
     _, times["sssp"] = orc.ref_sssp(g, 0, threads)
     _, times["cc"] = orc.ref_cc(g, threads)
     _, _, times["mis"] = orc.ref_mis(g, threads)
-    variants = {a: "C++ threads" for a in ALGS}
-    if orc.ref_available("mis_omp"):
-        _, _, t_omp = orc.ref_mis_omp(g, threads)
-        if t_omp < times["mis"]:
-            times["mis"], variants["mis"] = t_omp, "OpenMP"
-    _, it, times["pr"] = orc.ref_pr(g, threads)
+    _, _, times["pr"] = orc.ref_pr(g, threads)
     _, times["tc"] = orc.ref_tc(g, threads)
-    if orc.ref_available("tc_omp"):
-        _, t_omp = orc.ref_tc_omp(g, threads)
-        if t_omp < times["tc"]:
-            times["tc"], variants["tc"] = t_omp, "OpenMP"
+    variants = {a: "C++ threads" for a in ALGS}
+    omp = {"bfs": lambda: orc.ref_bfs_omp(g, 0, threads)[-1], "sssp": lambda: orc.ref_sssp_omp(g, 0, threads)[-1],
+           "cc": lambda: orc.ref_cc_omp(g, threads)[-1], "mis": lambda: orc.ref_mis_omp(g, threads)[-1],
+           "pr": lambda: orc.ref_pr_omp(g, threads)[-1], "tc": lambda: orc.ref_tc_omp(g, threads)[-1]}
+    for a, run in omp.items():
+        if orc.ref_available(f"{a}_omp"):
+            t_omp = run()
+            if 0 < t_omp < times[a]:
+                times[a], variants[a] = t_omp, "OpenMP"
     per = {a: 1e-9 * g.edges / max(t, 1e-9) for a, t in times.items()}
     total = sum(times.values())
     log("cpu baseline times " + ", ".join(f"{a}={t:.3f}s" for a, t in times.items()))
@@ -501,8 +502,8 @@ def cpu_baseline(scale: int, threads: int) -> dict | None:
         "cores": threads,
         "kind": "reference",
         "sample": f"rmat scale {scale} ef16 (V={g.nodes}, E={g.edges}), one run of each algorithm's best published "
-                  f"C++-threads variant (data/*_cpp.csv) and, for MIS and TC, also its best OpenMP variant "
-                  f"(data/mis_omp.csv, data/tc_omp.csv), the faster counting; {threads} threads",
+                  f"C++-threads variant (data/*_cpp.csv) and an OpenMP variant (best published for MIS and TC, "
+                  f"sweep winner for the others), the faster counting; {threads} threads",
         "variants": variants,
         "per_algorithm_gteps": {a: round(x, 6) for a, x in per.items()},
         "seconds": {a: round(t, 4) for a, t in times.items()},

@@ -160,6 +160,18 @@ def _ref_lib(alg: str):
         elif alg == "pr":
             lib.ref_pr_threads.argtypes = [c_int, c_int, _PI, _PI, POINTER(c_float), _PI, c_int]
             lib.ref_pr_threads.restype = c_double
+        elif alg == "bfs_omp":
+            lib.ref_bfs_omp.argtypes = [c_int, c_int, _PI, _PI, c_int, _PI, c_int]
+            lib.ref_bfs_omp.restype = c_double
+        elif alg == "sssp_omp":
+            lib.ref_sssp_omp.argtypes = [c_int, c_int, _PI, _PI, _PI, c_int, _PI, c_int]
+            lib.ref_sssp_omp.restype = c_double
+        elif alg == "cc_omp":
+            lib.ref_cc_omp.argtypes = [c_int, c_int, _PI, _PI, _PI, c_int]
+            lib.ref_cc_omp.restype = c_double
+        elif alg == "pr_omp":
+            lib.ref_pr_omp.argtypes = [c_int, c_int, _PI, _PI, POINTER(c_float), _PI, c_int]
+            lib.ref_pr_omp.restype = c_double
         elif alg == "tc_omp":
             lib.ref_tc_omp.argtypes = [c_int, _PI, _PI, POINTER(c_longlong), c_int]
             lib.ref_tc_omp.restype = c_double
@@ -246,3 +258,38 @@ def ref_tc_omp(g, threads: int = 1):
     c = c_longlong()
     rt = _ref_lib("tc_omp").ref_tc_omp(g.nodes, _p(ni), _p(nl), ctypes.byref(c), threads)
     return int(c.value), rt
+
+
+def ref_bfs_omp(g, src: int, threads: int = 1):
+    """The OpenMP BFS variant that won the CPU sweep (oracle/_ref)."""
+    ni, nl = _csr(g)
+    out = np.empty(g.nodes, np.int32)
+    rt = _ref_lib("bfs_omp").ref_bfs_omp(g.nodes, g.edges, _p(ni), _p(nl), src, _p(out), threads)
+    return out, rt
+
+
+def ref_sssp_omp(g, src: int, threads: int = 1):
+    """The OpenMP edge-centric SSSP variant that won the CPU sweep (oracle/_ref)."""
+    ni, nl = _csr(g)
+    w = np.ascontiguousarray(g.eweight, dtype=np.int32)
+    out = np.empty(g.nodes, np.int32)
+    rt = _ref_lib("sssp_omp").ref_sssp_omp(g.nodes, g.edges, _p(ni), _p(nl), _p(w), src, _p(out), threads)
+    return out, rt
+
+
+def ref_cc_omp(g, threads: int = 1):
+    """The OpenMP edge-centric CC variant that won the CPU sweep (oracle/_ref)."""
+    ni, nl = _csr(g)
+    out = np.empty(g.nodes, np.int32)
+    rt = _ref_lib("cc_omp").ref_cc_omp(g.nodes, g.edges, _p(ni), _p(nl), _p(out), threads)
+    return out, rt
+
+
+def ref_pr_omp(g, threads: int = 1):
+    """The OpenMP PageRank variant that won the CPU sweep (oracle/_ref)."""
+    ni, nl = _csr(g)
+    sc = np.empty(g.nodes, np.float32)
+    it = c_int()
+    rt = _ref_lib("pr_omp").ref_pr_omp(g.nodes, g.edges, _p(ni), _p(nl), sc.ctypes.data_as(POINTER(c_float)),
+                                       ctypes.byref(it), threads)
+    return sc, it.value, rt

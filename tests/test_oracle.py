@@ -80,6 +80,16 @@ def test_oracle_matches_compiled_reference(ig, orc, seed):
         assert orc.tc(g) == orc.ref_tc(g, 4)[0], g.name
         if orc.ref_available("tc_omp"):
             assert orc.tc(g) == orc.ref_tc_omp(g, 4)[0], g.name
+        # the OpenMP variants the CPU baseline may time compute the same results
+        if orc.ref_available("bfs_omp"):
+            assert np.array_equal(orc.bfs(g, 0), orc.ref_bfs_omp(g, 0, 4)[0]), g.name
+        if orc.ref_available("sssp_omp"):
+            assert np.array_equal(orc.sssp(g, 0), orc.ref_sssp_omp(g, 0, 4)[0]), g.name
+        if orc.ref_available("cc_omp"):
+            assert np.array_equal(orc.cc(g), orc.ref_cc_omp(g, 4)[0]), g.name
+        if orc.ref_available("pr_omp"):
+            sc, it3, _ = orc.ref_pr_omp(g, 4)
+            assert it3 == it2 and np.allclose(sc, mine, rtol=0, atol=1e-6), g.name
 
 
 # ---- (c) independent facts ----------------------------------------------------
