@@ -19,6 +19,7 @@ import argparse
 import json
 import os
 import statistics
+import subprocess
 import sys
 import time
 
@@ -510,6 +511,19 @@ def cpu_baseline(scale: int, threads: int) -> dict | None:
     }
 
 
+def cpu_baseline_isolated(scale: int) -> dict | None:
+    """cpu_baseline in a child process: the reference's CPU code runs there,
+    so a crash in it (some of its variants fault on some graphs) costs the
+    baseline, not the benchmark.  The child touches no GPU."""
+    r = subprocess.run([sys.executable, os.path.abspath(__file__), "--cpu-baseline-only", "--cpu-scale", str(scale)],
+                       stdout=subprocess.PIPE, text=True, timeout=600)
+    lines = r.stdout.strip().splitlines()
+    if r.returncode != 0 or not lines:
+        log(f"cpu baseline skipped: child exited with status {r.returncode}")
+        return None
+    return json.loads(lines[-1])
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -535,7 +549,12 @@ def main() -> int:
                     help="only run a 1 GiB device-to-device copy 5 times (the PMC scale reference) and exit")
     ap.add_argument("--record-traffic", nargs=4, metavar=("KFETCH", "KWRITE", "CFETCH", "CWRITE"),
                     help="write profiles/traffic_<workload>.json for the main kernels from four PMC CSVs")
+    ap.add_argument("--cpu-baseline-only", action="store_true",
+                    help="print the CPU baseline record alone (the child process of a full run)")
     args = ap.parse_args()
+    if args.cpu_baseline_only:
+        print(json.dumps(cpu_baseline(args.cpu_scale, min(16, os.cpu_count() or 1))), flush=True)
+        return 0
     if args.record_traffic:
         workload = f"rmat{args.scale} ef{args.edge_factor}"
         return record_traffic(args.record_traffic, workload,
@@ -658,9 +677,8 @@ def main() -> int:
 
     cpu = None
     if not args.no_cpu_baseline:
-        threads = min(16, os.cpu_count() or 1)
         try:
-            cpu = cpu_baseline(args.cpu_scale, threads)
+            cpu = cpu_baseline_isolated(args.cpu_scale)
         except Exception as exc:  # the baseline is reported, never required
             log(f"cpu baseline skipped: {exc}")
 

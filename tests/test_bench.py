@@ -63,3 +63,14 @@ def test_traffic_record_needs_the_copy(tmp_path):
         _pmc_csv(tmp_path / n, [("x", "FETCH_SIZE", 1.0)])
     assert bench.record_traffic([str(tmp_path / n) for n in ("kf.csv", "kw.csv", "cf.csv", "cw.csv")], "w",
                                 str(tmp_path / "t.json")) == 1
+
+
+@pytest.mark.skipif(not os.path.exists(os.path.join(REPO, "oracle", "_ref", "libref_tc.so")),
+                    reason="oracle/_ref not built (needs the reference sources)")
+def test_cpu_baseline_runs_in_a_child_process():
+    import bench
+
+    rec = bench.cpu_baseline_isolated(12)
+    assert rec is not None and rec["kind"] == "reference" and rec["value"] > 0
+    assert set(rec["variants"]) == set(bench.ALGS)
+    assert all(v in ("C++ threads", "OpenMP") for v in rec["variants"].values())
