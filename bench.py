@@ -588,6 +588,14 @@ def main() -> int:
         print(f"unknown algorithm in --algs {args.algs}", file=sys.stderr)
         return 2
 
+    # the CPU baseline first, in a child process, before this process uses the GPU
+    cpu = None
+    if not args.no_cpu_baseline:
+        try:
+            cpu = cpu_baseline_isolated(args.cpu_scale)
+        except Exception as exc:  # the baseline is reported, never required
+            log(f"cpu baseline skipped: {exc}")
+
     import indigo2 as ig
 
     t0 = time.time()
@@ -674,13 +682,6 @@ def main() -> int:
         t = read_traffic(traffic_path, workload, kname[a])
         roof[a]["traffic"] = t["bytes_per_launch"] if t else None
     traffic = read_traffic(traffic_path, workload, kname[dom])
-
-    cpu = None
-    if not args.no_cpu_baseline:
-        try:
-            cpu = cpu_baseline_isolated(args.cpu_scale)
-        except Exception as exc:  # the baseline is reported, never required
-            log(f"cpu baseline skipped: {exc}")
 
     line = {
         "metric": "GTEPS per algorithm (six Indigo2 kernels), aggregate over one step on 1 MI355X",
