@@ -61,22 +61,12 @@ int ensure_host_pinned(indigo_ctx* ctx, size_t bytes)
   return INDIGO_OK;
 }
 
-namespace {
-std::mutex g_cap_mu;
-std::unordered_map<const void*, int> g_cap;  // learned co-residency per kernel
-}  // namespace
-
 int resident_per_cu(indigo_ctx* ctx, const void* kernel, int block, size_t shmem, int* per_cu)
 {
   int n = 0;
   IG_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kernel, block, shmem));
   if (n < 1) return set_error(INDIGO_ERR_DEVICE, "kernel cannot be resident with block %d", block);
-  // A grid that needs every wave slot of every CU (4 x 512 threads on a
-  // 2048-thread CU) did not assemble on MI355X: the first such launch waited
-  // out the residency deadline and ran again at one workgroup less per CU.
-  // Leave that workgroup out from the start.
-  if (n > 1 && static_cast<long long>(n) * block >= ctx->max_threads_per_cu) n -= 1;
-  *per_cu = n;
+  *per_cu = plan_per_cu(n, block, ctx->max_threads_per_cu);
   return INDIGO_OK;
 }
 
@@ -85,21 +75,19 @@ int coop_launch(indigo_ctx* ctx, const void* kernel, int block, size_t shmem, vo
 {
   int per_cu = 0;
   IG_TRY(resident_per_cu(ctx, kernel, block, shmem, &per_cu));
-  {
-    std::lock_guard<std::mutex> lock(g_cap_mu);
-    auto it = g_cap.find(kernel);
-    if (it != g_cap.end()) per_cu = std::min(per_cu, it->second);
-  }
-  if (ctx->hints[INDIGO_HINT_BLOCKS_PER_CU] > 0) per_cu = std::min(per_cu, ctx->hints[INDIGO_HINT_BLOCKS_PER_CU]);
-  int grid = per_cu * ctx->compute_units;
-  if (max_blocks > 0) grid = std::min(grid, max_blocks);
-  if (ctx->hints[INDIGO_HINT_MAX_WORKGROUPS] > 0) grid = std::min(grid, ctx->hints[INDIGO_HINT_MAX_WORKGROUPS]);
-  grid = std::max(grid, 1);
+  GridLimits lim;
+  lim.compute_units = ctx->compute_units;
+  lim.hint_blocks_per_cu = ctx->hints[INDIGO_HINT_BLOCKS_PER_CU];
+  lim.max_blocks = max_blocks;
+  lim.hint_max_workgroups = ctx->hints[INDIGO_HINT_MAX_WORKGROUPS];
+  const auto it = ctx->grid_caps.find(kernel);
+  lim.learned_cap = it != ctx->grid_caps.end() ? it->second : 0;
+  const int grid = plan_grid(per_cu, lim);
   if (name != nullptr) IG_TRY(kernel_begin(ctx, name));
   IG_HIP_TRY(hipLaunchKernel(kernel, dim3(grid), dim3(block), args, static_cast<unsigned>(shmem), ctx->stream));
   if (name != nullptr) IG_TRY(kernel_end(ctx));
   ctx->last_grid = grid;
-  ctx->last_per_cu = per_cu;
+  ctx->last_per_cu = lim.hint_blocks_per_cu > 0 ? std::min(per_cu, lim.hint_blocks_per_cu) : per_cu;
   if (grid_out) *grid_out = grid;
   return INDIGO_OK;
 }
@@ -108,12 +96,10 @@ int check_abort(indigo_ctx* ctx, const void* kernel, unsigned abort_code, const 
 {
   if (abort_code == 0u) return INDIGO_OK;
   if (abort_code == kAbortNotResident) {
-    const int launched = ctx->last_per_cu;
-    if (launched <= 1)
-      return set_error(INDIGO_ERR_DEVICE, "%s: grid of %d workgroups never assembled even at 1 per CU", what,
-                       ctx->last_grid);
-    std::lock_guard<std::mutex> lock(g_cap_mu);
-    g_cap[kernel] = launched - 1;
+    const int cap = lowered_cap(ctx->last_grid, ctx->compute_units);
+    if (cap < 1)
+      return set_error(INDIGO_ERR_DEVICE, "%s: a grid of %d workgroup never assembled", what, ctx->last_grid);
+    ctx->grid_caps[kernel] = cap;
     return kNotResident;
   }
   return set_error(INDIGO_ERR_TIMEOUT, "%s: persistent kernel stopped at a bounded wait (abort %u)", what,
@@ -217,12 +203,15 @@ static int ctx_init(int device, indigo_ctx* c)
   return INDIGO_OK;
 }
 
+namespace {
+std::mutex g_default_mu;
+indigo_ctx* g_default = nullptr;  // the drop-in entries' context
+}  // namespace
+
 indigo_ctx* default_ctx(int* rc)
 {
-  static std::mutex mu;
-  static indigo_ctx* ctx = nullptr;
-  std::lock_guard<std::mutex> lock(mu);
-  if (ctx == nullptr) {
+  std::lock_guard<std::mutex> lock(g_default_mu);
+  if (g_default == nullptr) {
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess) {
       (void)hipGetLastError();
@@ -231,14 +220,15 @@ indigo_ctx* default_ctx(int* rc)
     indigo_ctx* c = new indigo_ctx();
     const int r = ctx_init(dev, c);
     if (r != INDIGO_OK) {
+      if (c->stream) (void)hipStreamDestroy(c->stream);
       delete c;
       *rc = r;
       return nullptr;
     }
-    ctx = c;
+    g_default = c;
   }
   *rc = INDIGO_OK;
-  return ctx;
+  return g_default;
 }
 
 }  // namespace indigo
@@ -336,6 +326,24 @@ int indigo_ctx_destroy(indigo_ctx* ctx)
   if (ctx->ev_k1) (void)hipEventDestroy(ctx->ev_k1);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
+  return INDIGO_OK;
+}
+
+int indigo_shutdown(void)
+{
+  indigo_ctx* c = nullptr;
+  {
+    std::lock_guard<std::mutex> lock(g_default_mu);
+    c = g_default;
+    g_default = nullptr;
+  }
+  if (c != nullptr) (void)indigo_ctx_destroy(c);
+  int count = 0;
+  if (hipGetDeviceCount(&count) == hipSuccess) {
+    for (int d = 0; d < count; d++)
+      if (hipSetDevice(d) == hipSuccess) (void)hipDeviceSynchronize();
+  }
+  (void)hipGetLastError();
   return INDIGO_OK;
 }
 

@@ -7,8 +7,10 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <unordered_map>
 
 #include "errors.hpp"
+#include "grid_plan.hpp"
 #include "indigo_hip.h"
 
 struct indigo_ctx {
@@ -54,6 +56,9 @@ struct indigo_ctx {
   hipEvent_t ev_stage[2] = {nullptr, nullptr};
   int last_grid = 0;    // workgroups of the last persistent launch
   int last_per_cu = 0;  // co-resident workgroups per CU it was sized for
+  // grid cap per persistent kernel, learned from grids of this context that
+  // did not assemble (check_abort); it ends with the context
+  std::unordered_map<const void*, int> grid_caps;
 };
 
 namespace indigo {
@@ -100,8 +105,9 @@ inline long long budget_ticks(const indigo_ctx* ctx)
 }
 
 // Launches a persistent `kernel` with as many workgroups of `block` threads
-// as can be co-resident: occupancy x compute units, capped at max_blocks and
-// by the per-kernel residency cap learned from earlier launches.  It is a
+// as can be co-resident: occupancy x compute units, capped at max_blocks, by
+// the context's hints and by the context's cap for this kernel learned from
+// earlier launches (plan_grid in grid_plan.hpp).  It is a
 // plain launch: the kernels' grid barrier is software (device atomics), and
 // co-residency is checked by the kernel itself (residency_barrier), which
 // aborts a grid that does not assemble so the host relaunches it smaller.
@@ -115,7 +121,8 @@ int coop_launch(indigo_ctx* ctx, const void* kernel, int block, size_t shmem, vo
 // cap or hint (occupancy, less one when that would fill the CU).
 int resident_per_cu(indigo_ctx* ctx, const void* kernel, int block, size_t shmem, int* per_cu);
 // Internal status: the last persistent launch did not assemble (see
-// residency_barrier); the cap of `kernel` has been lowered, launch again.
+// residency_barrier); the context's cap of `kernel` has been lowered below
+// the grid actually launched (lowered_cap in grid_plan.hpp), launch again.
 constexpr int kNotResident = -100;
 // Checks a finished persistent kernel's barrier abort code: 0 ok,
 // kNotResident (cap lowered), or INDIGO_ERR_TIMEOUT.
@@ -147,7 +154,7 @@ int kernel_end(indigo_ctx* ctx);
 int timing_end(indigo_ctx* ctx, int launches, int iterations);
 
 // Process-wide default context for the drop-in entries (created lazily on
-// device 0, or on the current device).
+// device 0, or on the current device); indigo_shutdown releases it.
 indigo_ctx* default_ctx(int* rc);
 
 }  // namespace indigo
