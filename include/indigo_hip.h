@@ -137,6 +137,14 @@ typedef struct indigo_ctx indigo_ctx;
  * used by two host threads at once. */
 int indigo_ctx_create(int device, indigo_ctx** out);
 int indigo_ctx_destroy(indigo_ctx* ctx);
+/* Releases the process-wide context of the drop-in entries (its stream,
+ * workspace and pinned buffers) and waits for every device to go idle.  Call
+ * it before the process exits, while the HIP runtime is still up: the
+ * harness mains do at the end of main, the Python binding from an atexit
+ * hook.  A later drop-in call creates the context again.  Contexts made
+ * with indigo_ctx_create are the caller's to destroy.  No counterpart in the
+ * reference (its harnesses leave teardown to process exit). */
+int indigo_shutdown(void);
 /* Milliseconds between the first and last kernel of the last resident call,
  * measured with HIP events on the context stream. */
 int indigo_ctx_last_ms(indigo_ctx* ctx, float* ms);

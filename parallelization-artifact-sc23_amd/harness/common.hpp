@@ -38,11 +38,15 @@ inline ECLgraph upload(const ECLgraph& g, bool weights)
   return d;
 }
 
+// Frees the device graph and releases the library's default context (its
+// stream, workspace and pinned buffers) while the HIP runtime is still up,
+// before main returns.
 inline void release(ECLgraph& d)
 {
   indigo_free(d.nindex);
   indigo_free(d.nlist);
   if (d.eweight != nullptr) indigo_free(d.eweight);
+  indigo_shutdown();
 }
 
 // Median as the reference harness takes it (bfs_vertex_cuda.h:111-118).

@@ -7,8 +7,10 @@ fallback implementation.
 """
 from __future__ import annotations
 
+import atexit
 import ctypes
 import os
+import weakref
 from ctypes import POINTER, c_char_p, c_double, c_float, c_int, c_longlong, c_size_t, c_uint, c_ulonglong, c_void_p
 
 PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -156,6 +158,7 @@ _SIGNATURES = {
     # resident layer
     "indigo_ctx_create": [c_int, POINTER(c_void_p)],
     "indigo_ctx_destroy": [c_void_p],
+    "indigo_shutdown": [],
     "indigo_ctx_last_ms": [c_void_p, _PF],
     "indigo_ctx_last_stats": [c_void_p, _PI, _PI],
     "indigo_ctx_trim": [c_void_p],
@@ -199,6 +202,45 @@ EXPORTED = tuple(_SIGNATURES) + ("indigo_last_error",)
 
 _lib = None
 
+# Live objects that hold library resources, released by the exit hook before
+# the interpreter tears modules down and before the HIP runtime unloads.
+_contexts: "weakref.WeakSet" = weakref.WeakSet()
+_buffers: "weakref.WeakSet" = weakref.WeakSet()
+_shut_down = False
+
+
+def track_context(obj) -> None:
+    _contexts.add(obj)
+
+
+def track_buffer(obj) -> None:
+    _buffers.add(obj)
+
+
+def shutting_down() -> bool:
+    """True once the exit hook has run: finalizers must not call the library."""
+    return _shut_down
+
+
+def _shutdown() -> None:
+    """Exit hook: destroys the live contexts (each waits for its stream),
+    frees the live device buffers, then releases the library's default
+    context and waits for the device (indigo_shutdown).  Runs from atexit,
+    i.e. before module teardown and before the C runtime's exit handlers, so
+    no finalizer reaches the HIP runtime after it has begun to unload."""
+    global _shut_down
+    if _shut_down:
+        return
+    if _lib is not None:
+        for group in (_contexts, _buffers):
+            for obj in list(group):
+                try:
+                    obj.close()
+                except Exception:
+                    pass
+        _lib.indigo_shutdown()
+    _shut_down = True
+
 
 def load(path: str = LIB_PATH) -> ctypes.CDLL:
     """Loads the HIP library (once).  Raises if it is missing."""
@@ -220,6 +262,7 @@ def load(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.indigo_last_error.argtypes = []
     lib.indigo_last_error.restype = c_char_p
     _lib = lib
+    atexit.register(_shutdown)
     return lib
 
 

@@ -169,12 +169,14 @@ class DeviceBuffer:
         p = ctypes.c_void_p()
         _lib.call("indigo_malloc", ctypes.byref(p), max(self.nbytes, 4))
         self.ptr = p.value
+        _lib.track_buffer(self)
 
     @classmethod
     def adopt(cls, ptr: int, nbytes: int) -> "DeviceBuffer":
         """Takes ownership of a device allocation made by the library."""
         buf = cls.__new__(cls)
         buf.nbytes, buf.ptr = int(nbytes), ptr
+        _lib.track_buffer(buf)
         return buf
 
     @classmethod
@@ -192,9 +194,9 @@ class DeviceBuffer:
         return out
 
     def close(self) -> None:
-        if self.ptr:
+        if self.ptr and not _lib.shutting_down():
             _lib.load().indigo_free(self.ptr)
-            self.ptr = None
+        self.ptr = None
 
     def __del__(self):
         try:

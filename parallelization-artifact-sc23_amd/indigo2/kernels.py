@@ -31,11 +31,12 @@ class Context:
         h = ctypes.c_void_p()
         _lib.call("indigo_ctx_create", device, ctypes.byref(h))
         self.h = h.value
+        _lib.track_context(self)
 
     def close(self) -> None:
-        if self.h:
+        if self.h and not _lib.shutting_down():
             _lib.load().indigo_ctx_destroy(self.h)
-            self.h = None
+        self.h = None
 
     def __enter__(self):
         return self
