@@ -75,6 +75,20 @@ void parallel_for(long long n, int threads, F&& fn)
   for (auto& th : pool) th.join();
 }
 
+// fn(i) on k threads, i = 0..k-1.
+template <typename F>
+void run_threads(int k, F&& fn)
+{
+  if (k <= 1) {
+    fn(0);
+    return;
+  }
+  std::vector<std::thread> pool;
+  pool.reserve(k);
+  for (int i = 0; i < k; i++) pool.emplace_back([&fn, i] { fn(i); });
+  for (auto& th : pool) th.join();
+}
+
 bool weights_wanted(int wmin, int wmax) { return wmin >= 1 && wmax >= wmin; }
 
 void clear(indigo_host_graph* g)
@@ -100,39 +114,103 @@ int build_csr_impl(int nodes, long long npairs, const int* src, const int* dst, 
       return set_error(INDIGO_ERR_INVALID_ARG, "build_csr: vertex id out of range");
   }
 
-  std::vector<std::atomic<long long>> cnt(static_cast<size_t>(nodes) + 1);
-  for (auto& c : cnt) c.store(0, std::memory_order_relaxed);
-  parallel_for(npairs, nt, [&](long long b, long long e, int) {
-    for (long long i = b; i < e; i++) {
-      const int s = src[i], d = dst[i];
-      if (s == d) continue;
-      cnt[s].fetch_add(1, std::memory_order_relaxed);
-      if (symmetrize) cnt[d].fetch_add(1, std::memory_order_relaxed);
-    }
-  });
+  // Bucket the pairs by source.  With room for one count array per thread,
+  // every thread counts and later places its own share of the pairs through
+  // private cursors (no atomics: hub vertices would make them contend);
+  // otherwise shared atomic counters.  Lists are sorted afterwards, so the
+  // order of placement does not show in the result.
   std::vector<long long> off(static_cast<size_t>(nodes) + 1, 0);
-  for (int v = 0; v < nodes; v++) off[v + 1] = off[v] + cnt[v].load(std::memory_order_relaxed);
-  const long long raw = off[nodes];
-  std::vector<int> buf(static_cast<size_t>(std::max<long long>(raw, 1)));
-  for (int v = 0; v < nodes; v++) cnt[v].store(off[v], std::memory_order_relaxed);
-  parallel_for(npairs, nt, [&](long long b, long long e, int) {
-    for (long long i = b; i < e; i++) {
-      const int s = src[i], d = dst[i];
-      if (s == d) continue;
-      buf[cnt[s].fetch_add(1, std::memory_order_relaxed)] = d;
-      if (symmetrize) buf[cnt[d].fetch_add(1, std::memory_order_relaxed)] = s;
-    }
-  });
-  // sort + unique each list; new degree into deg[]
+  std::vector<int> buf;
+  const int parts = static_cast<int>(std::min<long long>(nt, std::max<long long>(1, npairs / 4096 + 1)));
+  if (parts > 1 && static_cast<long long>(parts) * nodes <= (1LL << 27)) {
+    std::vector<std::vector<int>> cnt(static_cast<size_t>(parts));
+    run_threads(parts, [&](int p) {
+      std::vector<int>& c = cnt[static_cast<size_t>(p)];
+      c.assign(static_cast<size_t>(nodes), 0);
+      for (long long i = npairs * p / parts; i < npairs * (p + 1) / parts; i++) {
+        const int a = src[i], b = dst[i];
+        if (a == b) continue;
+        c[a]++;
+        if (symmetrize) c[b]++;
+      }
+    });
+    std::vector<long long> tot(static_cast<size_t>(nodes));
+    parallel_for(nodes, nt, [&](long long b, long long e, int) {
+      for (long long v = b; v < e; v++) {
+        long long t = 0;
+        for (int p = 0; p < parts; p++) t += cnt[static_cast<size_t>(p)][v];
+        tot[v] = t;
+      }
+    });
+    for (int v = 0; v < nodes; v++) off[v + 1] = off[v] + tot[v];
+    buf.resize(static_cast<size_t>(std::max<long long>(off[nodes], 1)));
+    // per-part cursors: part p writes after the entries of parts 0..p-1; the
+    // cursors (64-bit list offsets of up to 2^31 entries) replace the counts
+    std::vector<std::vector<long long>> cur(static_cast<size_t>(parts));
+    run_threads(parts, [&](int p) { cur[static_cast<size_t>(p)].resize(static_cast<size_t>(nodes)); });
+    parallel_for(nodes, nt, [&](long long b, long long e, int) {
+      for (long long v = b; v < e; v++) {
+        long long at = off[v];
+        for (int p = 0; p < parts; p++) {
+          cur[static_cast<size_t>(p)][v] = at;
+          at += cnt[static_cast<size_t>(p)][v];
+        }
+      }
+    });
+    cnt.clear();
+    run_threads(parts, [&](int p) {
+      std::vector<long long>& c = cur[static_cast<size_t>(p)];
+      for (long long i = npairs * p / parts; i < npairs * (p + 1) / parts; i++) {
+        const int a = src[i], b = dst[i];
+        if (a == b) continue;
+        buf[static_cast<size_t>(c[a]++)] = b;
+        if (symmetrize) buf[static_cast<size_t>(c[b]++)] = a;
+      }
+    });
+  } else {
+    std::vector<std::atomic<long long>> cnt(static_cast<size_t>(nodes) + 1);
+    for (auto& c : cnt) c.store(0, std::memory_order_relaxed);
+    parallel_for(npairs, nt, [&](long long b, long long e, int) {
+      for (long long i = b; i < e; i++) {
+        const int a = src[i], d = dst[i];
+        if (a == d) continue;
+        cnt[a].fetch_add(1, std::memory_order_relaxed);
+        if (symmetrize) cnt[d].fetch_add(1, std::memory_order_relaxed);
+      }
+    });
+    for (int v = 0; v < nodes; v++) off[v + 1] = off[v] + cnt[v].load(std::memory_order_relaxed);
+    buf.resize(static_cast<size_t>(std::max<long long>(off[nodes], 1)));
+    for (int v = 0; v < nodes; v++) cnt[v].store(off[v], std::memory_order_relaxed);
+    parallel_for(npairs, nt, [&](long long b, long long e, int) {
+      for (long long i = b; i < e; i++) {
+        const int a = src[i], d = dst[i];
+        if (a == d) continue;
+        buf[cnt[a].fetch_add(1, std::memory_order_relaxed)] = d;
+        if (symmetrize) buf[cnt[d].fetch_add(1, std::memory_order_relaxed)] = a;
+      }
+    });
+  }
+  // sort + unique each list; new degree into deg[].  Vertices are handed out
+  // in chunks from a shared counter: R-MAT puts its long lists at small ids,
+  // so static ranges would leave one thread with all of them.
   std::vector<int> deg(nodes);
-  parallel_for(nodes, nt, [&](long long b, long long e, int) {
-    for (long long v = b; v < e; v++) {
-      int* first = buf.data() + off[v];
-      int* last = buf.data() + off[v + 1];
-      std::sort(first, last);
-      deg[v] = static_cast<int>(std::unique(first, last) - first);
-    }
-  });
+  {
+    constexpr int kChunk = 2048;
+    std::atomic<long long> next{0};
+    run_threads(std::min<int>(nt, nodes / kChunk + 1), [&](int) {
+      for (;;) {
+        const long long b = next.fetch_add(kChunk, std::memory_order_relaxed);
+        if (b >= nodes) break;
+        const long long e = std::min<long long>(nodes, b + kChunk);
+        for (long long v = b; v < e; v++) {
+          int* first = buf.data() + off[v];
+          int* last = buf.data() + off[v + 1];
+          std::sort(first, last);
+          deg[v] = static_cast<int>(std::unique(first, last) - first);
+        }
+      }
+    });
+  }
   long long total = 0;
   for (int v = 0; v < nodes; v++) total += deg[v];
   if (total > 0x7fffffffLL) return set_error(INDIGO_ERR_CAPACITY, "build_csr: more than 2^31-1 CSR entries");
