@@ -332,22 +332,6 @@ def measure_batched() -> dict:
     return line
 
 
-def serving_pool(ig, seed: int = 77):
-    """The serving configuration's 64 resident graphs: 32 R-MAT (edge factor
-    16, a/b/c/d .57/.19/.19/.05) with scales 14, 15, ..., 22 in turn, and 32
-    square grids with sides 256, 512, 1024, 2048 in turn; weights [1, 255]."""
-    gs, names = [], []
-    for i in range(32):
-        sc = 14 + i % 9
-        gs.append(ig.rmat(sc, 16, seed=seed + i, wmin=1, wmax=255))
-        names.append(f"rmat{sc}")
-    for i in range(32):
-        side = 256 << (i % 4)
-        gs.append(ig.grid(side, side, seed=seed + 100 + i, wmin=1, wmax=255))
-        names.append(f"grid{side}")
-    return gs, names
-
-
 def measure_serving(rates=(1000.0,), nq: int = 10000, seed: int = 5, slots: int = 4, max_batch: int = 256,
                     mem_budget: int = 8 << 30) -> dict:
     """10,000 seeded requests with Poisson arrivals at each rate over the pool;
@@ -356,16 +340,14 @@ def measure_serving(rates=(1000.0,), nq: int = 10000, seed: int = 5, slots: int 
     the requests' CSR entries / makespan."""
     import indigo2 as ig
 
+    from indigo2.workloads import serving_pool, serving_requests
+
     t0 = time.time()
-    gs, names = serving_pool(ig)
+    gs, names = serving_pool()
     log(f"serving pool: 64 graphs, V={sum(g.nodes for g in gs)} E={sum(g.edges for g in gs)} "
         f"({time.time() - t0:.1f}s)")
     dgs = [ig.DeviceGraph(g) for g in gs]
-    rng = np.random.default_rng(seed)
-    algs = list(rng.choice(["bfs", "sssp", "cc", "pr", "tc"], size=nq, p=[0.4, 0.3, 0.1, 0.1, 0.1]))
-    gid = rng.integers(0, len(gs), nq)
-    src = np.array([rng.integers(0, gs[g].nodes) for g in gid])
-    gaps = rng.exponential(1.0, nq)
+    algs, gid, src, gaps = serving_requests(gs, nq, seed)
     edges = np.array([gs[g].edges for g in gid], dtype=np.float64)
     # warm-up: a short run touches every code path and sizes the workspaces
     log("serving warm-up: 200 requests at once")
@@ -460,54 +442,86 @@ def measure_deep(kind: str, steps: int, warmup: int) -> dict:
             **extra}
 
 
-def cpu_baseline(scale: int, threads: int) -> dict | None:
-    """The reference's own best published CPU variants (compiled from its
-    sources into oracle/_ref) on a bounded sample of the workload: the best
-    C++-threads variant of each algorithm (data/*_cpp.csv) and an OpenMP
-    variant: for MIS and TC the best published one on rmat (data/mis_omp.csv,
-    data/tc_omp.csv), for the others the winner of a sweep of all variants
-    (oracle/sweep_cpu.py); the faster of the two counts."""
+def cpu_threads() -> int:
+    """Host threads for the CPU baseline: OMP_NUM_THREADS when it is set (the
+    GPU box sets it to the one GPU's share of the host), else every core."""
+    n = os.cpu_count() or 1
+    try:
+        return max(1, min(n, int(os.environ.get("OMP_NUM_THREADS", "") or n)))
+    except ValueError:
+        return n
+
+
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(scale: int, threads: int, runs: int = 3) -> dict | None:
+    """The reference's own CPU variants (compiled from its sources into
+    oracle/_ref) on the benchmark's own graph: R-MAT `scale`, edge factor 16,
+    seed 1, weights [1, 255], source 0.  Per algorithm the best published
+    C++-threads variant (data/*_cpp.csv) and an OpenMP variant (the best
+    published for MIS and TC, data/mis_omp.csv and data/tc_omp.csv; the
+    winner of the all-variant sweep, oracle/sweep_cpu.py, for the others) are
+    each timed `runs` times, and the faster median counts - the reference
+    harness's own rule, median of 3 (include/bfs_vertex_cpp.h:106-133).  TC
+    is bounded: one run of the OpenMP variant only (its C++-threads variant
+    is several times slower on this graph)."""
     sys.path.insert(0, os.path.join(REPO, "oracle"))
     try:
         import oracle as orc  # checker / baseline only
     except ImportError:
         return None
-    if not all(orc.ref_available(a) for a in ALGS):
+    if not all(orc.ref_available(a) for a in ALGS) or not orc.ref_available("tc_omp"):
         return None
     import indigo2 as ig
 
     g = ig.rmat(scale, 16, seed=1, wmin=1, wmax=255)
-    times = {}
-    log(f"cpu baseline: rmat{scale} V={g.nodes} E={g.edges}, {threads} threads")
-    _, times["bfs"] = orc.ref_bfs(g, 0, threads)
-    _, times["sssp"] = orc.ref_sssp(g, 0, threads)
-    _, times["cc"] = orc.ref_cc(g, threads)
-    _, _, times["mis"] = orc.ref_mis(g, threads)
-    _, _, times["pr"] = orc.ref_pr(g, threads)
-    _, times["tc"] = orc.ref_tc(g, threads)
-    variants = {a: "C++ threads" for a in ALGS}
-    omp = {"bfs": lambda: orc.ref_bfs_omp(g, 0, threads)[-1], "sssp": lambda: orc.ref_sssp_omp(g, 0, threads)[-1],
-           "cc": lambda: orc.ref_cc_omp(g, threads)[-1], "mis": lambda: orc.ref_mis_omp(g, threads)[-1],
-           "pr": lambda: orc.ref_pr_omp(g, threads)[-1], "tc": lambda: orc.ref_tc_omp(g, threads)[-1]}
-    for a, run in omp.items():
-        if orc.ref_available(f"{a}_omp"):
-            t_omp = run()
-            if 0 < t_omp < times[a]:
-                times[a], variants[a] = t_omp, "OpenMP"
+    log(f"cpu baseline: rmat{scale} V={g.nodes} E={g.edges}, {threads} threads on {cpu_model()}")
+    cands = {
+        "bfs": {"C++ threads": lambda: orc.ref_bfs(g, 0, threads)[-1], "OpenMP": lambda: orc.ref_bfs_omp(g, 0, threads)[-1]},
+        "sssp": {"C++ threads": lambda: orc.ref_sssp(g, 0, threads)[-1],
+                 "OpenMP": lambda: orc.ref_sssp_omp(g, 0, threads)[-1]},
+        "cc": {"C++ threads": lambda: orc.ref_cc(g, threads)[-1], "OpenMP": lambda: orc.ref_cc_omp(g, threads)[-1]},
+        "mis": {"C++ threads": lambda: orc.ref_mis(g, threads)[-1], "OpenMP": lambda: orc.ref_mis_omp(g, threads)[-1]},
+        "pr": {"C++ threads": lambda: orc.ref_pr(g, threads)[-1], "OpenMP": lambda: orc.ref_pr_omp(g, threads)[-1]},
+        "tc": {"OpenMP": lambda: orc.ref_tc_omp(g, threads)[-1]},
+    }
+    times, variants, all_runs = {}, {}, {}
+    for a in ALGS:
+        for kind, run in cands[a].items():
+            if kind == "OpenMP" and a != "tc" and not orc.ref_available(f"{a}_omp"):
+                continue
+            ts = [run() for _ in range(1 if a == "tc" else runs)]
+            all_runs[f"{a} {kind}"] = [round(t, 4) for t in ts]
+            med = statistics.median(ts)
+            if med > 0 and (a not in times or med < times[a]):
+                times[a], variants[a] = med, kind
+        log(f"cpu baseline {a}: {times[a]:.3f} s ({variants[a]})")
     per = {a: 1e-9 * g.edges / max(t, 1e-9) for a, t in times.items()}
     total = sum(times.values())
-    log("cpu baseline times " + ", ".join(f"{a}={t:.3f}s" for a, t in times.items()))
     return {
         "value": round(6 * 1e-9 * g.edges / total, 6),
         "unit": "GTEPS",
         "cores": threads,
         "kind": "reference",
-        "sample": f"rmat scale {scale} ef16 (V={g.nodes}, E={g.edges}), one run of each algorithm's best published "
-                  f"C++-threads variant (data/*_cpp.csv) and an OpenMP variant (best published for MIS and TC, "
-                  f"sweep winner for the others), the faster counting; {threads} threads",
+        "sample": f"the benchmark's own graph, rmat{scale} ef16 seed 1 (V={g.nodes}, E={g.edges}), source 0; per "
+                  f"algorithm the faster median of {runs} runs of the best published C++-threads variant and an "
+                  f"OpenMP variant; TC bounded to one run of its OpenMP variant; {threads} threads of "
+                  f"{os.cpu_count()} on {cpu_model()}",
+        "cpu_model": cpu_model(),
+        "host_cpus": os.cpu_count(),
         "variants": variants,
         "per_algorithm_gteps": {a: round(x, 6) for a, x in per.items()},
         "seconds": {a: round(t, 4) for a, t in times.items()},
+        "runs": all_runs,
     }
 
 
@@ -516,7 +530,7 @@ def cpu_baseline_isolated(scale: int) -> dict | None:
     so a crash in it (some of its variants fault on some graphs) costs the
     baseline, not the benchmark.  The child touches no GPU."""
     r = subprocess.run([sys.executable, os.path.abspath(__file__), "--cpu-baseline-only", "--cpu-scale", str(scale)],
-                       stdout=subprocess.PIPE, text=True, timeout=600)
+                       stdout=subprocess.PIPE, text=True, timeout=900)
     lines = r.stdout.strip().splitlines()
     if r.returncode != 0 or not lines:
         log(f"cpu baseline skipped: child exited with status {r.returncode}")
@@ -531,10 +545,13 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--scale", type=int, default=22)
     ap.add_argument("--edge-factor", type=int, default=16)
-    ap.add_argument("--cpu-scale", type=int, default=20)
+    ap.add_argument("--cpu-scale", type=int, default=None,
+                    help="R-MAT scale of the CPU baseline (default: --scale, the benchmark's own graph)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched-vs-plain sub-measurement")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving sub-measurement")
+    ap.add_argument("--no-deep", action="store_true",
+                    help="skip the 1024^2 grid and 4096^2 road-like sub-measurements")
     ap.add_argument("--algs", default=",".join(ALGS), help="subset to run (profiling); the JSON line needs all six")
     ap.add_argument("--rates", default="500,1000,2000", help="serving workload: arrival rates in requests/s")
     ap.add_argument("--slots", type=int, default=4, help="serving workload: concurrent slots")
@@ -553,7 +570,7 @@ def main() -> int:
                     help="print the CPU baseline record alone (the child process of a full run)")
     args = ap.parse_args()
     if args.cpu_baseline_only:
-        print(json.dumps(cpu_baseline(args.cpu_scale, min(16, os.cpu_count() or 1))), flush=True)
+        print(json.dumps(cpu_baseline(args.cpu_scale or args.scale, cpu_threads())), flush=True)
         return 0
     if args.record_traffic:
         workload = f"rmat{args.scale} ef{args.edge_factor}"
@@ -592,7 +609,7 @@ def main() -> int:
     cpu = None
     if not args.no_cpu_baseline:
         try:
-            cpu = cpu_baseline_isolated(args.cpu_scale)
+            cpu = cpu_baseline_isolated(args.cpu_scale or args.scale)
         except Exception as exc:  # the baseline is reported, never required
             log(f"cpu baseline skipped: {exc}")
 
@@ -732,6 +749,13 @@ def main() -> int:
     ctx.close()
     del dg, out_i, out_f
     line["host_transfer"] = host_io
+    if not args.no_deep:
+        for kind in ("grid", "road"):  # the high-diameter configurations, a few seconds each
+            try:
+                line[kind] = measure_deep(kind, 3, 1)
+            except Exception as exc:  # reported beside the main line, never required by it
+                log(f"{kind} measurement skipped: {exc}")
+                line[kind] = None
     if not args.no_batched:
         try:
             line["batched"] = measure_batched()
@@ -739,8 +763,8 @@ def main() -> int:
             log(f"batched measurement skipped: {exc}")
             line["batched"] = None
     if not args.no_serving:
-        if time.time() - t0 > 240:
-            log("serving measurement skipped: the run has already taken over 4 minutes")
+        if time.time() - t0 > 420:
+            log("serving measurement skipped: the run has already taken over 7 minutes")
             line["serving"] = None
         else:
             try:

@@ -3,7 +3,7 @@ from __future__ import annotations
 
 import numpy as np
 
-from .graph import HostGraph, erdos_renyi, rmat
+from .graph import HostGraph, erdos_renyi, grid, rmat
 
 
 def batched_mix(ngraphs: int = 512, seed: int = 2024) -> tuple[list[HostGraph], list[tuple[np.ndarray, np.ndarray]]]:
@@ -36,3 +36,33 @@ def batched_specs(srcs, algs=("bfs", "sssp", "cc", "mis")) -> list[tuple[str, in
             src = b if alg == "bfs" else (s if alg == "sssp" else [])
             out.append((alg, i, [int(x) for x in src]))
     return out
+
+
+def serving_pool(seed: int = 77) -> tuple[list[HostGraph], list[str]]:
+    """The serving configuration's 64 resident graphs: 32 R-MAT (edge factor
+    16, a/b/c/d .57/.19/.19/.05) with scales 14, 15, ..., 22 in turn, and 32
+    square grids with sides 256, 512, 1024, 2048 in turn; weights [1, 255].
+    Returns the graphs and their short names."""
+    gs, names = [], []
+    for i in range(32):
+        sc = 14 + i % 9
+        gs.append(rmat(sc, 16, seed=seed + i, wmin=1, wmax=255))
+        names.append(f"rmat{sc}")
+    for i in range(32):
+        side = 256 << (i % 4)
+        gs.append(grid(side, side, seed=seed + 100 + i, wmin=1, wmax=255))
+        names.append(f"grid{side}")
+    return gs, names
+
+
+def serving_requests(graphs: list[HostGraph], nq: int, seed: int = 5):
+    """`nq` seeded requests over the pool: algorithm mix BFS .4 / SSSP .3 /
+    CC .1 / PageRank .1 / TC .1, graph and source uniform, and unit-mean
+    exponential gaps (divide by the rate for Poisson arrivals).  Returns
+    (algs, graph ids, sources, gaps)."""
+    rng = np.random.default_rng(seed)
+    algs = list(rng.choice(["bfs", "sssp", "cc", "pr", "tc"], size=nq, p=[0.4, 0.3, 0.1, 0.1, 0.1]))
+    gid = rng.integers(0, len(graphs), nq)
+    src = np.array([rng.integers(0, graphs[g].nodes) for g in gid])
+    gaps = rng.exponential(1.0, nq)
+    return algs, gid, src, gaps
