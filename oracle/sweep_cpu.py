@@ -16,6 +16,9 @@ its own timed runs.
 
     python3 oracle/sweep_cpu.py --rmat 16 --threads 8 --csv sweep.csv
     python3 oracle/sweep_cpu.py --egr graph.egr --algs tc --kinds omp
+
+Where the reference sources are absent (the GPU box), --build-only here and
+--prebuilt there run the binaries built beforehand into oracle/_ref/sweep/.
 """
 from __future__ import annotations
 
@@ -38,6 +41,15 @@ KINDS = ("cpp", "omp")
 def variants(ref: str, alg: str, kind: str) -> list[str]:
     d = os.path.join(ref, "codes", kind, f"{alg}-{kind}")
     return sorted(os.path.join(d, f) for f in os.listdir(d) if f.endswith(".cpp")) if os.path.isdir(d) else []
+
+
+def prebuilt(alg: str, kind: str) -> list[str]:
+    """Binaries already built into OUT/<kind>/ for one algorithm."""
+    d = os.path.join(OUT, kind)
+    if not os.path.isdir(d):
+        return []
+    pre = f"{alg}_{kind}_".upper()
+    return sorted(os.path.join(d, f) for f in os.listdir(d) if f.upper().startswith(pre))
 
 
 def compile_one(ref: str, src: str, kind: str) -> tuple[str, str | None]:
@@ -96,25 +108,41 @@ def main() -> int:
     ap.add_argument("--source", type=int, default=0)
     ap.add_argument("--timeout", type=float, default=120.0, help="seconds per variant run")
     ap.add_argument("--csv", default=None, help="write one row per variant")
+    ap.add_argument("--build-only", action="store_true", help="compile every variant into oracle/_ref/sweep and stop")
+    ap.add_argument("--prebuilt", action="store_true",
+                    help="run the binaries already in oracle/_ref/sweep (no reference sources needed)")
     args = ap.parse_args()
-    if shutil.which("g++") is None or not os.path.isdir(os.path.join(args.ref, "codes")):
-        print("needs g++ and the reference sources", file=sys.stderr)
-        return 2
     algs = [a for a in args.algs.split(",") if a]
     kinds = [k for k in args.kinds.split(",") if k]
-    for k in kinds:
-        os.makedirs(os.path.join(OUT, k), exist_ok=True)
+    if args.prebuilt:
+        todo = [(a, k, b) for a in algs for k in kinds for b in prebuilt(a, k)]
+        built = [(b, None) for _, _, b in todo]
+    else:
+        if shutil.which("g++") is None or not os.path.isdir(os.path.join(args.ref, "codes")):
+            print("needs g++ and the reference sources (or --prebuilt)", file=sys.stderr)
+            return 2
+        for k in kinds:
+            os.makedirs(os.path.join(OUT, k), exist_ok=True)
+        todo = [(a, k, s) for a in algs for k in kinds for s in variants(args.ref, a, k)]
+        with ThreadPoolExecutor(args.jobs) as pool:
+            built = list(pool.map(lambda t: compile_one(args.ref, t[2], t[1]), todo))
+        if args.build_only:
+            bad = [(s, e) for (_, _, s), (_, e) in zip(todo, built) if e is not None]
+            print(f"built {len(todo) - len(bad)} of {len(todo)} variants into {OUT}", flush=True)
+            for s, e in bad:
+                print(f"  {os.path.basename(s)}: {e}")
+            return 0
+    if not todo:
+        print("no variants to run", file=sys.stderr)
+        return 2
     egr = os.path.abspath(args.egr) if args.egr else make_graph(args.rmat, args.ef)
-
-    todo = [(a, k, s) for a in algs for k in kinds for s in variants(args.ref, a, k)]
     print(f"{len(todo)} variants, graph {egr}, {args.threads} threads", flush=True)
-    with ThreadPoolExecutor(args.jobs) as pool:
-        built = list(pool.map(lambda t: compile_one(args.ref, t[2], t[1]), todo))
 
     rows, best = [], {}
     env = dict(os.environ, OMP_NUM_THREADS=str(args.threads))
     for (alg, kind, s), (exe, err) in zip(todo, built):
         name = os.path.basename(s)
+        name = name if name.endswith(".cpp") else name + ".cpp"
         gteps, note = None, err
         if err is None:
             t0 = time.time()
