@@ -192,6 +192,17 @@ int indigo_bfs(indigo_ctx* ctx, int nodes, int edges, const int* d_nindex, const
 int indigo_sssp(indigo_ctx* ctx, int nodes, int edges, const int* d_nindex, const int* d_nlist,
                 const int* d_eweight, int src, int* d_dist);
 int indigo_cc(indigo_ctx* ctx, int nodes, int edges, const int* d_nindex, const int* d_nlist, int* d_label);
+/* indigo_cc without any allocation: no counterpart in the reference.  For
+ * callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * result as indigo_cc, but the workspace is a fixed array in device memory
+ * that the library holds on each device until the process ends (about
+ * 64 MiB), so the call makes no hipMalloc / hipFree.  Graphs of at most
+ * INDIGO_CC_NOALLOC_MAX_NODES vertices; larger ones get INDIGO_ERR_CAPACITY.
+ * All calls on one device share that array: calls that overlap on one
+ * device (two contexts, or two host threads) race. */
+#define INDIGO_CC_NOALLOC_MAX_NODES 4194304 /* 2^22 */
+int indigo_cc_noalloc(indigo_ctx* ctx, int nodes, int edges, const int* d_nindex, const int* d_nlist, int* d_label);
 int indigo_mis(indigo_ctx* ctx, int nodes, int edges, const int* d_nindex, const int* d_nlist,
                int signed_priority, int* d_status);
 /* d_scores: device float[nodes]; initialised to 1/nodes by the call. */

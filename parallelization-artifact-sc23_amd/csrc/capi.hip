@@ -279,6 +279,16 @@ int indigo_cc(indigo_ctx* ctx, int nodes, int edges, const int* d_nindex, const 
   return cc_device(ctx, nodes, edges, d_nindex, d_nlist, d_label, nullptr);
 }
 
+static_assert(INDIGO_CC_NOALLOC_MAX_NODES == kCcScratchMaxNodes, "scratch limit of the C header");
+
+int indigo_cc_noalloc(indigo_ctx* ctx, int nodes, int edges, const int* d_nindex, const int* d_nlist, int* d_label)
+{
+  IG_TRY(ctx_prologue(ctx));
+  IG_TRY(check_graph_args(nodes, edges, d_nindex, d_nlist));
+  if (d_label == nullptr) return set_error(INDIGO_ERR_INVALID_ARG, "cc_noalloc: label is NULL");
+  return cc_noalloc_device(ctx, nodes, edges, d_nindex, d_nlist, d_label);
+}
+
 int indigo_cc_vertex(int nodes, int edges, const int* d_nindex, const int* d_nlist, int* h_label, double* seconds)
 {
   indigo_ctx* ctx = nullptr;

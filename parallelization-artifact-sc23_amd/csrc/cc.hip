@@ -35,6 +35,7 @@
 #include "cc.hpp"
 
 #include <climits>
+#include <mutex>
 
 #include "frontier.hpp"
 
@@ -226,8 +227,13 @@ size_t cc_workspace_bytes(int nodes)
 
 int cc_run(indigo_ctx* ctx, int nodes, int edges, const int* nindex, const int* nlist, int* label, int* launches)
 {
-  (void)edges;
-  Carver cv(ctx->ws);
+  return cc_run_in(ctx, ctx->ws, nodes, edges, nindex, nlist, label, launches);
+}
+
+int cc_run_in(indigo_ctx* ctx, void* ws, int nodes, int edges, const int* nindex, const int* nlist, int* label,
+              int* launches)
+{
+  Carver cv(ws);
   CcControl* ctl = cv.take<CcControl>(1);
   int* parent = cv.take<int>(nodes);
   FrontierQueues heavy{nullptr, cv.take<int>(nodes), cv.take<int>(nodes), cv.take<unsigned>(nodes)};
@@ -259,6 +265,63 @@ int cc_run(indigo_ctx* ctx, int nodes, int edges, const int* nindex, const int* 
   IG_HIP_TRY(hipGetLastError());
   if (launches) *launches = afforest ? 8 : 5;
   return INDIGO_OK;
+}
+
+// ---- scratch of the allocation-free entry (indigo_cc_noalloc) ------------------
+// A file-scope device array stands in for the context workspace, so a call
+// makes no allocation.  Each device holds its own copy for the life of the
+// process; host code must use the address hipGetSymbolAddress returns for the
+// current device, never the array's name (a host-side stand-in).  Every call
+// on a device shares that device's copy: cc_run_in clears what must start at
+// zero (the control block) with a kernel and initialises the rest, and two
+// calls that overlap on one device race.
+namespace {
+
+constexpr size_t round256(size_t b) { return (b + 255) & ~static_cast<size_t>(255); }
+constexpr size_t kScratchBytes =
+    round256(sizeof(CcControl)) + 4 * round256(static_cast<size_t>(kCcScratchMaxNodes) * 4) + 4096;
+
+__device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of `scratch` on ctx's device, looked up once per device
+// with that device current.
+int scratch_address(indigo_ctx* ctx, void** out)
+{
+  if (ctx->device < 0 || ctx->device >= kMaxDevices)
+    return set_error(INDIGO_ERR_INVALID_ARG, "cc_noalloc: device %d out of range", ctx->device);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_ptr[ctx->device] == nullptr) {
+    IG_HIP_TRY(hipSetDevice(ctx->device));
+    void* p = nullptr;
+    IG_HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (p == nullptr) return set_error(INDIGO_ERR_DEVICE, "cc_noalloc: scratch has no device address");
+    g_scratch_ptr[ctx->device] = p;
+  }
+  *out = g_scratch_ptr[ctx->device];
+  return INDIGO_OK;
+}
+
+}  // namespace
+
+int cc_noalloc_device(indigo_ctx* ctx, int nodes, int edges, const int* nindex, const int* nlist, int* label)
+{
+  if (nodes > kCcScratchMaxNodes)
+    return set_error(INDIGO_ERR_CAPACITY, "cc_noalloc: %d vertices exceed the scratch limit of %d", nodes,
+                     kCcScratchMaxNodes);
+  if (cc_workspace_bytes(nodes) > kScratchBytes)
+    return set_error(INDIGO_ERR_CAPACITY, "cc_noalloc: workspace of %zu bytes exceeds the %zu-byte scratch",
+                     cc_workspace_bytes(nodes), kScratchBytes);
+  void* ws = nullptr;
+  IG_TRY(scratch_address(ctx, &ws));
+  IG_HIP_TRY(hipStreamSynchronize(ctx->stream));
+  IG_TRY(timing_begin(ctx));
+  int launches = 0;
+  IG_TRY(cc_run_in(ctx, ws, nodes, edges, nindex, nlist, label, &launches));
+  return timing_end(ctx, launches, 1);
 }
 
 }  // namespace indigo

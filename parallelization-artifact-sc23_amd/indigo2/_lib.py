@@ -34,6 +34,8 @@ HINT_TC_LDS_LIST = 6
 HINT_RESIDENCY_MS = 7
 HINT_MAX_WORKGROUPS = 8
 
+CC_NOALLOC_MAX_NODES = 1 << 22
+
 
 class IndigoError(RuntimeError):
     """A C-ABI call returned a negative status."""
@@ -183,6 +185,7 @@ _SIGNATURES = {
     "indigo_mis": [c_void_p, c_int, c_int, c_void_p, c_void_p, c_int, c_void_p],
     "indigo_pagerank": [c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p, _PI],
     "indigo_tc": [c_void_p, c_int, c_int, c_void_p, c_void_p, _PLL],
+    "indigo_cc_noalloc": [c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p],
     # host graph tools
     "indigo_build_csr": [c_int, c_longlong, _PI, _PI, c_int, c_ulonglong, c_int, c_int, c_int, _PHG],
     "indigo_gen_grid": [c_int, c_int, c_ulonglong, c_int, c_int, _PHG],
