@@ -340,7 +340,7 @@ def measure_serving(rates=(1000.0,), nq: int = 10000, seed: int = 5, slots: int 
     the requests' CSR entries / makespan."""
     import indigo2 as ig
 
-    from indigo2.workloads import serving_pool, serving_requests
+    from indigo2.workloads import SERVE_LONG_US, serving_cost_us, serving_pool, serving_requests
 
     t0 = time.time()
     gs, names = serving_pool()
@@ -365,6 +365,13 @@ def measure_serving(rates=(1000.0,), nq: int = 10000, seed: int = 5, slots: int 
             m = np.array([x == a for x in algs])
             per[a] = {"n": int(m.sum()), "p50_ms": round(1e3 * float(np.percentile(lat[m], 50)), 3),
                       "p99_ms": round(1e3 * float(np.percentile(lat[m], 99)), 3)}
+        # by the loop's own cost classes (short requests must not queue behind long ones)
+        long_m = np.array([serving_cost_us(gs[g], a) > SERVE_LONG_US for a, g in zip(algs, gid)])
+        classes = {}
+        for name, m in (("short", ~long_m), ("long", long_m)):
+            if m.any():
+                classes[name] = {"n": int(m.sum()), "p50_ms": round(1e3 * float(np.percentile(lat[m], 50)), 3),
+                                 "p99_ms": round(1e3 * float(np.percentile(lat[m], 99)), 3)}
         runs.append({"rate_qps": rate, "p50_ms": round(1e3 * float(np.percentile(lat, 50)), 3),
                      "p99_ms": round(1e3 * float(np.percentile(lat, 99)), 3),
                      "mean_batch": round(float(np.mean(batch)), 2), "makespan_s": round(st["makespan"], 3),
@@ -372,7 +379,7 @@ def measure_serving(rates=(1000.0,), nq: int = 10000, seed: int = 5, slots: int 
                      "aggregate_gteps": round(1e-9 * edges.sum() / st["makespan"], 3),
                      "launches": st["launches"], "tc_launches": st["tc_launches"],
                      "peak_output_bytes": st["peak_bytes"], "max_waiting": st["max_waiting"],
-                     "per_algorithm": per})
+                     "slots_used": st["slots"], "per_class": classes, "per_algorithm": per})
         log(f"serving at {rate:.0f} q/s: p50 {runs[-1]['p50_ms']} ms, p99 {runs[-1]['p99_ms']} ms, "
             f"{runs[-1]['aggregate_gteps']} GTEPS, mean batch {runs[-1]['mean_batch']}")
     return {"workload": "serving: 64 resident graphs (32 R-MAT ef16 scales 14-22, 32 grids 256^2-2048^2), "

@@ -75,8 +75,10 @@ int indigo_memcpy_d2h(void* h_dst, const void* d_src, size_t bytes);
 int indigo_bfs_vertex(int src, int nodes, int edges, const int* d_nindex, const int* d_nlist,
                       int* h_dist, double* seconds);
 
-/* GPUbfs_edge, include/bfs_edge_cuda.h:61: same output; h_sp is the host
- * edge-source array sp[e] (bfs_edge_cuda.h:140-146), validated for length only. */
+/* GPUbfs_edge, include/bfs_edge_cuda.h:61: same output, computed by the
+ * edge-centric, topology-driven kernel (one lane per CSR entry): h_sp is the
+ * host edge-source array sp[e] (bfs_edge_cuda.h:140-146), copied to the
+ * device before the timer starts; its contents are trusted like the CSR's. */
 int indigo_bfs_edge(int src, int nodes, int edges, const int* d_nindex, const int* d_nlist,
                     const int* h_sp, int* h_dist, double* seconds);
 
@@ -85,7 +87,8 @@ int indigo_bfs_edge(int src, int nodes, int edges, const int* d_nindex, const in
 int indigo_sssp_vertex(int src, int nodes, int edges, const int* d_nindex, const int* d_nlist,
                        const int* d_eweight, int* h_dist, double* seconds);
 
-/* GPUsssp_edge, include/sssp_edge_cuda.h:60 (h_sp host, as in the reference). */
+/* GPUsssp_edge, include/sssp_edge_cuda.h:60: edge-centric Bellman-Ford sweeps
+ * over (h_sp[e], d_nlist[e], d_eweight[e]); h_sp host, as in the reference. */
 int indigo_sssp_edge(int src, int nodes, int edges, const int* d_nindex, const int* d_nlist,
                      const int* d_eweight, const int* h_sp, int* h_dist, double* seconds);
 
@@ -94,7 +97,8 @@ int indigo_sssp_edge(int src, int nodes, int edges, const int* d_nindex, const i
 int indigo_cc_vertex(int nodes, int edges, const int* d_nindex, const int* d_nlist,
                      int* h_label, double* seconds);
 
-/* GPUcc_edge, include/cc_edge_cuda.h:57 (h_sp host). */
+/* GPUcc_edge, include/cc_edge_cuda.h:57: one union-find linking sweep over
+ * the entries (h_sp[e], d_nlist[e]) with h_sp[e] > d_nlist[e] (h_sp host). */
 int indigo_cc_edge(int nodes, int edges, const int* d_nindex, const int* d_nlist,
                    const int* h_sp, int* h_label, double* seconds);
 
@@ -108,7 +112,9 @@ int indigo_cc_edge(int nodes, int edges, const int* d_nindex, const int* d_nlist
 int indigo_mis_vertex(int nodes, int edges, const int* d_nindex, const int* d_nlist,
                       int signed_priority, unsigned* h_priority, int* h_status, double* seconds);
 
-/* GPUmis_edge, include/mis_edge_cuda.h:54 (d_sp device, as in the reference). */
+/* GPUmis_edge, include/mis_edge_cuda.h:54: rounds of edge-centric sweeps
+ * (candidates withdrawn by larger-key neighbours, neighbours of members
+ * excluded) over d_sp (device, as in the reference). */
 int indigo_mis_edge(int nodes, int edges, const int* d_nindex, const int* d_nlist, const int* d_sp,
                     int signed_priority, unsigned* h_priority, int* h_status, double* seconds);
 
@@ -167,7 +173,9 @@ int indigo_ctx_last_grid(indigo_ctx* ctx, int* grid, int* per_cu);
 /* Per-level trace of the last BFS or SSSP call (first 256 levels): start of
  * each level in microseconds since the kernel began, frontier size, its
  * degree sum, and the mode (0 top-down, 1 bottom-up, 2 one-workgroup phase,
- * 3 SSSP far-pile split).  Any output pointer may be NULL. */
+ * 3 SSSP far-pile split).  After an MIS call, one entry per round: its start,
+ * the undecided vertices with short lists and (as degree_sum) those with
+ * long lists.  Any output pointer may be NULL. */
 int indigo_ctx_last_trace(indigo_ctx* ctx, int max, float* us, unsigned* frontier, double* degree_sum, int* mode,
                           int* n);
 /* Release the cached workspace (it regrows on the next call). */
@@ -184,6 +192,14 @@ int indigo_ctx_set_hint(indigo_ctx* ctx, int key, int value);
                                         the GPU is shared with other work */
 #define INDIGO_HINT_MAX_WORKGROUPS 8 /* cap on the workgroups of any persistent grid of the context (0: the
                                         whole GPU); for contexts that share the GPU */
+#define INDIGO_HINT_VARIANT 9        /* INDIGO_VARIANT_*: which form indigo_bfs / _sssp / _cc / _mis run */
+#define INDIGO_HINT_BLOCK_SIZE 10    /* workgroup size of the edge-centric kernels: 256, 512 (0: default) or 1024 */
+#define INDIGO_HINT_GRAPH 11        /* 1: CC and TC run their kernel sequence as a HIP graph, captured on the
+                                        first call with the same graph, output and workspace and replayed after
+                                        (one launch instead of 5-10); such calls report no separate main kernel */
+#define INDIGO_VARIANT_VERTEX 0      /* vertex-centric, data-driven (degree-binned worklists); the default */
+#define INDIGO_VARIANT_EDGE 1        /* edge-centric, topology-driven: one lane per CSR entry over an edge-source
+                                        array built on the device (the suite's E_Topo_Push family) */
 
 /* All resident entries take the device CSR as (nodes, edges, d_nindex,
  * d_nlist[, d_eweight]) and return after the result is in device memory. */
@@ -280,10 +296,12 @@ int indigo_build_sp(indigo_ctx* ctx, int nodes, int edges, const int* d_nindex, 
 /* ---- serving ----------------------------------------------------------------- */
 /* A request loop over resident graphs (no counterpart in the reference).
  * Requests arrive over time; every free slot (a context with its own stream
- * and a share of the GPU) takes the oldest waiting requests that fit the
+ * and a share of the GPU) takes the oldest waiting requests of one cost class
+ * (short or long, from the graph's size and the algorithm) that fit the
  * memory budget, merges BFS / SSSP requests on the same graph into
  * multi-source tasks and runs them as one batched launch, while the other
- * slots keep running theirs.  A triangle count runs alone on its slot, and
+ * slots keep running theirs.  Long work never takes the last free slot, so
+ * short requests are not queued behind long launches.  A triangle count runs alone on its slot, and
  * so does an SSSP request on a deep, sparse graph (average degree below 6,
  * at least 65536 vertices), on the single-query near-far kernel. */
 #define INDIGO_QUERY_TC 6 /* serving only */
@@ -307,7 +325,8 @@ typedef struct indigo_request {
 
 typedef struct indigo_serve_config {
   int device;
-  int slots;         /* concurrent launches, 1..16 */
+  int slots;         /* concurrent launches, 1..16; capped at the process's hardware queues
+                        (GPU_MAX_HW_QUEUES, 4 when unset), beyond which streams serialise */
   int max_batch;     /* requests per launch */
   size_t mem_budget; /* device bytes for the outputs of admitted requests */
   double progress_interval; /* > 0: a progress line on stderr every so many seconds */
@@ -319,6 +338,7 @@ typedef struct indigo_serve_stats {
   int tc_launches;        /* triangle counts */
   size_t peak_bytes;      /* largest output footprint held at once */
   int max_waiting;        /* longest queue of arrived, unadmitted requests */
+  int slots;              /* slots used: cfg.slots capped at the process's hardware queues */
 } indigo_serve_stats;
 
 /* Serves reqs (any order of arrival) in real time and returns when every

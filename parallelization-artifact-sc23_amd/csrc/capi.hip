@@ -9,6 +9,7 @@
 #include "server.hpp"
 #include "cc.hpp"
 #include "device_common.hpp"
+#include "edge.hpp"
 #include "mis.hpp"
 #include "pr.hpp"
 #include "runtime.hpp"
@@ -18,12 +19,6 @@
 using namespace indigo;
 
 namespace {
-
-double wall_seconds()
-{
-  using clock = std::chrono::steady_clock;
-  return std::chrono::duration<double>(clock::now().time_since_epoch()).count();
-}
 
 // Device buffer released on scope exit (the drop-in entries own their
 // temporaries, as the reference's entry functions do).
@@ -65,6 +60,19 @@ int legacy_ctx(indigo_ctx** out)
   if (ctx == nullptr) return rc;
   IG_HIP_TRY(hipSetDevice(ctx->device));
   *out = ctx;
+  return INDIGO_OK;
+}
+
+// INDIGO_HINT_VARIANT selects the edge-centric form for the resident calls.
+bool edge_variant(const indigo_ctx* ctx) { return ctx->hints[INDIGO_HINT_VARIANT] == INDIGO_VARIANT_EDGE; }
+
+// Copies the harness's host edge-source array to the device (the reference's
+// edge forms copy sp before their timer starts, e.g.
+// codes/cuda/bfs-cuda/BFS_CUDA_E_Topo_*.cu).
+int upload_sp(DeviceBuffer& d, const int* h_sp, int edges)
+{
+  IG_TRY(d.alloc(sizeof(int) * static_cast<size_t>(edges > 0 ? edges : 1)));
+  if (edges > 0) IG_HIP_TRY(hipMemcpy(d.p, h_sp, sizeof(int) * static_cast<size_t>(edges), hipMemcpyHostToDevice));
   return INDIGO_OK;
 }
 
@@ -120,6 +128,37 @@ int sssp_device(indigo_ctx* ctx, int nodes, int edges, const int* nindex, const 
   return INDIGO_OK;
 }
 
+// Arguments of a captured kernel sequence (run_captured).
+struct SeqArgs {
+  int nodes, edges;
+  const int* nindex;
+  const int* nlist;
+  int* out;
+  int launches;
+};
+
+unsigned long long seq_key(int alg, const SeqArgs& a)
+{
+  unsigned long long k = key_mix(0, static_cast<unsigned>(alg));
+  k = key_mix(k, static_cast<unsigned>(a.nodes));
+  k = key_mix(k, static_cast<unsigned>(a.edges));
+  k = key_mix(k, reinterpret_cast<uintptr_t>(a.nindex));
+  k = key_mix(k, reinterpret_cast<uintptr_t>(a.nlist));
+  return key_mix(k, reinterpret_cast<uintptr_t>(a.out));
+}
+
+int tc_sequence(indigo_ctx* ctx, void* p)
+{
+  SeqArgs* a = static_cast<SeqArgs*>(p);
+  return tc_run(ctx, a->nodes, a->edges, a->nindex, a->nlist, &a->launches);
+}
+
+int cc_sequence(indigo_ctx* ctx, void* p)
+{
+  SeqArgs* a = static_cast<SeqArgs*>(p);
+  return cc_run(ctx, a->nodes, a->edges, a->nindex, a->nlist, a->out, &a->launches);
+}
+
 int cc_device(indigo_ctx* ctx, int nodes, int edges, const int* nindex, const int* nlist, int* label,
               double* seconds)
 {
@@ -127,9 +166,9 @@ int cc_device(indigo_ctx* ctx, int nodes, int edges, const int* nindex, const in
   IG_HIP_TRY(hipStreamSynchronize(ctx->stream));
   const double t0 = wall_seconds();
   IG_TRY(timing_begin(ctx));
-  int launches = 0;
-  IG_TRY(cc_run(ctx, nodes, edges, nindex, nlist, label, &launches));
-  IG_TRY(timing_end(ctx, launches, 1));
+  SeqArgs a{nodes, edges, nindex, nlist, label, 0};
+  IG_TRY(run_captured(ctx, seq_key(3, a), cc_sequence, &a));
+  IG_TRY(timing_end(ctx, a.launches, 1));
   if (seconds) *seconds = wall_seconds() - t0;
   return INDIGO_OK;
 }
@@ -185,9 +224,9 @@ int tc_device(indigo_ctx* ctx, int nodes, int edges, const int* nindex, const in
   IG_HIP_TRY(hipStreamSynchronize(ctx->stream));
   const double t0 = wall_seconds();
   IG_TRY(timing_begin(ctx));
-  int launches = 0;
-  IG_TRY(tc_run(ctx, nodes, edges, nindex, nlist, &launches));
-  IG_TRY(timing_end(ctx, launches, 1));
+  SeqArgs a{nodes, edges, nindex, nlist, nullptr, 10};
+  IG_TRY(run_captured(ctx, seq_key(6, a), tc_sequence, &a));
+  IG_TRY(timing_end(ctx, a.launches, 1));
   IG_TRY(tc_collect(ctx, count));
   if (seconds) *seconds = wall_seconds() - t0;
   return INDIGO_OK;
@@ -212,6 +251,8 @@ int indigo_bfs(indigo_ctx* ctx, int nodes, int edges, const int* d_nindex, const
   IG_TRY(check_graph_args(nodes, edges, d_nindex, d_nlist));
   IG_TRY(check_src(src, nodes));
   if (d_dist == nullptr) return set_error(INDIGO_ERR_INVALID_ARG, "bfs: dist is NULL");
+  if (edge_variant(ctx))
+    return edge_device(ctx, kEdgeBfs, nodes, edges, d_nindex, d_nlist, nullptr, nullptr, src, 0, d_dist, nullptr);
   return bfs_device(ctx, nodes, edges, d_nindex, d_nlist, src, d_dist, nullptr);
 }
 
@@ -233,8 +274,19 @@ int indigo_bfs_vertex(int src, int nodes, int edges, const int* d_nindex, const 
 int indigo_bfs_edge(int src, int nodes, int edges, const int* d_nindex, const int* d_nlist, const int* h_sp,
                     int* h_dist, double* seconds)
 {
+  indigo_ctx* ctx = nullptr;
+  IG_TRY(legacy_ctx(&ctx));
+  IG_TRY(check_graph_args(nodes, edges, d_nindex, d_nlist));
+  IG_TRY(check_src(src, nodes));
   if (h_sp == nullptr && edges > 0) return set_error(INDIGO_ERR_INVALID_ARG, "bfs_edge: sp is NULL");
-  return indigo_bfs_vertex(src, nodes, edges, d_nindex, d_nlist, h_dist, seconds);
+  if (h_dist == nullptr) return set_error(INDIGO_ERR_INVALID_ARG, "bfs_edge: dist is NULL");
+  DeviceBuffer d, dsp;
+  IG_TRY(d.alloc(sizeof(int) * static_cast<size_t>(nodes)));
+  IG_TRY(upload_sp(dsp, h_sp, edges));
+  IG_TRY(edge_device(ctx, kEdgeBfs, nodes, edges, d_nindex, d_nlist, nullptr, dsp.as<int>(), src, 0, d.as<int>(),
+                     seconds));
+  IG_HIP_TRY(hipMemcpy(h_dist, d.p, sizeof(int) * static_cast<size_t>(nodes), hipMemcpyDeviceToHost));
+  return INDIGO_OK;
 }
 
 int indigo_sssp(indigo_ctx* ctx, int nodes, int edges, const int* d_nindex, const int* d_nlist,
@@ -245,6 +297,8 @@ int indigo_sssp(indigo_ctx* ctx, int nodes, int edges, const int* d_nindex, cons
   IG_TRY(check_src(src, nodes));
   if (d_dist == nullptr) return set_error(INDIGO_ERR_INVALID_ARG, "sssp: dist is NULL");
   if (d_eweight == nullptr && edges > 0) return set_error(INDIGO_ERR_INVALID_ARG, "sssp: eweight is NULL");
+  if (edge_variant(ctx))
+    return edge_device(ctx, kEdgeSssp, nodes, edges, d_nindex, d_nlist, d_eweight, nullptr, src, 0, d_dist, nullptr);
   return sssp_device(ctx, nodes, edges, d_nindex, d_nlist, d_eweight, src, d_dist, nullptr);
 }
 
@@ -267,8 +321,20 @@ int indigo_sssp_vertex(int src, int nodes, int edges, const int* d_nindex, const
 int indigo_sssp_edge(int src, int nodes, int edges, const int* d_nindex, const int* d_nlist,
                      const int* d_eweight, const int* h_sp, int* h_dist, double* seconds)
 {
+  indigo_ctx* ctx = nullptr;
+  IG_TRY(legacy_ctx(&ctx));
+  IG_TRY(check_graph_args(nodes, edges, d_nindex, d_nlist));
+  IG_TRY(check_src(src, nodes));
   if (h_sp == nullptr && edges > 0) return set_error(INDIGO_ERR_INVALID_ARG, "sssp_edge: sp is NULL");
-  return indigo_sssp_vertex(src, nodes, edges, d_nindex, d_nlist, d_eweight, h_dist, seconds);
+  if (h_dist == nullptr) return set_error(INDIGO_ERR_INVALID_ARG, "sssp_edge: dist is NULL");
+  if (d_eweight == nullptr && edges > 0) return set_error(INDIGO_ERR_INVALID_ARG, "sssp_edge: eweight is NULL");
+  DeviceBuffer d, dsp;
+  IG_TRY(d.alloc(sizeof(int) * static_cast<size_t>(nodes)));
+  IG_TRY(upload_sp(dsp, h_sp, edges));
+  IG_TRY(edge_device(ctx, kEdgeSssp, nodes, edges, d_nindex, d_nlist, d_eweight, dsp.as<int>(), src, 0, d.as<int>(),
+                     seconds));
+  IG_HIP_TRY(hipMemcpy(h_dist, d.p, sizeof(int) * static_cast<size_t>(nodes), hipMemcpyDeviceToHost));
+  return INDIGO_OK;
 }
 
 int indigo_cc(indigo_ctx* ctx, int nodes, int edges, const int* d_nindex, const int* d_nlist, int* d_label)
@@ -276,6 +342,8 @@ int indigo_cc(indigo_ctx* ctx, int nodes, int edges, const int* d_nindex, const 
   IG_TRY(ctx_prologue(ctx));
   IG_TRY(check_graph_args(nodes, edges, d_nindex, d_nlist));
   if (d_label == nullptr) return set_error(INDIGO_ERR_INVALID_ARG, "cc: label is NULL");
+  if (edge_variant(ctx))
+    return edge_device(ctx, kEdgeCc, nodes, edges, d_nindex, d_nlist, nullptr, nullptr, 0, 0, d_label, nullptr);
   return cc_device(ctx, nodes, edges, d_nindex, d_nlist, d_label, nullptr);
 }
 
@@ -305,8 +373,17 @@ int indigo_cc_vertex(int nodes, int edges, const int* d_nindex, const int* d_nli
 int indigo_cc_edge(int nodes, int edges, const int* d_nindex, const int* d_nlist, const int* h_sp, int* h_label,
                    double* seconds)
 {
+  indigo_ctx* ctx = nullptr;
+  IG_TRY(legacy_ctx(&ctx));
+  IG_TRY(check_graph_args(nodes, edges, d_nindex, d_nlist));
   if (h_sp == nullptr && edges > 0) return set_error(INDIGO_ERR_INVALID_ARG, "cc_edge: sp is NULL");
-  return indigo_cc_vertex(nodes, edges, d_nindex, d_nlist, h_label, seconds);
+  if (h_label == nullptr) return set_error(INDIGO_ERR_INVALID_ARG, "cc_edge: label is NULL");
+  DeviceBuffer d, dsp;
+  IG_TRY(d.alloc(sizeof(int) * static_cast<size_t>(nodes)));
+  IG_TRY(upload_sp(dsp, h_sp, edges));
+  IG_TRY(edge_device(ctx, kEdgeCc, nodes, edges, d_nindex, d_nlist, nullptr, dsp.as<int>(), 0, 0, d.as<int>(), seconds));
+  IG_HIP_TRY(hipMemcpy(h_label, d.p, sizeof(int) * static_cast<size_t>(nodes), hipMemcpyDeviceToHost));
+  return INDIGO_OK;
 }
 
 int indigo_mis(indigo_ctx* ctx, int nodes, int edges, const int* d_nindex, const int* d_nlist, int signed_priority,
@@ -315,6 +392,9 @@ int indigo_mis(indigo_ctx* ctx, int nodes, int edges, const int* d_nindex, const
   IG_TRY(ctx_prologue(ctx));
   IG_TRY(check_graph_args(nodes, edges, d_nindex, d_nlist));
   if (d_status == nullptr) return set_error(INDIGO_ERR_INVALID_ARG, "mis: status is NULL");
+  if (edge_variant(ctx))
+    return edge_device(ctx, kEdgeMis, nodes, edges, d_nindex, d_nlist, nullptr, nullptr, 0, signed_priority, d_status,
+                       nullptr);
   return mis_device(ctx, nodes, d_nindex, d_nlist, signed_priority, d_status, nullptr);
 }
 
@@ -337,8 +417,19 @@ int indigo_mis_vertex(int nodes, int edges, const int* d_nindex, const int* d_nl
 int indigo_mis_edge(int nodes, int edges, const int* d_nindex, const int* d_nlist, const int* d_sp,
                     int signed_priority, unsigned* h_priority, int* h_status, double* seconds)
 {
+  indigo_ctx* ctx = nullptr;
+  IG_TRY(legacy_ctx(&ctx));
+  IG_TRY(check_graph_args(nodes, edges, d_nindex, d_nlist));
   if (d_sp == nullptr && edges > 0) return set_error(INDIGO_ERR_INVALID_ARG, "mis_edge: sp is NULL");
-  return indigo_mis_vertex(nodes, edges, d_nindex, d_nlist, signed_priority, h_priority, h_status, seconds);
+  if (h_status == nullptr) return set_error(INDIGO_ERR_INVALID_ARG, "mis_edge: status is NULL");
+  DeviceBuffer d;
+  IG_TRY(d.alloc(sizeof(int) * static_cast<size_t>(nodes)));
+  IG_TRY(edge_device(ctx, kEdgeMis, nodes, edges, d_nindex, d_nlist, nullptr, d_sp, 0, signed_priority, d.as<int>(),
+                     seconds));
+  IG_HIP_TRY(hipMemcpy(h_status, d.p, sizeof(int) * static_cast<size_t>(nodes), hipMemcpyDeviceToHost));
+  if (h_priority != nullptr)
+    for (int v = 0; v < nodes; v++) h_priority[v] = mis_hash(static_cast<unsigned>(v) + 712313887u);
+  return INDIGO_OK;
 }
 
 int indigo_pagerank(indigo_ctx* ctx, int nodes, int edges, const int* d_nindex, const int* d_nlist, float* d_scores,

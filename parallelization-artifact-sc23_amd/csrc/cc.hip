@@ -38,6 +38,7 @@
 #include <mutex>
 
 #include "frontier.hpp"
+#include "unionfind.hpp"
 
 namespace indigo {
 
@@ -54,36 +55,6 @@ struct CcControl {
   int giant;               // root of the most frequent component in the sample
   int pad[31];
 };
-
-IG_DEV int find_root(int* parent, int x)
-{
-  int p = ld_relaxed(&parent[x]);
-  while (p != x) {
-    const int gp = ld_relaxed(&parent[p]);
-    if (gp == p) return p;
-    st_relaxed(&parent[x], gp);  // path halving: gp is an ancestor of x
-    x = gp;
-    p = ld_relaxed(&parent[x]);
-  }
-  return x;
-}
-
-IG_DEV void link(int* parent, int a, int b)
-{
-  int ra = find_root(parent, a);
-  int rb = find_root(parent, b);
-  while (ra != rb) {
-    const int hi = ra > rb ? ra : rb;
-    const int lo = ra > rb ? rb : ra;
-    int expected = hi;
-    if (__hip_atomic_compare_exchange_strong(&parent[hi], &expected, lo, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
-                                             __HIP_MEMORY_SCOPE_AGENT))
-      return;
-    // hi stopped being a root meanwhile: climb from where it now points
-    ra = find_root(parent, ra == hi ? expected : ra);
-    rb = find_root(parent, rb == hi ? expected : rb);
-  }
-}
 
 __global__ __launch_bounds__(kCcBlock) void cc_init(int nodes, const int* __restrict__ nindex,
                                                     const int* __restrict__ nlist, int* parent)

@@ -19,6 +19,7 @@
 
 #include "device_common.hpp"
 #include "errors.hpp"
+#include "graphio.hpp"
 #include "runtime.hpp"
 
 using namespace indigo;
@@ -99,6 +100,17 @@ __global__ __launch_bounds__(256) void sp_kernel(const int* __restrict__ nindex,
 
 }  // namespace
 
+int indigo::build_sp_async(indigo_ctx* ctx, int nodes, int edges, const int* d_nindex, int* d_sp)
+{
+  if (edges <= 0) return INDIGO_OK;
+  const int block = 256;
+  const long long want = (static_cast<long long>(edges) + block - 1) / block;
+  const int grid = static_cast<int>(std::min<long long>(want, 32LL * ctx->compute_units));
+  sp_kernel<<<grid, block, 0, ctx->stream>>>(d_nindex, nodes, edges, d_sp);
+  IG_HIP_TRY(hipGetLastError());
+  return INDIGO_OK;
+}
+
 extern "C" {
 
 int indigo_graph_upload(indigo_ctx* ctx, int nodes, int edges, const int* nindex, const int* nlist,
@@ -154,12 +166,8 @@ int indigo_build_sp(indigo_ctx* ctx, int nodes, int edges, const int* d_nindex, 
   IG_HIP_TRY(hipSetDevice(ctx->device));
   if (edges == 0) return INDIGO_OK;
   IG_TRY(timing_begin(ctx));
-  const int block = 256;
-  const long long want = (static_cast<long long>(edges) + block - 1) / block;
-  const int grid = static_cast<int>(std::min<long long>(want, 32LL * ctx->compute_units));
   IG_TRY(kernel_begin(ctx, "sp_kernel"));
-  sp_kernel<<<grid, block, 0, ctx->stream>>>(d_nindex, nodes, edges, d_sp);
-  IG_HIP_TRY(hipGetLastError());
+  IG_TRY(build_sp_async(ctx, nodes, edges, d_nindex, d_sp));
   IG_TRY(kernel_end(ctx));
   return timing_end(ctx, 1, 1);
 }

@@ -28,6 +28,7 @@
 #include "mis.hpp"
 
 #include "frontier.hpp"
+#include "trace.hpp"
 
 namespace indigo {
 
@@ -44,6 +45,7 @@ struct MisControl {
   int rounds;
   int status;
   int pad[30];
+  LevelTrace trace;  // per round: start, undecided short-list and long-list vertices
 };
 
 struct MisArgs {
@@ -149,6 +151,7 @@ __global__ __launch_bounds__(kMisBlock) void mis_kernel(MisArgs a)
   const int tid = blockIdx.x * blockDim.x + threadIdx.x;
   const int stride = gridDim.x * blockDim.x;
   const int lane = lane_id();
+  if (tid == 0) ctl->trace.start = now_ticks();
   for (int i = tid; i < a.nodes; i += stride) st_relaxed(&a.status[i], kUndecided);
   // round 0 long-list worklist: every vertex with >= kMidDeg entries
   {
@@ -175,6 +178,7 @@ __global__ __launch_bounds__(kMisBlock) void mis_kernel(MisArgs a)
     const unsigned n = (r == 0) ? static_cast<unsigned>(a.nodes) : ld_relaxed(&ic->n);
     const unsigned nh = ld_relaxed(&ic->nm);
     if (n == 0 && nh == 0) break;
+    trace_level(&ctl->trace, r, n, nh, 0);
     if (tid == 0) counters_clear(&ctl->cnt[(r + 2) % 3]);
     FrontierCounters* oc = &ctl->cnt[(r + 1) % 3];
     const int* in = a.wl[r & 1];
@@ -271,6 +275,7 @@ int mis_collect(indigo_ctx* ctx, int* rounds)
   IG_TRY(check_abort(ctx, reinterpret_cast<const void*>(mis_kernel), h->bar.abort.v, "mis"));
   if (h->status != 0) return set_error(INDIGO_ERR_TIMEOUT, "mis: persistent kernel stopped (status %d)", h->status);
   if (rounds) *rounds = h->rounds;
+  store_trace(ctx, h->trace);
   return INDIGO_OK;
 }
 

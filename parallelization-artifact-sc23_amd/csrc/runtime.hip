@@ -4,6 +4,7 @@
 #include <cstdio>
 
 #include <algorithm>
+#include <chrono>
 #include <cstring>
 #include <mutex>
 #include <unordered_map>
@@ -23,11 +24,53 @@ __global__ void zero_words_kernel(unsigned int* p, size_t words)
 
 }  // namespace
 
+void drop_captured(indigo_ctx* ctx)
+{
+  for (auto& g : ctx->graphs) (void)hipGraphExecDestroy(g.exec);
+  ctx->graphs.clear();
+}
+
+int run_captured(indigo_ctx* ctx, unsigned long long key, int (*enqueue)(indigo_ctx*, void*), void* arg)
+{
+  if (ctx->hints[INDIGO_HINT_GRAPH] == 0) return enqueue(ctx, arg);
+  key = key_mix(key, reinterpret_cast<uintptr_t>(ctx->ws));
+  key = key_mix(key, ctx->ws_bytes);
+  for (int h = 0; h < 16; h++) key = key_mix(key, static_cast<unsigned>(ctx->hints[h]));
+  for (const auto& g : ctx->graphs)
+    if (g.key == key) {
+      IG_HIP_TRY(hipGraphLaunch(g.exec, ctx->stream));
+      return INDIGO_OK;
+    }
+  IG_HIP_TRY(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+  ctx->capturing = true;
+  const int rc = enqueue(ctx, arg);
+  ctx->capturing = false;
+  hipGraph_t graph = nullptr;
+  const hipError_t e = hipStreamEndCapture(ctx->stream, &graph);
+  if (rc != INDIGO_OK) {
+    if (e == hipSuccess && graph != nullptr) (void)hipGraphDestroy(graph);
+    return rc;
+  }
+  IG_HIP_TRY(e);
+  hipGraphExec_t exec = nullptr;
+  const hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  IG_HIP_TRY(ie);
+  if (ctx->graphs.size() >= 16) {  // keep the 16 most recent sequences
+    (void)hipGraphExecDestroy(ctx->graphs.front().exec);
+    ctx->graphs.erase(ctx->graphs.begin());
+  }
+  ctx->graphs.push_back(indigo_ctx::Captured{key, exec});
+  IG_HIP_TRY(hipGraphLaunch(exec, ctx->stream));
+  return INDIGO_OK;
+}
+
 int ensure_workspace(indigo_ctx* ctx, size_t bytes)
 {
   if (bytes <= ctx->ws_bytes && ctx->ws != nullptr) return INDIGO_OK;
   if (ctx->ws != nullptr) {
     IG_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    drop_captured(ctx);
     IG_HIP_TRY(hipFree(ctx->ws));
     ctx->ws = nullptr;
     ctx->ws_bytes = 0;
@@ -106,6 +149,12 @@ int check_abort(indigo_ctx* ctx, const void* kernel, unsigned abort_code, const 
                    abort_code);
 }
 
+double wall_seconds()
+{
+  using clock = std::chrono::steady_clock;
+  return std::chrono::duration<double>(clock::now().time_since_epoch()).count();
+}
+
 int zero_async(indigo_ctx* ctx, void* p, size_t bytes)
 {
   const size_t words = (bytes + 3) / 4;
@@ -135,6 +184,7 @@ int timing_begin(indigo_ctx* ctx)
 
 int kernel_begin(indigo_ctx* ctx, const char* name)
 {
+  if (ctx->capturing) return INDIGO_OK;  // a replayed graph is timed as a whole
   IG_HIP_TRY(hipEventRecord(ctx->ev_k0, ctx->stream));
   ctx->kernel_name = name;
   return INDIGO_OK;
@@ -142,6 +192,7 @@ int kernel_begin(indigo_ctx* ctx, const char* name)
 
 int kernel_end(indigo_ctx* ctx)
 {
+  if (ctx->capturing) return INDIGO_OK;
   IG_HIP_TRY(hipEventRecord(ctx->ev_k1, ctx->stream));
   return INDIGO_OK;
 }
@@ -314,6 +365,7 @@ int indigo_ctx_destroy(indigo_ctx* ctx)
   if (ctx == nullptr) return INDIGO_OK;
   (void)hipSetDevice(ctx->device);
   (void)hipStreamSynchronize(ctx->stream);
+  drop_captured(ctx);
   if (ctx->ws) (void)hipFree(ctx->ws);
   if (ctx->host_pinned) (void)hipHostFree(ctx->host_pinned);
   for (int b = 0; b < 2; b++) {
@@ -412,6 +464,7 @@ int indigo_ctx_trim(indigo_ctx* ctx)
 {
   if (ctx == nullptr) return set_error(INDIGO_ERR_INVALID_ARG, "ctx_trim: null ctx");
   IG_HIP_TRY(hipStreamSynchronize(ctx->stream));
+  drop_captured(ctx);
   if (ctx->ws) IG_HIP_TRY(hipFree(ctx->ws));
   ctx->ws = nullptr;
   ctx->ws_bytes = 0;

@@ -8,6 +8,7 @@
 #include <cstddef>
 #include <cstdint>
 #include <unordered_map>
+#include <vector>
 
 #include "errors.hpp"
 #include "grid_plan.hpp"
@@ -59,6 +60,15 @@ struct indigo_ctx {
   // grid cap per persistent kernel, learned from grids of this context that
   // did not assemble (check_abort); it ends with the context
   std::unordered_map<const void*, int> grid_caps;
+  // kernel sequences captured into HIP graphs (INDIGO_HINT_GRAPH), keyed by
+  // everything the sequence depends on; they refer to the workspace, so any
+  // change of it destroys them
+  struct Captured {
+    unsigned long long key;
+    hipGraphExec_t exec;
+  };
+  std::vector<Captured> graphs;
+  bool capturing = false;  // a sequence is being captured: no event records
 };
 
 namespace indigo {
@@ -134,6 +144,22 @@ inline long long residency_ticks(const indigo_ctx* ctx)
   return ms > 0 ? ctx->wall_hz / 1000 * ms : ctx->wall_hz / 10;
 }
 
+// Enqueues the kernel sequence `enqueue(ctx)` on ctx->stream.  With
+// INDIGO_HINT_GRAPH set, the sequence is captured into a HIP graph the first
+// time a call with this `key` comes (the key must cover every argument,
+// pointer and hint the sequence depends on) and the graph is replayed
+// afterwards: one launch instead of one per kernel.  The sequence must not
+// synchronise or copy to the host.
+int run_captured(indigo_ctx* ctx, unsigned long long key, int (*enqueue)(indigo_ctx*, void*), void* arg);
+// Destroys the context's captured graphs.
+void drop_captured(indigo_ctx* ctx);
+// Mixes one more word into a graph key.
+inline unsigned long long key_mix(unsigned long long h, unsigned long long v)
+{
+  h ^= v + 0x9E3779B97F4A7C15ull + (h << 6) + (h >> 2);
+  return h * 0xff51afd7ed558ccdull;
+}
+
 // Zeroes `bytes` of device memory with a kernel (no memset node, so the
 // sequence stays safe to capture into a HIP graph).
 int zero_async(indigo_ctx* ctx, void* p, size_t bytes);
@@ -152,6 +178,9 @@ int timing_finish(indigo_ctx* ctx, int launches, int iterations);
 int kernel_begin(indigo_ctx* ctx, const char* name);
 int kernel_end(indigo_ctx* ctx);
 int timing_end(indigo_ctx* ctx, int launches, int iterations);
+
+// Host wall clock in seconds (the drop-in entries' timer).
+double wall_seconds();
 
 // Process-wide default context for the drop-in entries (created lazily on
 // device 0, or on the current device); indigo_shutdown releases it.

@@ -16,9 +16,25 @@
 // are admitted as soon as a slot and memory are free, and their work
 // overlaps on the device.  Completion is polled with events; results are
 // copied to the caller's buffers and the arena space released.
+//
+// Head-of-line blocking.  A launch returns all its results together, so a
+// short request batched with a long one waits for the long one, and short
+// requests queued behind long launches on every slot wait for all of them.
+// Each request therefore gets an estimated cost (est_us: its graph's CSR
+// entries at the algorithm's measured rate, or its level count on deep
+// sparse graphs) and a class: short (at most kLongUs) or long.  A launch
+// holds requests of one class only; long work never occupies the last free
+// slot (one slot stays reserved for short work when there are two or more);
+// and a free slot serves the class of the oldest waiting request, or short
+// work when long work may not take it - short requests overtake long ones,
+// never the other way round.  Slots are capped at the process's hardware
+// queues (GPU_MAX_HW_QUEUES, 4 when unset): more streams than queues share
+// queues and serialise instead of overlapping.
 #include <algorithm>
 #include <chrono>
+#include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <map>
 #include <string>
 #include <thread>
@@ -109,9 +125,14 @@ struct Route {
   size_t bytes;
 };
 
+// Requests estimated to run longer than this (microseconds on the whole
+// GPU) are long.
+constexpr double kLongUs = 2000.0;
+
 struct Slot {
   indigo_ctx* ctx = nullptr;
   int state = 0;  // 0 idle, 1 batch, 2 triangle count, 3 single SSSP
+  bool long_work = false;  // the running launch holds long requests
   std::vector<int> reqs;
   std::vector<Piece> pieces;
   std::vector<indigo_query> queries;
@@ -141,14 +162,48 @@ struct Server {
     return r.alg == INDIGO_QUERY_SSSP && G.nodes >= (1 << 16) && static_cast<long long>(G.edges) < 6LL * G.nodes;
   }
 
+  bool deep(const indigo_request& r) const
+  {
+    const indigo_graph_ref& G = graphs[r.graph];
+    return G.nodes >= (1 << 16) && static_cast<long long>(G.edges) < 6LL * G.nodes;
+  }
+
+  // Estimated microseconds of one request on the whole GPU: CSR entries at
+  // the algorithm's rate on R-MAT scale 22 (BFS 300, SSSP 27, CC 500, MIS
+  // 134, PageRank 18, TC 9 GTEPS), plus a launch; BFS and SSSP on deep
+  // sparse graphs by their level count instead (about 2 sqrt(V) levels from
+  // a random source at 10 and 20 us per level).
+  double est_us(const indigo_request& r) const
+  {
+    const indigo_graph_ref& G = graphs[r.graph];
+    const double e = static_cast<double>(G.edges);
+    if (deep(r) && (r.alg == INDIGO_QUERY_BFS || r.alg == INDIGO_QUERY_SSSP)) {
+      const double levels = 2.0 * std::sqrt(static_cast<double>(G.nodes));
+      return levels * (r.alg == INDIGO_QUERY_BFS ? 10.0 : 20.0);
+    }
+    double gteps = 300.0;
+    switch (r.alg) {
+      case INDIGO_QUERY_SSSP: gteps = 27.0; break;
+      case INDIGO_QUERY_CC: gteps = 500.0; break;
+      case INDIGO_QUERY_MIS: gteps = 134.0; break;
+      case INDIGO_QUERY_PR: gteps = 18.0; break;
+      case INDIGO_QUERY_TC: gteps = 9.0; break;
+      default: break;
+    }
+    return 50.0 + e / (gteps * 1e3);
+  }
+
+  bool is_long(const indigo_request& r) const { return est_us(r) > kLongUs; }
+
   size_t out_bytes(const indigo_request& r) const
   {
     const size_t v = static_cast<size_t>(graphs[r.graph].nodes);
     return r.alg == INDIGO_QUERY_PR ? 4 * (v + 1) : 4 * v;
   }
 
-  // Takes waiting requests into one batched launch on slot s; false if none fit.
-  int dispatch_batch(Slot& s, bool* launched)
+  // Takes waiting requests of one class (long or short) into one batched
+  // launch on slot s; false if none fit.
+  int dispatch_batch(Slot& s, bool want_long, bool* launched)
   {
     *launched = false;
     struct Build {
@@ -162,9 +217,9 @@ struct Server {
     int count = 0;
     for (size_t i = 0; i < waiting.size() && count < cfg.max_batch; i++) {
       const indigo_request& r = reqs[waiting[i]];
-      if (r.alg == INDIGO_QUERY_TC || single(r)) continue;
+      if (r.alg == INDIGO_QUERY_TC || single(r) || is_long(r) != want_long) continue;
       const size_t b = round_up(out_bytes(r));
-      if (need + b > arena.free_bytes()) break;  // keep arrival order: no overtaking
+      if (need + b > arena.free_bytes()) break;  // within a class, arrival order: no overtaking
       const bool multi = r.alg == INDIGO_QUERY_BFS || r.alg == INDIGO_QUERY_SSSP;
       const int cap = r.alg == INDIGO_QUERY_BFS ? INDIGO_MAX_BFS_SOURCES : INDIGO_MAX_SSSP_SOURCES;
       auto it = multi ? open.find({r.alg, r.graph}) : open.end();
@@ -222,6 +277,7 @@ struct Server {
     IG_TRY(batch_launch(s.ctx, graphs, ngraphs, s.queries.data(), static_cast<int>(s.queries.size()),
                         grid_per_slot));
     s.state = 1;
+    s.long_work = want_long;
     stats.launches++;
     *launched = true;
     return INDIGO_OK;
@@ -242,6 +298,7 @@ struct Server {
     IG_TRY(timing_record_stop(s.ctx));
     s.tc_req = q;
     s.state = 2;
+    s.long_work = is_long(reqs[q]);
     stats.tc_launches++;
     return INDIGO_OK;
   }
@@ -263,6 +320,7 @@ struct Server {
     s.routes.assign(1, Route{q, arena.ptr(off), out_bytes(r)});
     s.reqs.assign(1, q);
     s.single_req = q;
+    s.long_work = is_long(r);
     IG_TRY(ensure_workspace(s.ctx, sssp_workspace_bytes(G.nodes)));
     IG_TRY(ensure_host_pinned(s.ctx, 4096));
     IG_TRY(timing_begin(s.ctx));
@@ -355,20 +413,30 @@ struct Server {
       }
       for (Slot& s : slots) {
         if (s.state != 0 || waiting.empty()) continue;
-        if (reqs[waiting.front()].alg == INDIGO_QUERY_TC) {
-          IG_TRY(dispatch_tc(s, waiting.front()));
+        // the request to serve: the oldest one, unless it is long and long
+        // work may not take this slot - then the oldest short one
+        int busy_long = 0;
+        for (const Slot& o : slots) busy_long += o.state != 0 && o.long_work;
+        const bool long_ok = slots.size() < 2 || busy_long < static_cast<int>(slots.size()) - 1;
+        int pick = -1;
+        for (size_t i = 0; i < waiting.size() && pick < 0; i++)
+          if (long_ok || !is_long(reqs[waiting[i]])) pick = waiting[i];
+        if (pick < 0) break;  // only long work waits, and it may not take another slot
+        const indigo_request& head = reqs[pick];
+        if (head.alg == INDIGO_QUERY_TC) {
+          IG_TRY(dispatch_tc(s, pick));
           progressed = true;
           continue;
         }
-        if (single(reqs[waiting.front()])) {
+        if (single(head)) {
           bool launched = false;
-          IG_TRY(dispatch_single(s, waiting.front(), &launched));
+          IG_TRY(dispatch_single(s, pick, &launched));
           if (!launched) break;  // its output does not fit until something completes
           progressed = true;
           continue;
         }
         bool launched = false;
-        IG_TRY(dispatch_batch(s, &launched));
+        IG_TRY(dispatch_batch(s, is_long(head), &launched));
         if (!launched) break;  // nothing fits until something completes
         progressed = true;
       }
@@ -402,7 +470,14 @@ int serve_run(const indigo_graph_ref* graphs, int ngraphs, indigo_request* reqs,
   sv.nreqs = nreqs;
   sv.cfg = *cfg;
   IG_TRY(sv.arena.init(cfg->mem_budget));
-  sv.slots.resize(static_cast<size_t>(cfg->slots));
+  int hw_queues = 4;  // HIP's default hardware queues per process
+  if (const char* q = std::getenv("GPU_MAX_HW_QUEUES")) {
+    const int n = std::atoi(q);
+    if (n > 0) hw_queues = n;
+  }
+  const int nslots = std::min(cfg->slots, hw_queues);
+  sv.slots.resize(static_cast<size_t>(nslots));
+  sv.stats.slots = nslots;
   int rc = INDIGO_OK;
   for (Slot& s : sv.slots) {
     rc = indigo_ctx_create(cfg->device, &s.ctx);
@@ -423,7 +498,7 @@ int serve_run(const indigo_graph_ref* graphs, int ngraphs, indigo_request* reqs,
     if (rc == INDIGO_OK && s.ctx != nullptr) rc = ensure_workspace(s.ctx, std::max(ws, size_t(1) << 28));
   int limit = 0;
   if (rc == INDIGO_OK) rc = batch_grid_limit(sv.slots[0].ctx, &limit);
-  sv.grid_per_slot = std::max(1, limit / cfg->slots);
+  sv.grid_per_slot = std::max(1, limit / nslots);
   // single-query persistent kernels take the slot's share of the GPU too: a
   // grid sized for the whole GPU cannot assemble beside the other slots
   for (Slot& s : sv.slots)

@@ -33,6 +33,11 @@ HINT_BLOCKS_PER_CU = 5
 HINT_TC_LDS_LIST = 6
 HINT_RESIDENCY_MS = 7
 HINT_MAX_WORKGROUPS = 8
+HINT_VARIANT = 9
+HINT_BLOCK_SIZE = 10
+HINT_GRAPH = 11
+VARIANT_VERTEX = 0
+VARIANT_EDGE = 1
 
 CC_NOALLOC_MAX_NODES = 1 << 22
 
@@ -127,7 +132,7 @@ class ServeConfigStruct(ctypes.Structure):
 
 class ServeStatsStruct(ctypes.Structure):
     _fields_ = [("makespan", c_double), ("launches", c_int), ("tc_launches", c_int), ("peak_bytes", c_size_t),
-                ("max_waiting", c_int)]
+                ("max_waiting", c_int), ("slots", c_int)]
 
 
 _PI = POINTER(c_int)

@@ -231,7 +231,7 @@ def serve(graphs: list[DeviceGraph], algs, graph_ids, sources, arrivals, outs=No
     batch = np.array([rq[i].batch for i in range(n)])
     slot = np.array([rq[i].slot for i in range(n)])
     stats = {"makespan": st.makespan, "launches": st.launches, "tc_launches": st.tc_launches,
-             "peak_bytes": st.peak_bytes, "max_waiting": st.max_waiting}
+             "peak_bytes": st.peak_bytes, "max_waiting": st.max_waiting, "slots": st.slots}
     return admitted, finished, batch, slot, stats
 
 

@@ -66,3 +66,19 @@ def serving_requests(graphs: list[HostGraph], nq: int, seed: int = 5):
     src = np.array([rng.integers(0, graphs[g].nodes) for g in gid])
     gaps = rng.exponential(1.0, nq)
     return algs, gid, src, gaps
+
+
+# Rates (GTEPS on R-MAT scale 22, whole GPU) of the serving loop's cost model.
+_SERVE_GTEPS = {"bfs": 300.0, "sssp": 27.0, "cc": 500.0, "mis": 134.0, "pr": 18.0, "tc": 9.0}
+SERVE_LONG_US = 2000.0
+
+
+def serving_cost_us(g: HostGraph, alg: str) -> float:
+    """The serving loop's estimate of one request's microseconds on the
+    whole GPU (the same rule as indigo_serve's admission): CSR entries at the
+    algorithm's rate, or 2 sqrt(V) levels at 10 / 20 us for BFS / SSSP on
+    deep sparse graphs (average degree below 6, at least 65536 vertices).
+    Requests above SERVE_LONG_US are long."""
+    if g.nodes >= (1 << 16) and g.edges < 6 * g.nodes and alg in ("bfs", "sssp"):
+        return 2.0 * float(np.sqrt(g.nodes)) * (10.0 if alg == "bfs" else 20.0)
+    return 50.0 + g.edges / (_SERVE_GTEPS[alg] * 1e3)
