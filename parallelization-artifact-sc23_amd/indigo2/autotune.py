@@ -10,16 +10,30 @@ diameter).
 The knobs stand for the suite's variant axes (the style tokens of
 analysis_median.ipynb, cell 1, and the file names under codes/cuda/*):
 
+  V / E and Topo / Data
+                       INDIGO_HINT_VARIANT: the vertex-centric data-driven
+                       kernels (degree-binned worklists) or the edge-centric
+                       topology-driven ones (one lane per CSR entry, every
+                       entry in every sweep; BFS, SSSP, CC, MIS)
   Push / Pull          INDIGO_HINT_BFS_DIRECTION: 1 top-down (push) only,
                        0 direction-optimising (push and pull)
-  Topo / Data          INDIGO_HINT_SSSP_DELTA: the near/far split width of
+  Data worklist width  INDIGO_HINT_SSSP_DELTA: the near/far split width of
                        the data-driven SSSP worklist
   Thread / Warp / Block
                        the gang size of a batched task (workgroups per task)
                        and INDIGO_HINT_SMALL_FRONTIER (levels run by one
                        workgroup while the frontier is small)
+  ThreadsPerBlock      INDIGO_HINT_BLOCK_SIZE: 256, 512 or 1024 threads per
+                       workgroup of the edge-centric kernels (the suite's
+                       ThreadsPerBlock, include/pr_cuda.h:44)
   Persist (grid size)  INDIGO_HINT_BLOCKS_PER_CU: resident workgroups per CU
                        of the persistent grid
+
+Priors come from the suite's own measurements: the best published CUDA
+variant per algorithm on its road-like grid (2d-2e20) and on its R-MAT
+(rmat22) (data/*_cuda_32.csv), mapped onto these
+knobs (PRIORS).  Each algorithm's prior is measured first, and another
+setting replaces it only when it is faster by the required margin.
 
 Every candidate is checked against the default configuration's output
 (bit-exact; PageRank to 1e-6 per vertex), so a tuned setting can change
@@ -48,11 +62,39 @@ from .kernels import Context, Query, batch_gang
 DEFAULT_DB = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuned_mi355x.json")
 
 HINTS = {"BFS_DIRECTION": _lib.HINT_BFS_DIRECTION, "SSSP_DELTA": _lib.HINT_SSSP_DELTA,
-         "SMALL_FRONTIER": _lib.HINT_SMALL_FRONTIER, "BLOCKS_PER_CU": _lib.HINT_BLOCKS_PER_CU}
+         "SMALL_FRONTIER": _lib.HINT_SMALL_FRONTIER, "BLOCKS_PER_CU": _lib.HINT_BLOCKS_PER_CU,
+         "VARIANT": _lib.HINT_VARIANT, "BLOCK_SIZE": _lib.HINT_BLOCK_SIZE}
 # hints every single-query tuning resets before it applies its own
 _TUNED_HINTS = tuple(HINTS.values())
 
 BATCH_ALGS = ("bfs", "sssp", "cc", "mis")
+
+# The suite's best published variant per algorithm and graph shape, mapped
+# onto the knobs above: (hints, the variant, its row in the reference data).
+# Shapes: "deep" is the road-like 2d-2e20 grid, "skewed" the R-MAT rmat22.
+EDGE = {"VARIANT": _lib.VARIANT_EDGE}
+PRIORS = {
+    "deep": {
+        "bfs": ({}, "V_Data_Push_NonDeterm_RMW_Persist_Warp_Dup", "data/bfs_cuda_32.csv:772"),
+        "sssp": ({}, "V_Data_Push_NonDeterm_RW_Persist_Warp_NonDup", "data/sssp_cuda_32.csv:1168"),
+        "cc": (EDGE, "E_Topo_Push_NonDeterm_RW_Thread", "data/cc_cuda_32.csv:1370"),
+        "mis": ({}, "V_Topo_Push_NonDeterm_NonPersist_Thread", "data/mis_cuda_32.csv:140"),
+        "pr": ({}, "Pull_Persist_Thread_Determ_Reduction", "data/pr_cuda_32.csv:184"),
+    },
+    "skewed": {
+        "bfs": ({}, "V_Topo_Push_Determ_RW_NonPersist_Thread", "data/bfs_cuda_32.csv:290"),
+        "sssp": ({}, "V_Data_Push_NonDeterm_RW_Persist_Warp_NonDup", "data/sssp_cuda_32.csv:1456"),
+        "cc": ({}, "V_Data_Push_NonDeterm_RW_NonPersist_Warp_NonDup", "data/cc_cuda_32.csv:138"),
+        "mis": ({}, "V_Topo_Push_NonDeterm_NonPersist_Thread", "data/mis_cuda_32.csv:364"),
+        "pr": ({}, "Pull_NonPersist_Thread_Determ_Reduction", "data/pr_cuda_16.csv:76"),
+    },
+}
+
+
+def prior(alg: str, klass: str) -> tuple[dict, str, str]:
+    """(hints, reference variant, data row) of the suite's best published
+    variant for this algorithm on graphs of this shape ({} when none)."""
+    return PRIORS.get(klass.split("-")[0], {}).get(alg, ({}, "", ""))
 GANG_SCALES = (0.25, 0.5, 2.0, 4.0)
 
 
@@ -157,8 +199,11 @@ def scaled_gang(alg: str, nodes: int, edges: int, scale: float) -> int:
 
 # ---- single-query tuning ----------------------------------------------------
 def _space(alg: str, g: HostGraph) -> list[dict]:
-    """Candidate hint settings of one algorithm ({} is the default)."""
+    """Candidate hint settings of one algorithm ({} is the default): the
+    vertex-centric kernels over their knobs, and the edge-centric kernels at
+    each workgroup size."""
     blocks = [0, 1, 2, 4]
+    edge = [dict(EDGE, **({"BLOCK_SIZE": b} if b else {})) for b in (0, 256, 1024)]
     if alg == "bfs":
         axes = {"BFS_DIRECTION": [0, 1], "SMALL_FRONTIER": [0, 1], "BLOCKS_PER_CU": blocks}
     elif alg == "sssp":
@@ -167,12 +212,17 @@ def _space(alg: str, g: HostGraph) -> list[dict]:
         auto = mean_w * 32.0 / max(1.0, g.edges / max(1, g.nodes))
         deltas = [0] + sorted({max(1, int(round(auto * m))) for m in (0.25, 0.5, 2.0, 4.0)})
         axes = {"SSSP_DELTA": deltas, "BLOCKS_PER_CU": blocks}
-    elif alg in ("mis", "pr"):
+    elif alg == "mis":
         axes = {"BLOCKS_PER_CU": blocks}
+    elif alg == "cc":
+        axes = {}
+    elif alg == "pr":
+        return [{n: v for n, v in (("BLOCKS_PER_CU", b),) if v} for b in blocks]
     else:
         return [{}]
     names = list(axes)
-    return [{n: v for n, v in zip(names, vals) if v != 0} for vals in itertools.product(*(axes[n] for n in names))]
+    vertex = [{n: v for n, v in zip(names, vals) if v != 0} for vals in itertools.product(*(axes[n] for n in names))]
+    return vertex + edge
 
 
 def _run_single(ctx: Context, alg: str, dg: DeviceGraph, src: int):
@@ -180,6 +230,8 @@ def _run_single(ctx: Context, alg: str, dg: DeviceGraph, src: int):
         return ctx.bfs(dg, src)
     if alg == "sssp":
         return ctx.sssp(dg, src)
+    if alg == "cc":
+        return ctx.cc(dg)
     if alg == "mis":
         return ctx.mis(dg)
     return ctx.pagerank(dg)[0]
@@ -191,19 +243,22 @@ def _same(alg: str, a: np.ndarray, b: np.ndarray) -> bool:
     return bool(np.array_equal(a, b))
 
 
-def tune_single(ctx: Context, g: HostGraph, dg: DeviceGraph, klass: str, algs=("bfs", "sssp", "mis", "pr"),
-                reps: int = 3, src: int = 0) -> dict:
+def tune_single(ctx: Context, g: HostGraph, dg: DeviceGraph, klass: str, algs=("bfs", "sssp", "cc", "mis", "pr"),
+                reps: int = 3, src: int = 0, gain: float = 0.03) -> dict:
     """Times every candidate of each algorithm (median of `reps` calls after
-    one warm-up; ctx.last_ms, the device time of the call) and keeps the
-    fastest whose output equals the default's.  A candidate must beat the
-    default by 3% to be kept."""
+    one warm-up; ctx.last_ms, the device time of the call), the prior first,
+    and keeps the prior unless another candidate whose output equals the
+    default's beats it by `gain`."""
     out = {}
     for alg in algs:
         for k in _TUNED_HINTS:
             ctx.set_hint(k, 0)
         ref = _run_single(ctx, alg, dg, src)
+        p_hints, p_variant, p_row = prior(alg, klass)
+        space = _space(alg, g)
+        cands = [p_hints] + [c for c in space if c != p_hints]
         results = []
-        for cand in _space(alg, g):
+        for cand in cands:
             for k in _TUNED_HINTS:
                 ctx.set_hint(k, 0)
             for name, val in cand.items():
@@ -218,10 +273,12 @@ def tune_single(ctx: Context, g: HostGraph, dg: DeviceGraph, klass: str, algs=("
             results.append((statistics.median(ts), cand))
             log(f"{klass} {alg} {cand or 'default'}: {results[-1][0]:.3f} ms")
         default_ms = next(t for t, c in results if not c)
+        prior_ms = results[0][0]
         best_ms, best = min(results, key=lambda r: r[0])
-        if best_ms > 0.97 * default_ms:
-            best_ms, best = default_ms, {}
+        if best_ms > (1.0 - gain) * prior_ms:
+            best_ms, best = prior_ms, p_hints
         out[alg] = {"hints": best, "ms": round(best_ms, 4), "default_ms": round(default_ms, 4),
+                    "prior": {"hints": p_hints, "ms": round(prior_ms, 4), "variant": p_variant, "source": p_row},
                     "candidates": len(results)}
     for k in _TUNED_HINTS:
         ctx.set_hint(k, 0)
