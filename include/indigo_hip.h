@@ -330,6 +330,11 @@ typedef struct indigo_serve_config {
   int max_batch;     /* requests per launch */
   size_t mem_budget; /* device bytes for the outputs of admitted requests */
   double progress_interval; /* > 0: a progress line on stderr every so many seconds */
+  /* must be 0.  1 would give the last slot compute units of its own (a
+   * CU-masked stream); that mode is switched off and refused with
+   * INDIGO_ERR_INVALID_ARG: a run with it ended in a host segmentation fault
+   * whose cause is not established */
+  int cu_partition;
 } indigo_serve_config;
 
 typedef struct indigo_serve_stats {

@@ -520,6 +520,8 @@ int indigo_serve(const indigo_graph_ref* graphs, int ngraphs, indigo_request* re
   clear_error();
   if (cfg == nullptr || cfg->slots < 1 || cfg->slots > 16 || cfg->max_batch < 1 || cfg->mem_budget == 0)
     return set_error(INDIGO_ERR_INVALID_ARG, "serve: bad configuration");
+  if (cfg->cu_partition != 0)
+    return set_error(INDIGO_ERR_INVALID_ARG, "serve: the CU partition between slots is switched off");
   if (nreqs < 0 || ngraphs < 1 || graphs == nullptr || (nreqs > 0 && reqs == nullptr))
     return set_error(INDIGO_ERR_INVALID_ARG, "serve: bad graph / request table");
   for (int g = 0; g < ngraphs; g++) IG_TRY(check_graph_args(graphs[g].nodes, graphs[g].edges, graphs[g].d_nindex,

@@ -64,14 +64,24 @@ def _check_alone(ig, orc, ctx, gs, dgs, algs, gid, src, outs):
             assert np.array_equal(got, want), (i, alg, g.name)
 
 
-def test_serve_mix_equals_alone(ig, orc, ctx):
+@pytest.mark.parametrize("cu_partition", [
+    False,
+    pytest.param(True, marks=pytest.mark.skip(
+        reason="CU-partitioned slots (CU-masked streams) are switched off: a run of this case ended in a host "
+               "segmentation fault inside indigo_serve whose cause is not established")),
+])
+def test_serve_mix_equals_alone(ig, orc, ctx, cu_partition):
+    """Every slot may use every CU; with cu_partition the last slot would get
+    CUs of its own (CU-masked stream) and serve short requests only."""
     rng = np.random.default_rng(1)
     gs = _pool(ig)
     dgs = [ig.DeviceGraph(g) for g in gs]
     algs, gid, src, arrivals = _requests(rng, gs, 400, rate=20000.0)
     outs = [_out(ig, a, gs[g]) for a, g in zip(algs, gid)]
-    print("serving 400 requests on 4 slots", flush=True)
-    adm, fin, batch, slot, st = ig.serve(dgs, algs, gid, src, arrivals, outs, slots=4, max_batch=64, progress=5.0)
+    print(f"serving 400 requests on 4 slots, cu_partition={cu_partition}", flush=True)
+    adm, fin, batch, slot, st = ig.serve(dgs, algs, gid, src, arrivals, outs, slots=4, max_batch=64, progress=5.0,
+                                         cu_partition=cu_partition)
+    assert st["slots"] == 4
     assert np.all(adm >= arrivals - 1e-9) and np.all(fin >= adm)
     assert batch.max() > 1  # requests were batched together
     assert len(set(slot.tolist())) > 1  # and spread over slots
@@ -96,6 +106,14 @@ def test_serve_under_tight_memory_budget(ig, orc, ctx):
     assert st["peak_bytes"] <= budget
     assert st["max_waiting"] > 3  # admission held requests back
     _check_alone(ig, orc, ctx, gs, dgs, algs, gid, src, outs)
+
+
+def test_serve_refuses_the_cu_partition(ig, ctx):
+    g = ig.rmat(10, 8, seed=1, wmin=1, wmax=255)
+    dg = ig.DeviceGraph(g)
+    with pytest.raises(ig.IndigoError) as e:
+        ig.serve([dg], ["bfs"], [0], [0], [0.0], None, cu_partition=True)
+    assert e.value.code == ig._lib.ERR_INVALID_ARG
 
 
 def test_serve_rejects_what_cannot_fit(ig, ctx):
