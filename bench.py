@@ -141,10 +141,11 @@ def kernel_bytes(alg: str, g, pr_iters: int, tc_model: dict) -> float:
     return tc_model["bytes"]
 
 
-# one main kernel per algorithm for the traffic record (CC's timed span is a
-# sequence of kernels: its traffic is not recorded)
+# one main kernel per algorithm for the traffic record; CC's timed span is its
+# whole kernel sequence, recorded as the sum over its kernels per call
 MAIN_KERNELS = {"bfs": "bfs_kernel", "sssp": "sssp_kernel", "mis": "mis_kernel", "pr": "pr_kernel",
-                "tc": "tc_count_group<512, 8192>"}
+                "tc": "tc_count_group<512, 8192>", "cc": "cc_init..cc_flatten"}
+CC_SEQUENCE = ("cc_init", "cc_sample", "cc_compress", "cc_giant", "cc_link", "cc_link_heavy", "cc_flatten")
 
 
 def read_traffic(path: str, workload: str, kernel: str) -> dict | None:
@@ -161,12 +162,25 @@ def read_traffic(path: str, workload: str, kernel: str) -> dict | None:
     return dict(rec["kernels"][kernel], source=rec.get("source"))
 
 
-def _pmc_values(path: str, counter: str, match) -> list[float]:
+def _pmc_rows(path: str, counter: str, match) -> list[tuple[str, float]]:
     import csv
 
     with open(path) as f:
-        return [float(r["Counter_Value"]) for r in csv.DictReader(f)
+        return [(r["Kernel_Name"], float(r["Counter_Value"])) for r in csv.DictReader(f)
                 if r["Counter_Name"] == counter and match(r["Kernel_Name"])]
+
+
+def _pmc_values(path: str, counter: str, match) -> list[float]:
+    return [v for _, v in _pmc_rows(path, counter, match)]
+
+
+def _per_call_sum(rows: list[tuple[str, float]]) -> tuple[float, int]:
+    """Sum over distinct kernels of each kernel's mean value per dispatch
+    (one call of a kernel sequence), and the dispatches seen."""
+    by = {}
+    for name, v in rows:
+        by.setdefault(name, []).append(v)
+    return sum(statistics.mean(v) for v in by.values()), len(rows)
 
 
 def record_traffic(paths: list[str], workload: str, out: str) -> int:
@@ -186,16 +200,18 @@ def record_traffic(paths: list[str], workload: str, out: str) -> int:
     ffac, wfac = gib_kib / statistics.median(cfetch), gib_kib / statistics.median(cwrite)
     kernels = {}
     for alg, kernel in MAIN_KERNELS.items():
-        # plain or templated symbol (the kernel may be named without its template arguments)
-        match = lambda name, k=kernel: f"::{k}(" in name or f"::{k}<" in name
-        fetch, write = _pmc_values(kf, "FETCH_SIZE", match), _pmc_values(kw, "WRITE_SIZE", match)
-        if not (fetch and write):
+        if alg == "cc":
+            match = lambda name: any(f"::{k}(" in name or f"::{k}<" in name for k in CC_SEQUENCE)
+        else:  # plain or templated symbol (the kernel may be named without its template arguments)
+            match = lambda name, k=kernel: f"::{k}(" in name or f"::{k}<" in name
+        fetch, nf = _per_call_sum(_pmc_rows(kf, "FETCH_SIZE", match))
+        write, nw = _per_call_sum(_pmc_rows(kw, "WRITE_SIZE", match))
+        if not (nf and nw):
             continue
-        rb = 1024.0 * statistics.mean(fetch) * ffac
-        wb = 1024.0 * statistics.mean(write) * wfac
+        rb = 1024.0 * fetch * ffac
+        wb = 1024.0 * write * wfac
         kernels[kernel] = {"algorithm": alg, "bytes_per_launch": rb + wb, "read_bytes": rb, "write_bytes": wb,
-                           "fetch_kib_per_dispatch": statistics.mean(fetch),
-                           "write_kib_per_dispatch": statistics.mean(write), "dispatches": [len(fetch), len(write)]}
+                           "fetch_kib_per_dispatch": fetch, "write_kib_per_dispatch": write, "dispatches": [nf, nw]}
     if not kernels:
         print("record-traffic: no main-kernel dispatches in the CSVs", file=sys.stderr)
         return 1
@@ -368,10 +384,15 @@ def measure_serving(rates=(1000.0,), nq: int = 10000, seed: int = 5, slots: int 
         # by the loop's own cost classes (short requests must not queue behind long ones)
         long_m = np.array([serving_cost_us(gs[g], a) > SERVE_LONG_US for a, g in zip(algs, gid)])
         classes = {}
+        wait, service = adm - arrivals, fin - adm
         for name, m in (("short", ~long_m), ("long", long_m)):
             if m.any():
                 classes[name] = {"n": int(m.sum()), "p50_ms": round(1e3 * float(np.percentile(lat[m], 50)), 3),
-                                 "p99_ms": round(1e3 * float(np.percentile(lat[m], 99)), 3)}
+                                 "p99_ms": round(1e3 * float(np.percentile(lat[m], 99)), 3),
+                                 "wait_p50_ms": round(1e3 * float(np.percentile(wait[m], 50)), 3),
+                                 "wait_p99_ms": round(1e3 * float(np.percentile(wait[m], 99)), 3),
+                                 "service_p50_ms": round(1e3 * float(np.percentile(service[m], 50)), 3),
+                                 "service_p99_ms": round(1e3 * float(np.percentile(service[m], 99)), 3)}
         runs.append({"rate_qps": rate, "p50_ms": round(1e3 * float(np.percentile(lat, 50)), 3),
                      "p99_ms": round(1e3 * float(np.percentile(lat, 99)), 3),
                      "mean_batch": round(float(np.mean(batch)), 2), "makespan_s": round(st["makespan"], 3),
