@@ -673,7 +673,7 @@ IG_DEV void append_parts(int parts, int x, unsigned long long* queue, unsigned* 
         (static_cast<unsigned long long>(x) << 32) | static_cast<unsigned>(q);
 }
 
-__global__ __launch_bounds__(kTcBlock) void tc_count(int nodes, const int2* __restrict__ orng,
+__global__ __launch_bounds__(kTcBlock) void tc_count(int nodes, int grab, const int2* __restrict__ orng,
                                                      const int* __restrict__ olist, const int* __restrict__ ilen,
                                                      const int* __restrict__ ilist, unsigned long long* mid,
                                                      unsigned long long* big, TcControl* ctl)
@@ -683,18 +683,21 @@ __global__ __launch_bounds__(kTcBlock) void tc_count(int nodes, const int2* __re
   int* tab = s_tab[threadIdx.x / kWave];
   unsigned long long cnt = 0;
   for (;;) {
-    // 64 consecutive vertices per grab (8 grabs per wave on average at
-    // R-MAT 22: coarser grabs leave a tail)
+    // `grab` (at most 64) consecutive vertices per grab, one per lane: 64 on
+    // large graphs (8 grabs per wave on average at R-MAT 22: coarser grabs
+    // leave a tail), fewer on small ones, whose few grabs would otherwise
+    // leave one wave with a long serial run of tasks
     unsigned v0 = 0;
-    if (lane == 0) v0 = atomic_add_relaxed(&ctl->next, static_cast<unsigned>(kWave));
+    if (lane == 0) v0 = atomic_add_relaxed(&ctl->next, static_cast<unsigned>(grab));
     v0 = wave_shfl(v0, 0);
     if (v0 >= static_cast<unsigned>(nodes)) break;
     // heavy tasks go to the workgroup passes in parts (|N+(x)| < 2: no
     // triangle can be found through N+(x))
     const int vl = static_cast<int>(v0) + lane;
-    const int2 rl = vl < nodes ? orng[vl] : make_int2(0, 0);
+    const bool mine = lane < grab && vl < nodes;
+    const int2 rl = mine ? orng[vl] : make_int2(0, 0);
     const int dl = rl.y;
-    const int il = vl < nodes ? ilen[vl] : 0;
+    const int il = mine ? ilen[vl] : 0;
     const bool here = dl >= 2 && dl <= kWaveList && (dl + il) * dl <= kWaveWork;
     const int parts = dl >= 2 && !here ? (dl + il + kPartItems - 1) / kPartItems : 0;
     append_parts(dl <= kMidList ? parts : 0, vl, mid, &ctl->nmid);
@@ -832,7 +835,13 @@ int tc_run(indigo_ctx* ctx, int nodes, int edges, const int* nindex, const int* 
   tc_select<false><<<gw, kTcBlock, 0, s>>>(nodes, nindex, nlist, in, irec, ilist, heavy, ctl);
   tc_select_heavy<false><<<gh, kTcBlock, 0, s>>>(nindex, nlist, in, irec, ilist, heavy, hcnt, ctl);
   tc_pack_huge<<<gh, kTcBlock, 0, s>>>(nindex, ilist, heavy, hcnt, irec, ctl);
-  tc_count<<<ctx->compute_units * 8, kTcBlock, 0, s>>>(nodes, orng, olist, ilen, ilist, mid, big, ctl);
+  {
+    // vertices per grab: 64 from 2^18 vertices on, else nodes / 4096 (at
+    // least 1), so a small graph still spreads over 4096 grabs while the
+    // grabs' same-address atomics stay few
+    const int grab = std::max(1, std::min(kWave, nodes / 4096));
+    tc_count<<<ctx->compute_units * 8, kTcBlock, 0, s>>>(nodes, grab, orng, olist, ilen, ilist, mid, big, ctl);
+  }
   const int hint = ctx->hints[INDIGO_HINT_TC_LDS_LIST];
   const int mid_list = hint > 0 && hint < kMidSlots / 4 ? hint : kMidSlots / 4;
   const int big_list = hint > 0 && hint < kBigSlots / 4 ? hint : kBigSlots / 4;
