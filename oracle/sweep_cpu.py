@@ -138,8 +138,13 @@ def main() -> int:
     egr = os.path.abspath(args.egr) if args.egr else make_graph(args.rmat, args.ef)
     print(f"{len(todo)} variants, graph {egr}, {args.threads} threads", flush=True)
 
-    rows, best = [], {}
+    best = {}
     env = dict(os.environ, OMP_NUM_THREADS=str(args.threads))
+    # rows are written as they finish, so a run cut short keeps what it measured
+    out = open(args.csv, "w", newline="") if args.csv else None
+    w = csv.DictWriter(out, fieldnames=["algorithm", "kind", "variant", "gteps", "note"]) if out else None
+    if w:
+        w.writeheader()
     for (alg, kind, s), (exe, err) in zip(todo, built):
         name = os.path.basename(s)
         name = name if name.endswith(".cpp") else name + ".cpp"
@@ -155,14 +160,13 @@ def main() -> int:
                 note = f"timeout after {args.timeout:.0f} s"
             print(f"{alg:4s} {kind:3s} {gteps if gteps is not None else '-':>10} {time.time() - t0:6.1f}s  {name}"
                   + (f"  ({note})" if note else ""), flush=True)
-        rows.append({"algorithm": alg, "kind": kind, "variant": name, "gteps": gteps, "note": note or ""})
+        if w:
+            w.writerow({"algorithm": alg, "kind": kind, "variant": name, "gteps": gteps, "note": note or ""})
+            out.flush()
         if gteps is not None and gteps > best.get((alg, kind), (0.0, ""))[0]:
             best[(alg, kind)] = (gteps, name)
-    if args.csv:
-        with open(args.csv, "w", newline="") as f:
-            w = csv.DictWriter(f, fieldnames=["algorithm", "kind", "variant", "gteps", "note"])
-            w.writeheader()
-            w.writerows(rows)
+    if out:
+        out.close()
     print("best per algorithm:")
     for alg in algs:
         for kind in kinds:

@@ -845,7 +845,7 @@ IG_DEV bool run_pr(const BatchTask& T, const Team& tm, double* s_red)
   return true;
 }
 
-__global__ __launch_bounds__(kBatchBlock) void batch_kernel(const BatchTask* __restrict__ tasks, int ntasks,
+__global__ __launch_bounds__(kBatchBlock) IG_SGPR_CAP void batch_kernel(const BatchTask* __restrict__ tasks, int ntasks,
                                                             GangState* gangs, BatchControl* ctl, long long budget,
                                                             long long residency_ticks)
 {
@@ -936,7 +936,7 @@ int gang_for(int alg, int nodes, int edges, int grid) { return std::min(batch_ga
 int batch_grid_limit(indigo_ctx* ctx, int* limit)
 {
   int per_cu = 0;
-  IG_TRY(resident_per_cu(ctx, reinterpret_cast<const void*>(batch_kernel), kBatchBlock, 0, &per_cu));
+  IG_TRY(resident_per_cu(ctx, reinterpret_cast<const void*>(batch_kernel), kBatchBlock, 0, &per_cu, true));
   *limit = std::max(1, per_cu * ctx->compute_units);
   return INDIGO_OK;
 }
@@ -954,7 +954,7 @@ static int launch_pending(indigo_ctx* ctx)
   IG_TRY(zero_async(ctx, gangs, static_cast<size_t>(ntasks) * sizeof(GangState)));
   void* args[] = {&d_tasks, &ntasks, &gangs, &ctl, &budget, &res_ticks};
   IG_TRY(coop_launch(ctx, reinterpret_cast<const void*>(batch_kernel), kBatchBlock, 0, args,
-                     ctx->pending_batch.grid_cap, nullptr, "batch_kernel"));
+                     ctx->pending_batch.grid_cap, nullptr, "batch_kernel", true));
   return timing_record_stop(ctx);
 }
 
@@ -999,7 +999,7 @@ int batch_launch(indigo_ctx* ctx, const indigo_graph_ref* graphs, int ngraphs, c
   }
   // grid: occupancy x CUs (capped), learned residency caps as for the other persistent kernels
   int per_cu = 0;
-  IG_TRY(resident_per_cu(ctx, reinterpret_cast<const void*>(batch_kernel), kBatchBlock, 0, &per_cu));
+  IG_TRY(resident_per_cu(ctx, reinterpret_cast<const void*>(batch_kernel), kBatchBlock, 0, &per_cu, true));
   const int grid_cap = std::max(1, max_workgroups > 0 ? std::min(max_workgroups, per_cu * ctx->compute_units)
                                                       : per_cu * ctx->compute_units);
   // tasks, largest gang and work first

@@ -137,13 +137,14 @@ struct SeqArgs {
   int launches;
 };
 
-unsigned long long seq_key(int alg, const SeqArgs& a)
+unsigned long long seq_key(const indigo_ctx* ctx, int alg, const SeqArgs& a)
 {
   unsigned long long k = key_mix(0, static_cast<unsigned>(alg));
   k = key_mix(k, static_cast<unsigned>(a.nodes));
   k = key_mix(k, static_cast<unsigned>(a.edges));
   k = key_mix(k, reinterpret_cast<uintptr_t>(a.nindex));
   k = key_mix(k, reinterpret_cast<uintptr_t>(a.nlist));
+  for (int h : ctx->hints) k = key_mix(k, static_cast<unsigned>(h));  // hints shape the launches
   return key_mix(k, reinterpret_cast<uintptr_t>(a.out));
 }
 
@@ -167,7 +168,7 @@ int cc_device(indigo_ctx* ctx, int nodes, int edges, const int* nindex, const in
   const double t0 = wall_seconds();
   IG_TRY(timing_begin(ctx));
   SeqArgs a{nodes, edges, nindex, nlist, label, 0};
-  IG_TRY(run_captured(ctx, seq_key(3, a), cc_sequence, &a));
+  IG_TRY(run_captured(ctx, seq_key(ctx, 3, a), cc_sequence, &a));
   IG_TRY(timing_end(ctx, a.launches, 1));
   if (seconds) *seconds = wall_seconds() - t0;
   return INDIGO_OK;
@@ -225,7 +226,7 @@ int tc_device(indigo_ctx* ctx, int nodes, int edges, const int* nindex, const in
   const double t0 = wall_seconds();
   IG_TRY(timing_begin(ctx));
   SeqArgs a{nodes, edges, nindex, nlist, nullptr, 10};
-  IG_TRY(run_captured(ctx, seq_key(6, a), tc_sequence, &a));
+  IG_TRY(run_captured(ctx, seq_key(ctx, 6, a), tc_sequence, &a));
   IG_TRY(timing_end(ctx, a.launches, 1));
   IG_TRY(tc_collect(ctx, count));
   if (seconds) *seconds = wall_seconds() - t0;

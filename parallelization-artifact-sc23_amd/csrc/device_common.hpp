@@ -14,6 +14,15 @@ constexpr int kInf = INT_MAX;  // unreachable marker (reference bfs_vertex_cuda.
 
 #define IG_DEV __device__ __forceinline__
 
+// Caps a kernel at 80 scalar registers.  On MI355X a wave that holds more
+// (96 or 112 after the allocation granule of 16) keeps its SIMD below 8
+// resident waves, which neither the compiler's occupancy figure nor the
+// occupancy API shows: a TC counting kernel ran 13.3 ms at 96 SGPRs and
+// 10.7 ms at 80 with otherwise the same code, and grids of 4 x 512 threads
+// per CU assemble only for kernels within 80.  The spilled values live in
+// VGPR lanes.
+#define IG_SGPR_CAP __attribute__((amdgpu_num_sgpr(80)))
+
 // ---- relaxed atomics on plain int arrays (the suite's atomicRead/atomicWrite,
 // include/bfs_vertex_cuda.h:49-59, as agent-scope relaxed HIP atomics) ------
 template <typename T>

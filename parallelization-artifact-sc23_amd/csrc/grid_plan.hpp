@@ -14,13 +14,15 @@
 
 namespace indigo {
 
-// Co-resident workgroups per CU from the occupancy API's answer: a grid that
+// Co-resident workgroups per CU from the occupancy API's answer.  A grid that
 // needs every wave slot of every CU (for example 4 x 512 threads on a
-// 2048-thread CU) did not assemble on MI355X, so one workgroup is left out.
-inline int plan_per_cu(int occupancy, int block, int max_threads_per_cu)
+// 2048-thread CU) did not assemble on MI355X for kernels of more than 80
+// scalar registers, which the occupancy API does not account for, so one
+// workgroup is left out; kernels held to 80 (full_cu) fill the CU.
+inline int plan_per_cu(int occupancy, int block, int max_threads_per_cu, bool full_cu = false)
 {
   int n = std::max(occupancy, 0);
-  if (n > 1 && static_cast<long long>(n) * block >= max_threads_per_cu) n -= 1;
+  if (!full_cu && n > 1 && static_cast<long long>(n) * block >= max_threads_per_cu) n -= 1;
   return n;
 }
 

@@ -141,7 +141,7 @@ IG_DEV void exclude_neighbours_wave(const MisArgs& a, int v, int beg, int end)
   }
 }
 
-__global__ __launch_bounds__(kMisBlock) void mis_kernel(MisArgs a)
+__global__ __launch_bounds__(kMisBlock) IG_SGPR_CAP void mis_kernel(MisArgs a)
 {
   __shared__ BlockQueue<4096> s_wl;
   __shared__ BlockQueue<1024> s_hwl;
@@ -260,7 +260,7 @@ int mis_run(indigo_ctx* ctx, int nodes, const int* nindex, const int* nlist, int
   a.start_ticks = residency_ticks(ctx);
   IG_TRY(zero_async(ctx, a.ctl, sizeof(MisControl)));
   void* args[] = {&a};
-  IG_TRY(coop_launch(ctx, reinterpret_cast<const void*>(mis_kernel), kMisBlock, 0, args, 0, nullptr, "mis_kernel"));
+  IG_TRY(coop_launch(ctx, reinterpret_cast<const void*>(mis_kernel), kMisBlock, 0, args, 0, nullptr, "mis_kernel", true));
   return INDIGO_OK;
 }
 

@@ -104,20 +104,20 @@ int ensure_host_pinned(indigo_ctx* ctx, size_t bytes)
   return INDIGO_OK;
 }
 
-int resident_per_cu(indigo_ctx* ctx, const void* kernel, int block, size_t shmem, int* per_cu)
+int resident_per_cu(indigo_ctx* ctx, const void* kernel, int block, size_t shmem, int* per_cu, bool full_cu)
 {
   int n = 0;
   IG_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kernel, block, shmem));
   if (n < 1) return set_error(INDIGO_ERR_DEVICE, "kernel cannot be resident with block %d", block);
-  *per_cu = plan_per_cu(n, block, ctx->max_threads_per_cu);
+  *per_cu = plan_per_cu(n, block, ctx->max_threads_per_cu, full_cu);
   return INDIGO_OK;
 }
 
 int coop_launch(indigo_ctx* ctx, const void* kernel, int block, size_t shmem, void** args, int max_blocks,
-                int* grid_out, const char* name)
+                int* grid_out, const char* name, bool full_cu)
 {
   int per_cu = 0;
-  IG_TRY(resident_per_cu(ctx, kernel, block, shmem, &per_cu));
+  IG_TRY(resident_per_cu(ctx, kernel, block, shmem, &per_cu, full_cu));
   GridLimits lim;
   lim.compute_units = ctx->compute_units;
   lim.hint_blocks_per_cu = ctx->hints[INDIGO_HINT_BLOCKS_PER_CU];

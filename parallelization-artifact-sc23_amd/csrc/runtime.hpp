@@ -124,12 +124,14 @@ inline long long budget_ticks(const indigo_ctx* ctx)
 // (A cooperative launch would add nothing, and on ROCm it fails while
 // another process holds a context on the same GPU.)
 // `name` (a string literal, or null) marks the launch as the call's main
-// kernel, timed by kernel_begin / kernel_end.
+// kernel, timed by kernel_begin / kernel_end.  full_cu: the kernel is held
+// to 80 scalar registers (IG_SGPR_CAP) and may fill every wave slot of a CU.
 int coop_launch(indigo_ctx* ctx, const void* kernel, int block, size_t shmem, void** args, int max_blocks,
-                int* grid_out, const char* name = nullptr);
+                int* grid_out, const char* name = nullptr, bool full_cu = false);
 // Workgroups of `kernel` that coop_launch places per CU before any learned
-// cap or hint (occupancy, less one when that would fill the CU).
-int resident_per_cu(indigo_ctx* ctx, const void* kernel, int block, size_t shmem, int* per_cu);
+// cap or hint (occupancy, less one when that would fill the CU unless
+// full_cu).
+int resident_per_cu(indigo_ctx* ctx, const void* kernel, int block, size_t shmem, int* per_cu, bool full_cu = false);
 // Internal status: the last persistent launch did not assemble (see
 // residency_barrier); the context's cap of `kernel` has been lowered below
 // the grid actually launched (lowered_cap in grid_plan.hpp), launch again.

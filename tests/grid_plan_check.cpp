@@ -41,6 +41,10 @@ int main()
   CHECK_EQ(plan_per_cu(3, 512, 2048), 3);
   CHECK_EQ(plan_per_cu(1, 1024, 1024), 1);  // a single workgroup is never dropped
   CHECK_EQ(plan_per_cu(0, 256, 2048), 0);
+  // kernels held to 80 scalar registers fill the CU
+  CHECK_EQ(plan_per_cu(4, 512, 2048, true), 4);
+  CHECK_EQ(plan_per_cu(2, 1024, 2048, true), 2);
+  CHECK_EQ(plan_per_cu(3, 512, 2048, true), 3);
 
   // each bound on its own, then the smallest of several
   CHECK_EQ(plan_grid(3, limits(256, 0, 0, 0, 0)), 768);
