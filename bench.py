@@ -494,14 +494,13 @@ def cpu_model() -> str:
 def cpu_baseline(scale: int, threads: int, runs: int = 3) -> dict | None:
     """The reference's own CPU variants (compiled from its sources into
     oracle/_ref) on the benchmark's own graph: R-MAT `scale`, edge factor 16,
-    seed 1, weights [1, 255], source 0.  Per algorithm the best published
-    C++-threads variant (data/*_cpp.csv) and an OpenMP variant (the best
-    published for MIS and TC, data/mis_omp.csv and data/tc_omp.csv; the
-    winner of the all-variant sweep, oracle/sweep_cpu.py, for the others) are
-    each timed `runs` times, and the faster median counts - the reference
-    harness's own rule, median of 3 (include/bfs_vertex_cpp.h:106-133).  TC
-    is bounded: one run of the OpenMP variant only (its C++-threads variant
-    is several times slower on this graph)."""
+    seed 1, weights [1, 255], source 0.  Per algorithm a C++-threads and an
+    OpenMP variant - the winners of the all-variant sweep (oracle/sweep_cpu.py)
+    on the GPU box's host, as scripts/run_codes.py picks the best variant per
+    machine - are each timed `runs` times, and the faster median counts: the
+    reference harness's own rule, median of 3 (include/bfs_vertex_cpp.h:106-133).
+    TC is bounded: one run of the OpenMP variant only (the C++-threads
+    variants are several times slower on this graph)."""
     sys.path.insert(0, os.path.join(REPO, "oracle"))
     try:
         import oracle as orc  # checker / baseline only
@@ -541,8 +540,8 @@ def cpu_baseline(scale: int, threads: int, runs: int = 3) -> dict | None:
         "cores": threads,
         "kind": "reference",
         "sample": f"the benchmark's own graph, rmat{scale} ef16 seed 1 (V={g.nodes}, E={g.edges}), source 0; per "
-                  f"algorithm the faster median of {runs} runs of the best published C++-threads variant and an "
-                  f"OpenMP variant; TC bounded to one run of its OpenMP variant; {threads} threads of "
+                  f"algorithm the faster median of {runs} runs of the C++-threads and the OpenMP variant that won "
+                  f"an all-variant sweep on this host; TC bounded to one run of its OpenMP variant; {threads} threads of "
                   f"{os.cpu_count()} on {cpu_model()}",
         "cpu_model": cpu_model(),
         "host_cpus": os.cpu_count(),
