@@ -62,6 +62,18 @@ int indigo_gen_road(int rows, int cols, double keep_p, double diag_p, unsigned l
 int indigo_graph_copy(const indigo_host_graph* src, indigo_host_graph* out);
 int indigo_graph_release(indigo_host_graph* g);
 
+/* Degree-ordered relabelling (no counterpart in the reference: a graph
+ * preparation step for callers that query one graph many times).  New ids go
+ * by degree, largest first, ties in id order; perm[v] (nodes ints, caller's
+ * memory) receives the new id of vertex v.  out holds the same graph under
+ * the new ids, each list sorted by new id, weights moved with their entries.
+ * Hubs then sit at the smallest ids, so the per-vertex data that many lists
+ * gather (distances, PageRank contributions) shares cache lines.  Results of
+ * BFS, SSSP and PageRank on out, read through perm (indigo_unpermute), equal
+ * those on src; CC labels and MIS sets depend on the ids and are not
+ * carried over. */
+int indigo_graph_reorder(const indigo_host_graph* src, int threads, indigo_host_graph* out, int* perm);
+
 /* .egr I/O in the ECLgraph format (reference include/ECLgraph.h:52-96);
  * unlike the header-only loader these return status codes instead of exiting. */
 int indigo_read_egr(const char* path, indigo_host_graph* out);

@@ -293,6 +293,13 @@ int indigo_graph_free(indigo_graph_ref* g);
  * host, include/bfs_edge_cuda.h:140-146, tc_edge_cuda.h:159-165). */
 int indigo_build_sp(indigo_ctx* ctx, int nodes, int edges, const int* d_nindex, int* d_sp);
 
+/* d_out[v] = d_in[d_perm[v]] for v < n, 4-byte values (no counterpart in the
+ * reference): reads a per-vertex result computed on a degree-ordered copy of
+ * a graph (indigo_graph_reorder in indigo_graph.h, perm[v] = new id of v)
+ * back in the original ids.  d_in and d_out must not overlap.  Asynchronous
+ * on the context stream; indigo_ctx_last_ms gives its time. */
+int indigo_unpermute(indigo_ctx* ctx, int n, const int* d_perm, const void* d_in, void* d_out);
+
 /* ---- serving ----------------------------------------------------------------- */
 /* A request loop over resident graphs (no counterpart in the reference).
  * Requests arrive over time; every free slot (a context with its own stream

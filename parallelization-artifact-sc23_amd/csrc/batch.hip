@@ -931,16 +931,21 @@ int gang_for(int alg, int nodes, int edges, int grid) { return std::min(batch_ga
 
 }  // namespace
 
-// (Re)launches the batch recorded in ctx->pending_batch; records the stop
-// event without waiting.
+// Workgroups that every persistent kernel of the library can hold resident
+// at once: the batch kernel's occupancy with one workgroup per CU left out,
+// as for the kernels of more than 80 scalar registers (BFS, SSSP, PageRank
+// stop at 3 x 512 threads per CU), so that shares of it assemble side by
+// side whatever kernels the sharers run.
 int batch_grid_limit(indigo_ctx* ctx, int* limit)
 {
   int per_cu = 0;
-  IG_TRY(resident_per_cu(ctx, reinterpret_cast<const void*>(batch_kernel), kBatchBlock, 0, &per_cu, true));
+  IG_TRY(resident_per_cu(ctx, reinterpret_cast<const void*>(batch_kernel), kBatchBlock, 0, &per_cu, false));
   *limit = std::max(1, per_cu * ctx->compute_units);
   return INDIGO_OK;
 }
 
+// (Re)launches the batch recorded in ctx->pending_batch; records the stop
+// event without waiting.
 static int launch_pending(indigo_ctx* ctx)
 {
   BatchTask* d_tasks = static_cast<BatchTask*>(ctx->pending_batch.tasks);

@@ -17,7 +17,8 @@ int batch_launch(indigo_ctx* ctx, const indigo_graph_ref* graphs, int ngraphs, c
                  int nqueries, int max_workgroups);
 int batch_ready(indigo_ctx* ctx, int* ready);
 int batch_wait(indigo_ctx* ctx);
-// Workgroups a batched launch can keep resident on the whole device.
+// Workgroups that any persistent kernel of the library can keep resident on
+// the whole device (the base of the serving slots' shares).
 int batch_grid_limit(indigo_ctx* ctx, int* limit);
 // The engine's gang size for a query (before clamping to the grid).
 int batch_gang(int alg, int nodes, int edges);

@@ -392,6 +392,92 @@ int indigo_graph_copy(const indigo_host_graph* src, indigo_host_graph* out)
   return INDIGO_OK;
 }
 
+int indigo_graph_reorder(const indigo_host_graph* src, int threads, indigo_host_graph* out, int* perm)
+{
+  if (src == nullptr || out == nullptr || perm == nullptr || src->nodes < 1 || src->edges < 0 ||
+      src->nindex == nullptr || (src->edges > 0 && src->nlist == nullptr))
+    return set_error(INDIGO_ERR_INVALID_ARG, "graph_reorder: bad arguments");
+  const int n = src->nodes;
+  const int nt = pick_threads(threads);
+  clear(out);
+  // new ids by degree, largest first, ties in id order: a counting sort
+  // over the degrees, stable because vertices are placed in id order
+  int maxdeg = 0;
+  for (int v = 0; v < n; v++) {
+    const int d = src->nindex[v + 1] - src->nindex[v];
+    if (d < 0 || src->nindex[v] < 0 || src->nindex[v + 1] > src->edges)
+      return set_error(INDIGO_ERR_INVALID_ARG, "graph_reorder: nindex is not a CSR offset array");
+    maxdeg = std::max(maxdeg, d);
+  }
+  std::vector<long long> start(static_cast<size_t>(maxdeg) + 2, 0);
+  for (int v = 0; v < n; v++) start[static_cast<size_t>(maxdeg - (src->nindex[v + 1] - src->nindex[v])) + 1]++;
+  for (size_t k = 1; k < start.size(); k++) start[k] += start[k - 1];
+  std::vector<int> order(static_cast<size_t>(n));
+  for (int v = 0; v < n; v++) {
+    const size_t k = static_cast<size_t>(maxdeg - (src->nindex[v + 1] - src->nindex[v]));
+    const int id = static_cast<int>(start[k]++);
+    perm[v] = id;
+    order[static_cast<size_t>(id)] = v;
+  }
+  out->nodes = n;
+  out->edges = src->edges;
+  const size_t ne = static_cast<size_t>(std::max(src->edges, 1));
+  out->nindex = static_cast<int*>(malloc(sizeof(int) * (static_cast<size_t>(n) + 1)));
+  out->nlist = static_cast<int*>(malloc(sizeof(int) * ne));
+  out->eweight = src->eweight ? static_cast<int*>(malloc(sizeof(int) * ne)) : nullptr;
+  if (!out->nindex || !out->nlist || (src->eweight && !out->eweight)) {
+    indigo_graph_release(out);
+    return set_error(INDIGO_ERR_OUT_OF_MEMORY, "graph_reorder: host allocation failed");
+  }
+  out->nindex[0] = 0;
+  for (int i = 0; i < n; i++) {
+    const int v = order[static_cast<size_t>(i)];
+    out->nindex[i + 1] = out->nindex[i] + (src->nindex[v + 1] - src->nindex[v]);
+  }
+  // rows: each vertex's list renamed and sorted by new id (weights moved
+  // with their entries); rows handed out in chunks, long ones come first
+  std::atomic<bool> bad{false};
+  {
+    constexpr int kChunk = 1024;
+    std::atomic<long long> next{0};
+    run_threads(std::min<int>(nt, n / kChunk + 1), [&](int) {
+      std::vector<std::pair<int, int>> row;
+      for (;;) {
+        const long long b = next.fetch_add(kChunk, std::memory_order_relaxed);
+        if (b >= n) break;
+        const long long e = std::min<long long>(n, b + kChunk);
+        for (long long i = b; i < e; i++) {
+          const int v = order[static_cast<size_t>(i)];
+          const int o = src->nindex[v], d = src->nindex[v + 1] - o;
+          row.resize(static_cast<size_t>(d));
+          for (int j = 0; j < d; j++) {
+            const int u = src->nlist[o + j];
+            if (u < 0 || u >= n) {
+              bad.store(true, std::memory_order_relaxed);
+              row[static_cast<size_t>(j)] = {0, 0};
+              continue;
+            }
+            row[static_cast<size_t>(j)] = {perm[u], src->eweight ? src->eweight[o + j] : 0};
+          }
+          std::sort(row.begin(), row.end(), [](const std::pair<int, int>& x, const std::pair<int, int>& y) {
+            return x.first < y.first;
+          });
+          const int q = out->nindex[i];
+          for (int j = 0; j < d; j++) {
+            out->nlist[q + j] = row[static_cast<size_t>(j)].first;
+            if (out->eweight) out->eweight[q + j] = row[static_cast<size_t>(j)].second;
+          }
+        }
+      }
+    });
+  }
+  if (bad.load()) {
+    indigo_graph_release(out);
+    return set_error(INDIGO_ERR_INVALID_ARG, "graph_reorder: a list entry is not a vertex id");
+  }
+  return INDIGO_OK;
+}
+
 int indigo_graph_release(indigo_host_graph* g)
 {
   if (g == nullptr) return set_error(INDIGO_ERR_INVALID_ARG, "graph_release: null");

@@ -98,6 +98,14 @@ __global__ __launch_bounds__(256) void sp_kernel(const int* __restrict__ nindex,
   }
 }
 
+// out[v] = in[perm[v]], 4-byte values.
+__global__ __launch_bounds__(256) void unpermute_kernel(int n, const int* __restrict__ perm, const int* __restrict__ in,
+                                                        int* __restrict__ out)
+{
+  const int stride = gridDim.x * blockDim.x;
+  for (int v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += stride) out[v] = in[perm[v]];
+}
+
 }  // namespace
 
 int indigo::build_sp_async(indigo_ctx* ctx, int nodes, int edges, const int* d_nindex, int* d_sp)
@@ -168,6 +176,23 @@ int indigo_build_sp(indigo_ctx* ctx, int nodes, int edges, const int* d_nindex, 
   IG_TRY(timing_begin(ctx));
   IG_TRY(kernel_begin(ctx, "sp_kernel"));
   IG_TRY(build_sp_async(ctx, nodes, edges, d_nindex, d_sp));
+  IG_TRY(kernel_end(ctx));
+  return timing_end(ctx, 1, 1);
+}
+
+int indigo_unpermute(indigo_ctx* ctx, int n, const int* d_perm, const void* d_in, void* d_out)
+{
+  clear_error();
+  if (ctx == nullptr || n < 0 || (n > 0 && (d_perm == nullptr || d_in == nullptr || d_out == nullptr)) ||
+      (n > 0 && d_in == d_out))
+    return set_error(INDIGO_ERR_INVALID_ARG, "unpermute: bad arguments");
+  IG_HIP_TRY(hipSetDevice(ctx->device));
+  if (n == 0) return INDIGO_OK;
+  IG_TRY(timing_begin(ctx));
+  IG_TRY(kernel_begin(ctx, "unpermute_kernel"));
+  const int grid = static_cast<int>(std::min<long long>((static_cast<long long>(n) + 255) / 256, 8LL * ctx->compute_units));
+  unpermute_kernel<<<grid, 256, 0, ctx->stream>>>(n, d_perm, static_cast<const int*>(d_in), static_cast<int*>(d_out));
+  IG_HIP_TRY(hipGetLastError());
   IG_TRY(kernel_end(ctx));
   return timing_end(ctx, 1, 1);
 }

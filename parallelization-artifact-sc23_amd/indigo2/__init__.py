@@ -7,7 +7,7 @@ works; importing does not load the shared library until first use.
 """
 from . import _lib
 from ._lib import IndigoError, load
-from .graph import DeviceBuffer, DeviceGraph, HostGraph, erdos_renyi, from_edges, grid, read_egr, rmat, road
+from .graph import DeviceBuffer, DeviceGraph, HostGraph, ReorderedGraph, erdos_renyi, from_edges, grid, read_egr, rmat, road
 from .kernels import (
     EXCLUDED,
     INCLUDED,
@@ -30,7 +30,7 @@ from .kernels import (
 )
 
 __all__ = [
-    "_lib", "IndigoError", "load", "DeviceBuffer", "DeviceGraph", "HostGraph", "erdos_renyi", "from_edges",
+    "_lib", "IndigoError", "load", "DeviceBuffer", "DeviceGraph", "HostGraph", "ReorderedGraph", "erdos_renyi", "from_edges",
     "grid", "read_egr", "rmat", "road", "EXCLUDED", "INCLUDED", "INF", "UNDECIDED", "Context", "Query", "serve", "GPUbfs_edge",
     "GPUbfs_vertex", "GPUcc_edge", "GPUcc_vertex", "GPUmis_edge", "GPUmis_vertex", "GPUsssp_edge",
     "GPUsssp_vertex", "GPUtc_edge", "GPUtc_vertex", "PR_GPU",

@@ -178,6 +178,7 @@ _SIGNATURES = {
                             POINTER(ctypes.c_double)],
     "indigo_graph_free": [POINTER(GraphRefStruct)],
     "indigo_build_sp": [c_void_p, c_int, c_int, c_void_p, c_void_p],
+    "indigo_unpermute": [c_void_p, c_int, c_void_p, c_void_p, c_void_p],
     "indigo_egr_open": [c_char_p, POINTER(EgrMapStruct)],
     "indigo_egr_close": [POINTER(EgrMapStruct)],
     "indigo_serve": [POINTER(GraphRefStruct), c_int, POINTER(RequestStruct), c_int, POINTER(ServeConfigStruct),
@@ -199,6 +200,7 @@ _SIGNATURES = {
     "indigo_gen_road": [c_int, c_int, c_double, c_double, c_ulonglong, c_int, c_int, c_int, _PHG],
     "indigo_graph_copy": [_PHG, _PHG],
     "indigo_graph_release": [_PHG],
+    "indigo_graph_reorder": [_PHG, c_int, _PHG, _PI],
     "indigo_read_egr": [c_char_p, _PHG],
     "indigo_write_egr": [c_char_p, _PHG],
     "indigo_graph_check": [_PHG],

@@ -16,7 +16,7 @@ import ctypes
 import numpy as np
 
 from . import _lib
-from .graph import DeviceBuffer, DeviceGraph, HostGraph
+from .graph import DeviceBuffer, DeviceGraph, HostGraph, ReorderedGraph
 
 INF = np.iinfo(np.int32).max
 UNDECIDED, INCLUDED, EXCLUDED = 0, 1, 2  # mis_vertex_cuda.h:50-52
@@ -133,6 +133,46 @@ class Context:
                 qt[i].src[j] = s_
             qt[i].d_out = q.out.ptr
         _lib.call("indigo_batch", self.h, gt, len(graphs), qt, len(queries), max_workgroups)
+
+    # ---- degree-ordered copies: results in the original ids -----------------
+    def unpermute(self, n: int, d_perm: DeviceBuffer, src: DeviceBuffer, out: DeviceBuffer) -> None:
+        """out[v] = src[perm[v]] for v < n (indigo_unpermute)."""
+        _lib.call("indigo_unpermute", self.h, n, d_perm.ptr, src.ptr, out.ptr)
+
+    def bfs_reordered_into(self, rg: ReorderedGraph, src: int, out: DeviceBuffer) -> float:
+        """BFS from original vertex src on rg; out in the original ids.
+        Returns the device ms of the search and the read-back together."""
+        self.bfs_into(rg.graph, int(rg.perm[src]), rg.scratch)
+        ms = self.last_ms
+        self.unpermute(rg.nodes, rg.d_perm, rg.scratch, out)
+        return ms + self.last_ms
+
+    def sssp_reordered_into(self, rg: ReorderedGraph, src: int, out: DeviceBuffer) -> float:
+        self.sssp_into(rg.graph, int(rg.perm[src]), rg.scratch)
+        ms = self.last_ms
+        self.unpermute(rg.nodes, rg.d_perm, rg.scratch, out)
+        return ms + self.last_ms
+
+    def pagerank_reordered_into(self, rg: ReorderedGraph, out: DeviceBuffer) -> tuple[int, float]:
+        it = self.pagerank_into(rg.graph, rg.scratch)
+        ms = self.last_ms
+        self.unpermute(rg.nodes, rg.d_perm, rg.scratch, out)
+        return it, ms + self.last_ms
+
+    def bfs_reordered(self, rg: ReorderedGraph, src: int) -> np.ndarray:
+        out = DeviceBuffer(4 * rg.nodes)
+        self.bfs_reordered_into(rg, src, out)
+        return out.to_array(np.int32, rg.nodes)
+
+    def sssp_reordered(self, rg: ReorderedGraph, src: int) -> np.ndarray:
+        out = DeviceBuffer(4 * rg.nodes)
+        self.sssp_reordered_into(rg, src, out)
+        return out.to_array(np.int32, rg.nodes)
+
+    def pagerank_reordered(self, rg: ReorderedGraph) -> tuple[np.ndarray, int]:
+        out = DeviceBuffer(4 * rg.nodes)
+        it, _ = self.pagerank_reordered_into(rg, out)
+        return out.to_array(np.float32, rg.nodes), it
 
     # ---- host-output conveniences -------------------------------------------
     def bfs(self, g: DeviceGraph, src: int) -> np.ndarray:

@@ -93,6 +93,25 @@ def test_rmat22_pagerank(orc, ctx, rmat22):
     assert abs(it32 - iters) <= 1, (it32, iters)
 
 
+def test_rmat22_reordered(ig, orc, ctx, rmat22):
+    """BFS, SSSP and PageRank on the degree-ordered copy of the graph, read
+    back in the original ids, as bench.py's reordered step runs them."""
+    g, _ = rmat22
+    rg = ig.ReorderedGraph(g)
+    try:
+        for src in _sources(g)[:3]:
+            print(f"rmat22 reordered bfs and sssp from {src}", flush=True)
+            assert np.array_equal(ctx.bfs_reordered(rg, src), orc.bfs(g, src)), src
+            assert np.array_equal(ctx.sssp_reordered(rg, src), orc.sssp(g, src)), src
+        scores, iters = ctx.pagerank_reordered(rg)
+        ref, _ = orc.pr64(g, iters)
+        err = np.abs(scores.astype(np.float64) - ref)
+        assert err.max() <= PR_MAX_ABS, err.max()
+        assert err.sum() <= PR_L1, err.sum()
+    finally:
+        rg.close()
+
+
 def test_rmat22_tc(ctx, rmat22, gold):
     _, dg = rmat22
     assert ctx.tc(dg) == gold["triangles"]
