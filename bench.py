@@ -349,7 +349,7 @@ def measure_batched() -> dict:
 
 
 def measure_serving(rates=(1000.0,), nq: int = 10000, seed: int = 5, slots: int = 4, max_batch: int = 256,
-                    mem_budget: int = 8 << 30) -> dict:
+                    mem_budget: int = 8 << 30, batch_work_us: int = 0) -> dict:
     """10,000 seeded requests with Poisson arrivals at each rate over the pool;
     mix BFS 40% / SSSP 30% / CC 10% / PageRank 10% / TC 10%, uniform graph
     and source.  Latency = finish - arrival; aggregate GTEPS = 1e-9 * sum of
@@ -374,7 +374,8 @@ def measure_serving(rates=(1000.0,), nq: int = 10000, seed: int = 5, slots: int 
         arrivals = np.cumsum(gaps / rate)
         log(f"serving {nq} requests at {rate:.0f} q/s")
         adm, fin, batch, slot, st = ig.serve(dgs, algs, gid, src, arrivals, None, slots=slots,
-                                             max_batch=max_batch, mem_budget=mem_budget, progress=10.0)
+                                             max_batch=max_batch, mem_budget=mem_budget, progress=10.0,
+                                             batch_work_us=batch_work_us)
         lat = fin - arrivals
         per = {}
         for a in ("bfs", "sssp", "cc", "pr", "tc"):
@@ -405,7 +406,8 @@ def measure_serving(rates=(1000.0,), nq: int = 10000, seed: int = 5, slots: int 
             f"{runs[-1]['aggregate_gteps']} GTEPS, mean batch {runs[-1]['mean_batch']}")
     return {"workload": "serving: 64 resident graphs (32 R-MAT ef16 scales 14-22, 32 grids 256^2-2048^2), "
                         f"{nq} requests, Poisson arrivals, mix BFS .4 / SSSP .3 / CC .1 / PR .1 / TC .1",
-            "slots": slots, "max_batch": max_batch, "mem_budget": mem_budget, "runs": runs}
+            "slots": slots, "max_batch": max_batch, "mem_budget": mem_budget,
+            "batch_work_us": batch_work_us or "no cap", "runs": runs}
 
 
 def measure_deep(kind: str, steps: int, warmup: int) -> dict:
@@ -638,6 +640,8 @@ def main() -> int:
     ap.add_argument("--rates", default="500,1000,2000", help="serving workload: arrival rates in requests/s")
     ap.add_argument("--slots", type=int, default=4, help="serving workload: concurrent slots")
     ap.add_argument("--max-batch", type=int, default=256, help="serving workload: requests per launch")
+    ap.add_argument("--batch-work-us", type=int, default=0,
+                    help="serving workload: estimated whole-GPU microseconds per launch (0: no cap)")
     ap.add_argument("--workload", choices=["rmat", "batched", "serving", "grid", "road"], default="rmat",
                     help="rmat: the six algorithms on the R-MAT graph (the default line); batched: batched vs "
                          "plain; serving: request loop; grid: BFS on the 1024^2 grid; road: BFS/SSSP/CC on the "
@@ -670,7 +674,7 @@ def main() -> int:
         return 0
     if args.workload == "serving":
         print(json.dumps(measure_serving(tuple(float(r) for r in args.rates.split(",")), slots=args.slots,
-                                         max_batch=args.max_batch)), flush=True)
+                                         max_batch=args.max_batch, batch_work_us=args.batch_work_us)), flush=True)
         return 0
     if args.copy_calibration:
         import indigo2 as ig

@@ -190,8 +190,9 @@ int indigo_ctx_set_hint(indigo_ctx* ctx, int key, int value);
 #define INDIGO_HINT_TC_LDS_LIST 6    /* longest oriented list TC hashes in LDS (0: 16384); longer ones use binary search */
 #define INDIGO_HINT_RESIDENCY_MS 7   /* how long a persistent grid may take to assemble (0: 100 ms); longer when
                                         the GPU is shared with other work */
-#define INDIGO_HINT_MAX_WORKGROUPS 8 /* cap on the workgroups of any persistent grid of the context (0: the
-                                        whole GPU); for contexts that share the GPU */
+#define INDIGO_HINT_MAX_WORKGROUPS 8 /* cap on the grids of the context, in workgroups of 512 threads (other
+                                        sizes: as many threads); 0: the whole GPU.  For contexts that share
+                                        the GPU (each serving slot holds its share) */
 #define INDIGO_HINT_VARIANT 9        /* INDIGO_VARIANT_*: which form indigo_bfs / _sssp / _cc / _mis run */
 #define INDIGO_HINT_BLOCK_SIZE 10    /* workgroup size of the edge-centric kernels: 256, 512 (0: default) or 1024 */
 #define INDIGO_HINT_GRAPH 11        /* 1: CC and TC run their kernel sequence as a HIP graph, captured on the
@@ -342,6 +343,13 @@ typedef struct indigo_serve_config {
    * INDIGO_ERR_INVALID_ARG: a run with it ended in a host segmentation fault
    * whose cause is not established */
   int cu_partition;
+  /* estimated microseconds of a slot's work one batched launch may hold
+   * (0: no cap); a request that would pass it waits for the next launch.  A
+   * launch finishes as a whole, so its size bounds the service time of
+   * every request in it (caps of 1000 to 4000 us raised p50 and p99 at
+   * 150 q/s on the 64-graph pool: the queue grows more than launches
+   * shrink) */
+  int batch_work_us;
 } indigo_serve_config;
 
 typedef struct indigo_serve_stats {

@@ -521,6 +521,7 @@ int indigo_serve(const indigo_graph_ref* graphs, int ngraphs, indigo_request* re
   clear_error();
   if (cfg == nullptr || cfg->slots < 1 || cfg->slots > 16 || cfg->max_batch < 1 || cfg->mem_budget == 0)
     return set_error(INDIGO_ERR_INVALID_ARG, "serve: bad configuration");
+  if (cfg->batch_work_us < 0) return set_error(INDIGO_ERR_INVALID_ARG, "serve: batch_work_us must be >= 0");
   if (cfg->cu_partition != 0)
     return set_error(INDIGO_ERR_INVALID_ARG, "serve: the CU partition between slots is switched off");
   if (nreqs < 0 || ngraphs < 1 || graphs == nullptr || (nreqs > 0 && reqs == nullptr))

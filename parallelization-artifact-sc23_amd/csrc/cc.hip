@@ -209,9 +209,9 @@ int cc_run_in(indigo_ctx* ctx, void* ws, int nodes, int edges, const int* nindex
   int* parent = cv.take<int>(nodes);
   FrontierQueues heavy{nullptr, cv.take<int>(nodes), cv.take<int>(nodes), cv.take<unsigned>(nodes)};
   IG_TRY(zero_async(ctx, ctl, sizeof(CcControl)));
-  const int gv = grid_for(nodes, kCcBlock, 8192);
-  const int gl = grid_for((nodes + kWave - 1) / kWave, kCcBlock / kWave, 1 << 16);
-  const int gh = ctx->compute_units * 8;
+  const int gv = share_grid(ctx, grid_for(nodes, kCcBlock, 8192), kCcBlock);
+  const int gl = share_grid(ctx, grid_for((nodes + kWave - 1) / kWave, kCcBlock / kWave, 1 << 16), kCcBlock);
+  const int gh = share_grid(ctx, ctx->compute_units * 8, kCcBlock);
   // The Afforest order pays an extra compress and a find per vertex to skip
   // the giant component's lists: worth it when lists are long (R-MAT: 1.04
   // -> 0.25 ms at scale 22), not on sparse high-diameter graphs, whose deep

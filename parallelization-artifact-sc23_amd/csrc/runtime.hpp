@@ -107,6 +107,20 @@ struct Carver {
   static size_t round(size_t bytes) { return (bytes + 255) & ~static_cast<size_t>(255); }
 };
 
+// Grid of an ordinary (grid-stride or work-queue) launch within the
+// context's share of the GPU: INDIGO_HINT_MAX_WORKGROUPS counts workgroups
+// of 512 threads, so a grid of `block`-thread workgroups is held to as many
+// threads.  Without it, one context's full-GPU grid (TC's counting kernels
+// hold every CU until their queue drains) keeps the persistent grids of the
+// contexts beside it from assembling.
+inline int share_grid(const indigo_ctx* ctx, int grid, int block)
+{
+  const int cap = ctx->hints[INDIGO_HINT_MAX_WORKGROUPS];
+  if (cap <= 0 || block <= 0) return grid;
+  const long long t = static_cast<long long>(cap) * 512 / block;
+  return static_cast<int>(std::max<long long>(1, std::min<long long>(grid, t)));
+}
+
 // Ticks of wall_clock64() that a persistent kernel may spend in bounded waits.
 inline long long budget_ticks(const indigo_ctx* ctx)
 {

@@ -8,10 +8,13 @@
 // multi-source task (up to 64 / 32 sources), CC, MIS and PR one task each,
 // all in one batched launch (indigo_batch); a triangle count at the head of
 // the queue takes the slot alone (its kernels are ordinary launches), and so
-// does an SSSP request on a deep, sparse graph (average degree below 6, at
-// least 65536 vertices: grids, road networks), on the single-query
-// near-far kernel: the batched Bellman-Ford rounds re-relax such graphs many
-// times over, while near-far settles them in distance order.  The
+// does a BFS or SSSP request on a deep, sparse graph (average degree below
+// 6, at least 65536 vertices: grids, road networks), on the single-query
+// kernels: their one-workgroup phase runs the thousands of narrow levels at
+// workgroup-barrier cost (the batched engine's gangs pay a device barrier
+// per level: 82 ms against about 45 for a 2048^2 grid's BFS), and the
+// batched Bellman-Ford rounds re-relax such graphs many times over, while
+// near-far settles them in distance order.  The
 // other slots keep running, so requests that arrive while the GPU is busy
 // are admitted as soon as a slot and memory are free, and their work
 // overlaps on the device.  Completion is polled with events; results are
@@ -43,6 +46,7 @@
 #include "server.hpp"
 
 #include "batch.hpp"
+#include "bfs.hpp"
 #include "sssp.hpp"
 #include "runtime.hpp"
 #include "tc.hpp"
@@ -127,11 +131,12 @@ struct Route {
 
 // Requests estimated to run longer than this (microseconds on the whole
 // GPU) are long.
-constexpr double kLongUs = 2000.0;
+constexpr double kLongUs = 8000.0;  // estimated slot microseconds above which a request is long
+constexpr double kBatchWorkUs = 1e30;  // default: no cap on a launch's estimated work
 
 struct Slot {
   indigo_ctx* ctx = nullptr;
-  int state = 0;  // 0 idle, 1 batch, 2 triangle count, 3 single SSSP
+  int state = 0;  // 0 idle, 1 batch, 2 triangle count, 3 single BFS or SSSP
   bool long_work = false;  // the running launch holds long requests
   std::vector<int> reqs;
   std::vector<Piece> pieces;
@@ -139,6 +144,7 @@ struct Slot {
   std::vector<Route> routes;
   int tc_req = -1;
   int single_req = -1;
+  BfsStats bfs_stats;
 };
 
 struct Server {
@@ -159,7 +165,8 @@ struct Server {
   bool single(const indigo_request& r) const
   {
     const indigo_graph_ref& G = graphs[r.graph];
-    return r.alg == INDIGO_QUERY_SSSP && G.nodes >= (1 << 16) && static_cast<long long>(G.edges) < 6LL * G.nodes;
+    return (r.alg == INDIGO_QUERY_SSSP || r.alg == INDIGO_QUERY_BFS) && G.nodes >= (1 << 16) &&
+           static_cast<long long>(G.edges) < 6LL * G.nodes;
   }
 
   bool deep(const indigo_request& r) const
@@ -168,11 +175,15 @@ struct Server {
     return G.nodes >= (1 << 16) && static_cast<long long>(G.edges) < 6LL * G.nodes;
   }
 
-  // Estimated microseconds of one request on the whole GPU: CSR entries at
-  // the algorithm's rate on R-MAT scale 22 (BFS 300, SSSP 27, CC 500, MIS
-  // 134, PageRank 18, TC 9 GTEPS), plus a launch; BFS and SSSP on deep
-  // sparse graphs by their level count instead (about 2 sqrt(V) levels from
-  // a random source at 10 and 20 us per level).
+  // Estimated microseconds of one request's service on a slot (its share of
+  // the GPU, beside the other slots), fitted to the median service times of
+  // the 64-graph serving pool at 150 requests/s: a fixed part (launch, grid
+  // assembly, gang barriers; PageRank's sweeps make it 13 ms) plus CSR
+  // entries at a per-algorithm rate (entries per microsecond: BFS 10000,
+  // SSSP 3000, CC 50000, MIS 10000, PageRank 1600, TC 2000).  BFS and SSSP
+  // on deep sparse graphs run alone on the single-query kernels and are
+  // costed by their level count instead (about 2 sqrt(V) levels from a
+  // random source at 10 and 20 us per level).
   double est_us(const indigo_request& r) const
   {
     const indigo_graph_ref& G = graphs[r.graph];
@@ -181,16 +192,15 @@ struct Server {
       const double levels = 2.0 * std::sqrt(static_cast<double>(G.nodes));
       return levels * (r.alg == INDIGO_QUERY_BFS ? 10.0 : 20.0);
     }
-    double gteps = 300.0;
+    double fixed = 2000.0, rate = 10000.0;
     switch (r.alg) {
-      case INDIGO_QUERY_SSSP: gteps = 27.0; break;
-      case INDIGO_QUERY_CC: gteps = 500.0; break;
-      case INDIGO_QUERY_MIS: gteps = 134.0; break;
-      case INDIGO_QUERY_PR: gteps = 18.0; break;
-      case INDIGO_QUERY_TC: gteps = 9.0; break;
+      case INDIGO_QUERY_SSSP: rate = 3000.0; break;
+      case INDIGO_QUERY_CC: rate = 50000.0; break;
+      case INDIGO_QUERY_PR: fixed = 13000.0; rate = 1600.0; break;
+      case INDIGO_QUERY_TC: rate = 2000.0; break;
       default: break;
     }
-    return 50.0 + e / (gteps * 1e3);
+    return fixed + e / rate;
   }
 
   bool is_long(const indigo_request& r) const { return est_us(r) > kLongUs; }
@@ -215,6 +225,8 @@ struct Server {
     std::vector<char> taken(waiting.size(), 0);
     size_t need = 0;
     int count = 0;
+    double work = 0.0;  // estimated whole-GPU microseconds taken so far
+    const double work_cap = cfg.batch_work_us > 0 ? cfg.batch_work_us : kBatchWorkUs;
     for (size_t i = 0; i < waiting.size() && count < cfg.max_batch; i++) {
       const indigo_request& r = reqs[waiting[i]];
       if (r.alg == INDIGO_QUERY_TC || single(r) || is_long(r) != want_long) continue;
@@ -223,7 +235,13 @@ struct Server {
       const bool multi = r.alg == INDIGO_QUERY_BFS || r.alg == INDIGO_QUERY_SSSP;
       const int cap = r.alg == INDIGO_QUERY_BFS ? INDIGO_MAX_BFS_SOURCES : INDIGO_MAX_SSSP_SOURCES;
       auto it = multi ? open.find({r.alg, r.graph}) : open.end();
-      if (it != open.end() && static_cast<int>(builds[it->second].reqs.size()) < cap) {
+      const bool joins = it != open.end() && static_cast<int>(builds[it->second].reqs.size()) < cap;
+      // one more source of a multi-source task shares its traversal: a
+      // quarter of a request's estimate
+      const double w = joins ? 0.25 * est_us(r) : est_us(r);
+      if (count > 0 && work + w > work_cap) break;  // within a class, arrival order: no overtaking
+      work += w;
+      if (joins) {
         builds[it->second].reqs.push_back(waiting[i]);
       } else {
         builds.push_back(Build{r.alg, r.graph, {waiting[i]}});
@@ -303,13 +321,12 @@ struct Server {
     return INDIGO_OK;
   }
 
-  // An SSSP request on a deep graph alone on slot s; launched false if its
-  // output does not fit yet.
+  // A BFS or SSSP request on a deep graph alone on slot s; launched false if
+  // its output does not fit yet.
   int dispatch_single(Slot& s, int q, bool* launched)
   {
     *launched = false;
     const indigo_request& r = reqs[q];
-    const indigo_graph_ref& G = graphs[r.graph];
     size_t off = 0;
     if (!arena.alloc(out_bytes(r), &off)) return INDIGO_OK;
     waiting.erase(std::find(waiting.begin(), waiting.end(), q));
@@ -321,11 +338,10 @@ struct Server {
     s.reqs.assign(1, q);
     s.single_req = q;
     s.long_work = is_long(r);
-    IG_TRY(ensure_workspace(s.ctx, sssp_workspace_bytes(G.nodes)));
+    IG_TRY(ensure_workspace(s.ctx, single_workspace_bytes(r)));
     IG_TRY(ensure_host_pinned(s.ctx, 4096));
     IG_TRY(timing_begin(s.ctx));
-    IG_TRY(sssp_run(s.ctx, G.nodes, G.edges, G.d_nindex, G.d_nlist, G.d_eweight, r.src,
-                    reinterpret_cast<int*>(arena.ptr(off))));
+    IG_TRY(run_single(s, r, reinterpret_cast<int*>(arena.ptr(off))));
     IG_TRY(timing_record_stop(s.ctx));
     s.state = 3;
     stats.launches++;
@@ -333,20 +349,41 @@ struct Server {
     return INDIGO_OK;
   }
 
+  size_t single_workspace_bytes(const indigo_request& r) const
+  {
+    const int n = graphs[r.graph].nodes;
+    return r.alg == INDIGO_QUERY_BFS ? bfs_workspace_bytes(n) : sssp_workspace_bytes(n);
+  }
+
+  int run_single(Slot& s, const indigo_request& r, int* out)
+  {
+    const indigo_graph_ref& G = graphs[r.graph];
+    if (r.alg == INDIGO_QUERY_BFS) {
+      s.bfs_stats = BfsStats{};
+      return bfs_run(s.ctx, G.nodes, G.edges, G.d_nindex, G.d_nlist, r.src, out, &s.bfs_stats);
+    }
+    return sssp_run(s.ctx, G.nodes, G.edges, G.d_nindex, G.d_nlist, G.d_eweight, r.src, out);
+  }
+
+  int collect_single(Slot& s, const indigo_request& r)
+  {
+    if (r.alg == INDIGO_QUERY_BFS) return bfs_collect(s.ctx, &s.bfs_stats);
+    int it = 0, ep = 0;
+    return sssp_collect(s.ctx, &it, &ep);
+  }
+
   int complete(Slot& s, int* done)
   {
     if (s.state == 3) {
-      IG_TRY(timing_finish(s.ctx, 2, 0));
       const indigo_request& r = reqs[s.single_req];
-      const indigo_graph_ref& G = graphs[r.graph];
-      int it = 0, ep = 0;
-      int rc = sssp_collect(s.ctx, &it, &ep);
+      const int nl = r.alg == INDIGO_QUERY_BFS ? s.bfs_stats.launches : 2;
+      IG_TRY(timing_finish(s.ctx, nl, 0));
+      int rc = collect_single(s, r);
       while (rc == kNotResident) {  // the grid did not assemble: again, with the lowered cap
         IG_TRY(timing_begin(s.ctx));
-        IG_TRY(sssp_run(s.ctx, G.nodes, G.edges, G.d_nindex, G.d_nlist, G.d_eweight, r.src,
-                        reinterpret_cast<int*>(const_cast<void*>(s.routes[0].src))));
-        IG_TRY(timing_end(s.ctx, 2, 0));
-        rc = sssp_collect(s.ctx, &it, &ep);
+        IG_TRY(run_single(s, r, reinterpret_cast<int*>(const_cast<void*>(s.routes[0].src))));
+        IG_TRY(timing_end(s.ctx, r.alg == INDIGO_QUERY_BFS ? s.bfs_stats.launches : 2, 0));
+        rc = collect_single(s, r);
       }
       IG_TRY(rc);
       return finish_routes(s, done);
@@ -492,7 +529,7 @@ int serve_run(const indigo_graph_ref* graphs, int ngraphs, indigo_request* reqs,
   {
     const indigo_graph_ref& G = graphs[reqs[i].graph];
     if (reqs[i].alg == INDIGO_QUERY_TC) ws = std::max(ws, tc_workspace_bytes(G.nodes, G.edges));
-    if (sv.single(reqs[i])) ws = std::max(ws, sssp_workspace_bytes(G.nodes));
+    if (sv.single(reqs[i])) ws = std::max(ws, sv.single_workspace_bytes(reqs[i]));
   }
   for (Slot& s : sv.slots)
     if (rc == INDIGO_OK && s.ctx != nullptr) rc = ensure_workspace(s.ctx, std::max(ws, size_t(1) << 28));

@@ -823,8 +823,8 @@ int tc_run(indigo_ctx* ctx, int nodes, int edges, const int* nindex, const int* 
   int* hcnt = cv.take<int>(hcnt_words(edges));
   IG_TRY(zero_async(ctx, ctl, sizeof(TcControl)));
   hipStream_t s = ctx->stream;
-  const int gw = grid_for((nodes + kWave - 1) / kWave, kTcBlock / kWave, 1 << 16);
-  const int gh = ctx->compute_units * 8;
+  const int gw = share_grid(ctx, grid_for((nodes + kWave - 1) / kWave, kTcBlock / kWave, 1 << 16), kTcBlock);
+  const int gh = share_grid(ctx, ctx->compute_units * 8, kTcBlock);
   const OutSel out{nindex};
   const OutRec orec{vert, orng};
   tc_select<true><<<gw, kTcBlock, 0, s>>>(nodes, nindex, nlist, out, orec, olist, heavy, ctl);
@@ -840,16 +840,17 @@ int tc_run(indigo_ctx* ctx, int nodes, int edges, const int* nindex, const int* 
     // least 1), so a small graph still spreads over 4096 grabs while the
     // grabs' same-address atomics stay few
     const int grab = std::max(1, std::min(kWave, nodes / 4096));
-    tc_count<<<ctx->compute_units * 8, kTcBlock, 0, s>>>(nodes, grab, orng, olist, ilen, ilist, mid, big, ctl);
+    tc_count<<<share_grid(ctx, ctx->compute_units * 8, kTcBlock), kTcBlock, 0, s>>>(nodes, grab, orng, olist, ilen,
+                                                                                   ilist, mid, big, ctl);
   }
   const int hint = ctx->hints[INDIGO_HINT_TC_LDS_LIST];
   const int mid_list = hint > 0 && hint < kMidSlots / 4 ? hint : kMidSlots / 4;
   const int big_list = hint > 0 && hint < kBigSlots / 4 ? hint : kBigSlots / 4;
   IG_TRY(kernel_begin(ctx, "tc_count_group<512, 8192>"));
-  tc_count_group<kMidBlock, kMidSlots><<<ctx->compute_units * 4, kMidBlock, 0, s>>>(
+  tc_count_group<kMidBlock, kMidSlots><<<share_grid(ctx, ctx->compute_units * 4, kMidBlock), kMidBlock, 0, s>>>(
       orng, olist, ilen, ilist, mid, &ctl->nmid, &ctl->next_mid, mid_list, ctl);
   IG_TRY(kernel_end(ctx));
-  tc_count_group<kBigBlock, kBigSlots><<<ctx->compute_units, kBigBlock, 0, s>>>(
+  tc_count_group<kBigBlock, kBigSlots><<<share_grid(ctx, ctx->compute_units, kBigBlock), kBigBlock, 0, s>>>(
       orng, olist, ilen, ilist, big, &ctl->nbig, &ctl->next_big, big_list, ctl);
   IG_HIP_TRY(hipGetLastError());
   if (launches) *launches = 10;

@@ -127,7 +127,7 @@ class RequestStruct(ctypes.Structure):
 
 class ServeConfigStruct(ctypes.Structure):
     _fields_ = [("device", c_int), ("slots", c_int), ("max_batch", c_int), ("mem_budget", c_size_t),
-                ("progress_interval", c_double), ("cu_partition", c_int)]
+                ("progress_interval", c_double), ("cu_partition", c_int), ("batch_work_us", c_int)]
 
 
 class ServeStatsStruct(ctypes.Structure):

@@ -251,7 +251,7 @@ def graph_table(graphs: list[DeviceGraph]):
 
 def serve(graphs: list[DeviceGraph], algs, graph_ids, sources, arrivals, outs=None, slots: int = 4,
           max_batch: int = 256, mem_budget: int = 1 << 30, device: int = 0, progress: float = 0.0,
-          cu_partition: bool = False):
+          cu_partition: bool = False, batch_work_us: int = 0):
     """Replays requests through indigo_serve.  algs: names ("bfs", "sssp",
     "cc", "mis", "pr", "tc"); outs: optional DeviceBuffer per request (None
     for a discarded result).  Returns (admitted, finished, batch, slot) numpy
@@ -264,7 +264,7 @@ def serve(graphs: list[DeviceGraph], algs, graph_ids, sources, arrivals, outs=No
         rq[i].src = int(sources[i])
         rq[i].arrival = float(arrivals[i])
         rq[i].d_out = outs[i].ptr if outs is not None and outs[i] is not None else None
-    cfg = _lib.ServeConfigStruct(device, slots, max_batch, mem_budget, progress, int(cu_partition))
+    cfg = _lib.ServeConfigStruct(device, slots, max_batch, mem_budget, progress, int(cu_partition), batch_work_us)
     st = _lib.ServeStatsStruct()
     _lib.call("indigo_serve", graph_table(graphs), len(graphs), rq, n, ctypes.byref(cfg), ctypes.byref(st))
     admitted = np.array([rq[i].admitted for i in range(n)])

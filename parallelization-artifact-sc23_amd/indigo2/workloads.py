@@ -68,17 +68,21 @@ def serving_requests(graphs: list[HostGraph], nq: int, seed: int = 5):
     return algs, gid, src, gaps
 
 
-# Rates (GTEPS on R-MAT scale 22, whole GPU) of the serving loop's cost model.
-_SERVE_GTEPS = {"bfs": 300.0, "sssp": 27.0, "cc": 500.0, "mis": 134.0, "pr": 18.0, "tc": 9.0}
-SERVE_LONG_US = 2000.0
+# The serving loop's cost model (csrc/server.hip est_us): per algorithm a
+# fixed part in microseconds and a rate in CSR entries per microsecond,
+# fitted to the median service times of the serving pool at 150 requests/s.
+_SERVE_COST = {"bfs": (2000.0, 10000.0), "sssp": (2000.0, 3000.0), "cc": (2000.0, 50000.0),
+               "mis": (2000.0, 10000.0), "pr": (13000.0, 1600.0), "tc": (2000.0, 2000.0)}
+SERVE_LONG_US = 8000.0
 
 
 def serving_cost_us(g: HostGraph, alg: str) -> float:
-    """The serving loop's estimate of one request's microseconds on the
-    whole GPU (the same rule as indigo_serve's admission): CSR entries at the
-    algorithm's rate, or 2 sqrt(V) levels at 10 / 20 us for BFS / SSSP on
-    deep sparse graphs (average degree below 6, at least 65536 vertices).
-    Requests above SERVE_LONG_US are long."""
+    """The serving loop's estimate of one request's microseconds of service
+    on a slot (the same rule as indigo_serve's admission): a fixed part plus
+    CSR entries at the algorithm's rate, or 2 sqrt(V) levels at 10 / 20 us
+    for BFS / SSSP on deep sparse graphs (average degree below 6, at least
+    65536 vertices).  Requests above SERVE_LONG_US are long."""
     if g.nodes >= (1 << 16) and g.edges < 6 * g.nodes and alg in ("bfs", "sssp"):
         return 2.0 * float(np.sqrt(g.nodes)) * (10.0 if alg == "bfs" else 20.0)
-    return 50.0 + g.edges / (_SERVE_GTEPS[alg] * 1e3)
+    fixed, rate = _SERVE_COST[alg]
+    return fixed + g.edges / rate

@@ -214,7 +214,8 @@ def test_copy_ms_copies_and_times(ig, ctx):
 def test_persistent_grid_cap(ig, orc, ctx, cap):
     """INDIGO_HINT_MAX_WORKGROUPS (contexts that share the GPU, as the serving
     slots do): the persistent kernels give the same results on a grid of one
-    workgroup and on an odd grid smaller than the eight barrier groups."""
+    workgroup and on an odd grid smaller than the eight barrier groups, and
+    CC's and TC's ordinary kernels on grids held to the same threads."""
     g = ig.rmat(13, 16, seed=5, wmin=1, wmax=255)
     road = ig.road(80, 60, keep_p=0.8, seed=2, wmin=1, wmax=1000)
     ctx.set_hint(ig._lib.HINT_MAX_WORKGROUPS, cap)
@@ -226,5 +227,7 @@ def test_persistent_grid_cap(ig, orc, ctx, cap):
             assert np.array_equal(ctx.mis(dg), orc.mis(h)), (h.name, "mis")
             scores, iters = ctx.pagerank(dg)
             _pr_check(ig, orc, h, scores, iters)
+            assert np.array_equal(ctx.cc(dg), orc.cc(h)), (h.name, "cc")
+            assert ctx.tc(dg) == orc.tc(h), (h.name, "tc")
     finally:
         ctx.set_hint(ig._lib.HINT_MAX_WORKGROUPS, 0)
