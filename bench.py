@@ -44,12 +44,23 @@ def tc_count_model(g) -> dict:
     longer out-list, iterating the shorter one (ties: v); the task of x is
     its out-list plus its in-list.  Tasks with 2 <= |N+(x)| <= 256 and
     (|N+| + |N-|) * |N+| <= 32768 run in the wave pass, the rest with
-    |N+(x)| <= 2048 in this kernel, in parts of 1024 items.  Algorithmic
-    bytes per launch: 4 per probe (the element of the iterated list), 12 per
-    item (its id and offset pair), 4 per table entry (N+(x) per part) and 16
-    per part (task header)."""
+    |N+(x)| <= 2048 in this kernel, in parts of 1024 items.  Hubs are the at
+    most 65536 vertices of largest degree (all vertices when they fit, else
+    min(degree, 65536) >= the smallest bin whose suffix holds at most 65536
+    vertices); a hub's out-list holds hubs only and is stored as 16-bit keys.
+    Algorithmic bytes per launch: per probe the element of the iterated list
+    (2 from a hub's list, else 4), 12 per item (its id and offset pair), per
+    table entry (N+(x) per part) 2 for a hub's task, else 4, and 16 per part
+    (task header)."""
     v = g.nodes
     deg = g.degree.astype(np.int64)
+    if v <= 65536:
+        hub = np.ones(v, dtype=bool)
+    else:
+        dcl = np.minimum(deg, 65536)
+        suffix = np.cumsum(np.bincount(dcl, minlength=65537)[::-1])[::-1]
+        ok = np.flatnonzero(suffix <= 65536)
+        hub = dcl >= (ok[0] if ok.size else 65537)
     src = np.repeat(np.arange(v, dtype=np.int64), deg)
     dst = g.nlist.astype(np.int64)
     rank = deg * v + np.arange(v, dtype=np.int64)
@@ -60,14 +71,18 @@ def tc_count_model(g) -> dict:
     to_d = oc[s] < oc[d]
     inl = np.bincount(d[to_d], minlength=v).astype(np.int64)
     owner = np.where(to_d, d, s)
-    probes = np.bincount(owner, weights=np.where(to_d, oc[s], oc[d]), minlength=v)
+    walked = np.where(to_d, s, d)  # the endpoint whose out-list is iterated
+    probes = np.bincount(owner, weights=oc[walked], minlength=v)
+    pbytes = np.bincount(owner, weights=oc[walked] * np.where(hub[walked], 2.0, 4.0), minlength=v)
     items = oc + inl
     wave = (oc >= 2) & (oc <= 256) & (items * oc <= 32768)
     mid = (oc >= 2) & ~wave & (oc <= 2048)
     parts = (items + 1023) // 1024
     m = {"probes": float(probes[mid].sum()), "items": int(items[mid].sum()), "parts": int(parts[mid].sum()),
-         "table_entries": int((oc * parts)[mid].sum()), "probes_all_passes": float(probes.sum())}
-    m["bytes"] = 4 * m["probes"] + 12 * m["items"] + 4 * m["table_entries"] + 16 * m["parts"]
+         "table_entries": int((oc * parts)[mid].sum()), "probes_all_passes": float(probes.sum()),
+         "hub_tasks": int((mid & hub).sum()), "tasks": int(mid.sum())}
+    tbytes = float((oc * parts * np.where(hub, 2, 4))[mid].sum())
+    m["bytes"] = float(pbytes[mid].sum()) + 12 * m["items"] + tbytes + 16 * m["parts"]
     return m
 
 

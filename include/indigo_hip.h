@@ -187,7 +187,8 @@ int indigo_ctx_set_hint(indigo_ctx* ctx, int key, int value);
 #define INDIGO_HINT_SMALL_FRONTIER 3 /* 0 auto, 1 off: never use the one-workgroup mode (BFS, SSSP) */
 #define INDIGO_HINT_BUDGET_MS 4      /* bound on device-side waits in ms (0: 60000) */
 #define INDIGO_HINT_BLOCKS_PER_CU 5  /* cap on co-resident workgroups per CU of persistent kernels (0: occupancy) */
-#define INDIGO_HINT_TC_LDS_LIST 6    /* longest oriented list TC hashes in LDS (0: 16384); longer ones use binary search */
+#define INDIGO_HINT_TC_LDS_LIST 6    /* entries of an oriented list TC hashes in LDS at once (0: 2048 or 8192 by
+                                        list length); longer lists are hashed that many at a time */
 #define INDIGO_HINT_RESIDENCY_MS 7   /* how long a persistent grid may take to assemble (0: 100 ms); longer when
                                         the GPU is shared with other work */
 #define INDIGO_HINT_MAX_WORKGROUPS 8 /* cap on the grids of the context, in workgroups of 512 threads (other
@@ -198,6 +199,7 @@ int indigo_ctx_set_hint(indigo_ctx* ctx, int key, int value);
 #define INDIGO_HINT_GRAPH 11        /* 1: CC and TC run their kernel sequence as a HIP graph, captured on the
                                         first call with the same graph, output and workspace and replayed after
                                         (one launch instead of 5-10); such calls report no separate main kernel */
+#define INDIGO_HINT_TC_HUBS 12       /* cap on TC's hub vertices, whose out-lists hold 16-bit keys (0: 65536) */
 #define INDIGO_VARIANT_VERTEX 0      /* vertex-centric, data-driven (degree-binned worklists); the default */
 #define INDIGO_VARIANT_EDGE 1        /* edge-centric, topology-driven: one lane per CSR entry over an edge-source
                                         array built on the device (the suite's E_Topo_Push family) */

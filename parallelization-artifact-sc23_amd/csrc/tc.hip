@@ -19,38 +19,56 @@
 // so every oriented edge is taken exactly once, by its endpoint with the
 // longer list (2.4x fewer probes than always iterating N+(u) on R-MAT).
 //
+// Keys.  The hubs are the at most 65536 vertices of largest degree (a
+// degree threshold: every vertex above it is a hub), and they are numbered
+// 0.. in id order; every other vertex v is named by the key 65536 + v.  The
+// hubs are closed upwards in the (degree, id) order, so the out-list of a hub
+// holds hubs only and is stored as 16-bit keys; on R-MAT graphs three
+// quarters of all probes read such lists, which halves their bytes.  Lists
+// hold keys, the LDS tables hold keys, and orngk[key] locates a list.
+//
 // Passes (plain launches on the context stream).  Both oriented lists of a
 // vertex are subsets of its adjacency list, so each is stored at the
-// vertex's own CSR position (olist / ilist[nindex[v] + k]) and one pass per
-// list builds it - no count pass, no scan:
-//   1. out-select : N+ lists, order-preserving (so still sorted); lists
-//                  < kMidDeg by the in-wave balancer, longer ones binned
-//                  into mid (one wave each) and huge (kHugeChunk-entry
-//                  items, one workgroup each, compacted per item); also
-//                  flags unsorted lists and self loops;  records
-//                  (deg, |N+|) per vertex;
+// vertex's own CSR position (olist / ilist[nindex[v] + k], or the 16-bit
+// entries of the same bytes for a hub's out-list) and one pass per list
+// builds it - no count pass, no scan:
+//   0. keys       : degree histogram, hub threshold, hub numbering (a scan
+//                  of the hub flags) and (degree, key) per vertex; on a
+//                  graph of at most 65536 vertices every vertex is a hub and
+//                  its key is its id, in one pass;
+//   1. out-select : N+ lists, order-preserving; lists < kMidDeg by the
+//                  in-wave balancer, longer ones binned into mid (one wave
+//                  each) and huge (kHugeChunk-entry items, one workgroup
+//                  each, compacted per item); also flags unsorted lists and
+//                  self loops;  records (deg, |N+|, key) per vertex;
 //   2. pack       : one workgroup per huge list closes the gaps between its
 //                  items;
 //   3-4.           the same for the in-lists N-(x), reading each
-//                  neighbour's (deg, |N+|) in one 8-byte load;
-//   5. count, wave pass: a wave grabs 64 consecutive vertices at a time from
-//      a global counter (a static stride would be badly unbalanced: on R-MAT
-//      graphs a vertex's degree follows the bit pattern of its id); a task
-//      with 2 <= |N+(x)| <= kWaveList and at most kWaveWork probes
+//                  neighbour's (deg, |N+|, key) in one 16-byte load;
+//   5. count, wave pass: a wave grabs up to 64 consecutive vertices at a time
+//      from a global counter (a static stride would be badly unbalanced: on
+//      R-MAT graphs a vertex's degree follows the bit pattern of its id); a
+//      task with 2 <= |N+(x)| <= kWaveList and at most kWaveWork probes
 //      ((|N+(x)| + |N-(x)|) * |N+(x)| bounds them) builds an open-addressing
-//      table of N+(x) in LDS (load factor <= 1/4, Fibonacci hashing, linear
-//      probing) and walks the (list, element) pairs 64 per step with the
-//      in-wave balancer, one probe each.  Every other task is cut into parts
-//      of kPartItems items (out-list, then in-list), queued for pass 6;
+//      table of N+(x) in LDS and walks the (list, element) pairs 64 per step
+//      with the in-wave balancer, one probe each.  Every other task is cut
+//      into parts of kPartItems items (out-list, then in-list), queued for
+//      pass 6;
 //   6. count, workgroup passes: a workgroup grabs a part, hashes N+(x) and
 //      spreads the part's items over its waves (512 threads and a 32 KiB
 //      table up to kMidList entries, 1024 threads and a 128 KiB table
 //      beyond).  Parts bound the work of one grab, so a few heavy vertices
 //      cannot form a tail.  A list too long for the table (m > 2^25 edges, or
-//      a hint) is searched by binary search in global memory instead.
+//      a hint) is hashed a table's worth at a time, and the part's items are
+//      probed against each piece in turn (N+(x) is the disjoint union of the
+//      pieces, so the counts add).
 #include "tc.hpp"
 
+#include <algorithm>
+#include <climits>
+
 #include "frontier.hpp"
+#include "scan.hpp"
 
 namespace indigo {
 
@@ -77,6 +95,10 @@ constexpr int kTinySlots = 64;
 constexpr int kWaveRegion = kWaveSlots;  // a wave's LDS words
 static_assert(4 * kTinyList <= kTinySlots && kTinyGroup * kTinySlots <= kWaveRegion && kTinyList < kLongList,
               "tiny-task tables: 64 slots each, all in the wave's region, no long lists");
+constexpr int kHubKeys = 1 << 16;  // keys below this number hubs (16-bit rows); others are 65536 + id
+constexpr int kHistLds = 2048;     // degree bins a workgroup counts in LDS
+constexpr int kThreshBlock = 1024;
+static_assert(kHubKeys % kThreshBlock == 0, "the threshold pass gives every thread a whole run of bins");
 
 struct TcControl {
   FrontierCounters heavy;  // lists of >= kMidDeg entries
@@ -84,32 +106,57 @@ struct TcControl {
   unsigned nmid, nbig;     // tasks deferred to the workgroup passes
   unsigned next, next_mid, next_big;
   int error;
-  int pad[24];
+  int hub_bin;             // hubs: min(degree, kHubKeys) >= hub_bin
+  int pad[23];
 };
 
 IG_DEV bool outranks(int du, int u, int dv, int v) { return du > dv || (du == dv && u > v); }
 
-// Orientation state of a vertex, written by the out-selection: its degree and
-// |N+(v)|.  The in-selection reads it for every neighbour in one 8-byte load.
-struct alignas(8) TcVert {
-  int deg, odeg;
+IG_DEV bool narrow_key(int key) { return key < kHubKeys; }
+
+// Entry k of the row at list + b: 16-bit keys (a hub's out-list, in the first
+// half of the row's bytes) or 32-bit keys.
+IG_DEV int row_get(const int* list, int b, int k, bool narrow)
+{
+  return narrow ? static_cast<int>(reinterpret_cast<const unsigned short*>(list + b)[k]) : list[b + k];
+}
+IG_DEV void row_put(int* list, int b, int k, int key, bool narrow)
+{
+  if (narrow) reinterpret_cast<unsigned short*>(list + b)[k] = static_cast<unsigned short>(key);
+  else list[b + k] = key;
+}
+
+// Orientation state of a vertex, written by the out-selection: its degree,
+// |N+(v)| and key.  The in-selection reads it for every neighbour in one
+// 16-byte load.
+struct alignas(16) TcVert {
+  int deg, odeg, key, pad;
 };
 
-// Out-list of v: the neighbours that outrank v.
+// Out-list of v: the keys of the neighbours that outrank v; 16-bit when v is
+// a hub.
 struct OutSel {
-  const int* __restrict__ nindex;
+  const int2* __restrict__ dk;  // (degree, key) per vertex
   IG_DEV int own_odeg(int) const { return 0; }
-  IG_DEV bool operator()(int v, int dv, int, int u) const { return outranks(nindex[u + 1] - nindex[u], u, dv, v); }
+  IG_DEV bool narrow(int v) const { return narrow_key(dk[v].y); }
+  IG_DEV bool operator()(int v, int dv, int, int u, int& key) const
+  {
+    const int2 r = dk[u];
+    key = r.y;
+    return outranks(r.x, u, dv, v);
+  }
 };
 
-// In-list of v: the neighbours u that v outranks (u -> v) whose out-list is
-// shorter than v's.
+// In-list of v: the keys of the neighbours u that v outranks (u -> v) whose
+// out-list is shorter than v's; always 32-bit.
 struct InSel {
   const TcVert* __restrict__ vert;
   IG_DEV int own_odeg(int v) const { return vert[v].odeg; }
-  IG_DEV bool operator()(int v, int dv, int odv, int u) const
+  IG_DEV bool narrow(int) const { return false; }
+  IG_DEV bool operator()(int v, int dv, int odv, int u, int& key) const
   {
     const TcVert k = vert[u];
+    key = k.key;
     return !outranks(k.deg, u, dv, v) && k.odeg < odv;
   }
 };
@@ -117,18 +164,97 @@ struct InSel {
 // Where a finished selection is recorded.  Lists live at their vertex's own
 // CSR position (list[nindex[v] + k]), so no count pass or scan is needed.
 struct OutRec {
+  const int2* __restrict__ dk;
   TcVert* vert;
-  int2* orng;  // (start, |N+(v)|)
+  int2* orng;   // (start, |N+(v)|) by vertex id
+  int2* orngk;  // the same by key
   IG_DEV void operator()(int v, int dv, int beg, int kept) const
   {
-    vert[v] = TcVert{dv, kept};
+    const int key = dk[v].y;
+    vert[v] = TcVert{dv, kept, key, 0};
     orng[v] = make_int2(beg, kept);
+    orngk[key] = make_int2(beg, kept);
   }
 };
 struct InRec {
   int* ilen;  // |N-(v)|
   IG_DEV void operator()(int v, int, int, int kept) const { ilen[v] = kept; }
 };
+
+// Degree histogram over min(degree, kHubKeys): the low bins, where most
+// vertices fall, per workgroup in LDS, the rest straight to `hist`.
+__global__ __launch_bounds__(kTcBlock) void tc_degree_hist(int nodes, const int* __restrict__ nindex,
+                                                           unsigned* hist)
+{
+  __shared__ unsigned s_h[kHistLds];
+  for (int i = threadIdx.x; i < kHistLds; i += kTcBlock) s_h[i] = 0;
+  __syncthreads();
+  for (int v = blockIdx.x * kTcBlock + threadIdx.x; v < nodes; v += gridDim.x * kTcBlock) {
+    const int d = min(nindex[v + 1] - nindex[v], kHubKeys);
+    if (d < kHistLds) atomicAdd(&s_h[d], 1u);
+    else atomic_add_relaxed(&hist[d], 1u);
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < kHistLds; i += kTcBlock)
+    if (s_h[i] != 0) atomic_add_relaxed(&hist[i], s_h[i]);
+}
+
+// One workgroup: hub_bin = the smallest bin b whose suffix (the vertices in
+// bins >= b) holds at most `cap` vertices; kHubKeys + 1 (no hubs) when even
+// the top bin holds more.  Suffixes only grow towards lower bins, so each
+// thread walks its run of 64 bins downwards until the cap is passed.
+__global__ __launch_bounds__(kThreshBlock) void tc_hub_threshold(const unsigned* __restrict__ hist, int cap,
+                                                                 TcControl* ctl)
+{
+  constexpr int kPer = kHubKeys / kThreshBlock;
+  constexpr int kWaves = kThreshBlock / kWave;
+  __shared__ int s_wave[kWaves];
+  __shared__ int s_best;
+  const int t = threadIdx.x, lane = lane_id(), wid = t / kWave;
+  const int lo = t * kPer;
+  const int hi = t == kThreshBlock - 1 ? kHubKeys + 1 : lo + kPer;  // the top bin joins the last run
+  int s = 0;
+  for (int b = lo; b < hi; b++) s += static_cast<int>(hist[b]);
+  // suffix sums over the lanes: an inclusive scan of the lanes in reverse
+  const int inc = wave_inclusive_scan(wave_shfl(s, kWave - 1 - lane));
+  const int suf = wave_shfl(inc, kWave - 1 - lane);  // s of this lane and the lanes above
+  if (lane == 0) s_wave[wid] = suf;
+  if (t == 0) s_best = kHubKeys + 1;
+  __syncthreads();
+  int acc = suf - s;  // vertices in bins >= hi
+  for (int w = wid + 1; w < kWaves; w++) acc += s_wave[w];
+  int best = kHubKeys + 1;
+  for (int b = hi - 1; b >= lo; b--) {
+    acc += static_cast<int>(hist[b]);
+    if (acc > cap) break;
+    best = b;
+  }
+  if (best <= kHubKeys) atomicMin(&s_best, best);
+  __syncthreads();
+  if (t == 0) ctl->hub_bin = s_best;
+}
+
+__global__ __launch_bounds__(kTcBlock) void tc_hub_flags(int nodes, const int* __restrict__ nindex,
+                                                         const TcControl* ctl, int* flags)
+{
+  const int hb = ctl->hub_bin;
+  for (int v = blockIdx.x * kTcBlock + threadIdx.x; v < nodes; v += gridDim.x * kTcBlock)
+    flags[v] = min(nindex[v + 1] - nindex[v], kHubKeys) >= hb ? 1 : 0;
+}
+
+// (degree, key) per vertex; hidx: the exclusive scan of the hub flags, or
+// null when every vertex is a hub (then a hub's number is its id).
+__global__ __launch_bounds__(kTcBlock) void tc_hub_keys(int nodes, const int* __restrict__ nindex,
+                                                        const TcControl* ctl, const int* __restrict__ hidx,
+                                                        int2* dk)
+{
+  const int hb = hidx ? ctl->hub_bin : 0;
+  for (int v = blockIdx.x * kTcBlock + threadIdx.x; v < nodes; v += gridDim.x * kTcBlock) {
+    const int d = nindex[v + 1] - nindex[v];
+    const int key = !hidx ? v : (min(d, kHubKeys) >= hb ? hidx[v] : kHubKeys + v);
+    dk[v] = make_int2(d, key);
+  }
+}
 
 // Selection over short lists (< kMidDeg): a wave takes 64 consecutive
 // vertices and walks their CSR entries 64 per step with the in-wave balancer;
@@ -137,7 +263,7 @@ struct InRec {
 // sorted.  The out pass (kOut) also bins the long lists into the mid / huge
 // queues and flags unsorted lists and self loops.
 template <bool kOut, typename Sel, typename Rec>
-__global__ __launch_bounds__(kTcBlock) void tc_select(int nodes, const int* __restrict__ nindex,
+__global__ __launch_bounds__(kTcBlock) IG_SGPR_CAP void tc_select(int nodes, const int* __restrict__ nindex,
                                                       const int* __restrict__ nlist, Sel sel, Rec rec, int* list,
                                                       FrontierQueues heavy, TcControl* ctl)
 {
@@ -156,6 +282,7 @@ __global__ __launch_bounds__(kTcBlock) void tc_select(int nodes, const int* __re
     if (kOut) frontier_emit_block(s_emit, long_list, v, dv, heavy, &ctl->heavy);
     const bool mine = valid && !long_list;
     const int odv = mine ? sel.own_odeg(v) : 0;
+    const int nar = mine && sel.narrow(v) ? 1 : 0;
     const int slen = mine ? dv : 0;
     const int incl = wave_inclusive_scan(slen);
     const int excl = incl - slen;
@@ -169,20 +296,21 @@ __global__ __launch_bounds__(kTcBlock) void tc_select(int nodes, const int* __re
       const int odeg = wave_shfl(dv, o);
       const int oodv = wave_shfl(odv, o);
       const int okept = wave_shfl(kept, o);
+      const bool onar = wave_shfl(nar, o) != 0;
       const int ov = c * kWave + o;
       bool keep = false;
-      int u = 0;
+      int key = 0;
       if (k < total) {
         const int e = obeg + (k - oexcl);
-        u = nlist[e];
+        const int u = nlist[e];
         if (kOut) bad |= (u == ov) || (e > obeg && nlist[e - 1] >= u);
-        keep = sel(ov, odeg, oodv, u);
+        keep = sel(ov, odeg, oodv, u, key);
       }
       const unsigned long long m = __ballot(keep);
       // rank among the kept entries of the same owner in this step
       const int seg0 = min(max(oexcl - base, 0), kWave - 1);
       const unsigned long long mine_below = m & ((1ull << lane) - 1ull) & ~((1ull << seg0) - 1ull);
-      if (keep) list[obeg + okept + __popcll(mine_below)] = u;
+      if (keep) row_put(list, obeg, okept + __popcll(mine_below), key, onar);
       // each owner adds its lanes' kept entries of this step
       const int s0 = min(max(excl - base, 0), kWave), s1 = min(max(incl - base, 0), kWave);
       const int w = s1 - s0;
@@ -202,7 +330,7 @@ __global__ __launch_bounds__(kTcBlock) void tc_select(int nodes, const int* __re
 // workgroup per kHugeChunk-entry item, compacted in order at the item's own
 // position with the item's count in hcnt; tc_pack_huge then closes the gaps.
 template <bool kOut, typename Sel, typename Rec>
-__global__ __launch_bounds__(kTcBlock) void tc_select_heavy(const int* __restrict__ nindex,
+__global__ __launch_bounds__(kTcBlock) IG_SGPR_CAP void tc_select_heavy(const int* __restrict__ nindex,
                                                             const int* __restrict__ nlist, Sel sel, Rec rec,
                                                             int* list, FrontierQueues heavy, int* hcnt, TcControl* ctl)
 {
@@ -221,17 +349,19 @@ __global__ __launch_bounds__(kTcBlock) void tc_select_heavy(const int* __restric
     const int vb = nindex[v], ve = nindex[v + 1];
     const int dv = ve - vb;
     const int odv = sel.own_odeg(v);
-    const int beg = vb + static_cast<int>(item - heavy.hoff[j]) * kHugeChunk;
+    const bool nar = sel.narrow(v);
+    const int off = static_cast<int>(item - heavy.hoff[j]) * kHugeChunk;  // the item's first entry in the row
+    const int beg = vb + off;
     const int end = min(ve, beg + kHugeChunk);
     int run = 0;
     for (int base = beg; base < end; base += kTcBlock) {
       const int e = base + static_cast<int>(threadIdx.x);
       bool keep = false;
-      int u = 0;
+      int key = 0;
       if (e < end) {
-        u = nlist[e];
+        const int u = nlist[e];
         if (kOut) bad |= (u == v) || (e > vb && nlist[e - 1] >= u);
-        keep = sel(v, dv, odv, u);
+        keep = sel(v, dv, odv, u, key);
       }
       const unsigned long long m = __ballot(keep);
       if (lane == 0) s_wcnt[wid] = __popcll(m);
@@ -241,7 +371,7 @@ __global__ __launch_bounds__(kTcBlock) void tc_select_heavy(const int* __restric
         before += w < wid ? s_wcnt[w] : 0;
         total += s_wcnt[w];
       }
-      if (keep) list[beg + run + before + lanes_below(m)] = u;
+      if (keep) row_put(list, vb, off + run + before + lanes_below(m), key, nar);
       run += total;
       __syncthreads();
     }
@@ -253,6 +383,7 @@ __global__ __launch_bounds__(kTcBlock) void tc_select_heavy(const int* __restric
     const int vb = nindex[v], ve = nindex[v + 1];
     const int dv = ve - vb;
     const int odv = sel.own_odeg(v);
+    const bool nar = sel.narrow(v);
     int k = 0;
     // kUnroll windows per step: their list and neighbour-state loads overlap
     for (int base = vb; base < ve; base += kUnroll * kWave) {
@@ -264,12 +395,16 @@ __global__ __launch_bounds__(kTcBlock) void tc_select_heavy(const int* __restric
         if (kOut && e < ve) bad |= (u[q] == v) || (e > vb && nlist[e - 1] >= u[q]);
       }
       bool keep[kUnroll];
+      int key[kUnroll];
 #pragma unroll
-      for (int q = 0; q < kUnroll; q++) keep[q] = u[q] >= 0 && sel(v, dv, odv, u[q]);
+      for (int q = 0; q < kUnroll; q++) {
+        key[q] = 0;
+        keep[q] = u[q] >= 0 && sel(v, dv, odv, u[q], key[q]);
+      }
 #pragma unroll
       for (int q = 0; q < kUnroll; q++) {
         const unsigned long long m = __ballot(keep[q]);
-        if (keep[q]) list[vb + k + lanes_below(m)] = u[q];
+        if (keep[q]) row_put(list, vb, k + lanes_below(m), key[q], nar);
         k += __popcll(m);
       }
     }
@@ -283,34 +418,34 @@ __global__ __launch_bounds__(kTcBlock) void tc_select_heavy(const int* __restric
 // the list.  An item's entries are all read before any is written, and its
 // destination never reaches past its own start, so nothing unread is
 // overwritten.
-template <typename Rec>
+template <typename Sel, typename Rec>
 __global__ __launch_bounds__(kTcBlock) void tc_pack_huge(const int* __restrict__ nindex, int* list,
-                                                         FrontierQueues heavy, const int* __restrict__ hcnt, Rec rec,
-                                                         const TcControl* ctl)
+                                                         FrontierQueues heavy, const int* __restrict__ hcnt, Sel sel,
+                                                         Rec rec, const TcControl* ctl)
 {
   constexpr int kPer = kHugeChunk / kTcBlock;
   const unsigned nh = huge_count(ctl->heavy.hpack);
   for (unsigned j = blockIdx.x; j < nh; j += gridDim.x) {
     const int v = heavy.hq[j];
     const int vb = nindex[v], dv = nindex[v + 1] - vb;
+    const bool nar = sel.narrow(v);
     const unsigned c0 = heavy.hoff[j];
     const int nc = (dv + kHugeChunk - 1) / kHugeChunk;
     int run = 0;
     for (int c = 0; c < nc; c++) {
       const int k = hcnt[c0 + c];
-      const int src = vb + c * kHugeChunk;
       int buf[kPer];
 #pragma unroll
       for (int q = 0; q < kPer; q++) {
         const int i = q * kTcBlock + static_cast<int>(threadIdx.x);
-        buf[q] = i < k ? list[src + i] : 0;
+        buf[q] = i < k ? row_get(list, vb, c * kHugeChunk + i, nar) : 0;
       }
       __syncthreads();
       if (run != c * kHugeChunk) {
 #pragma unroll
         for (int q = 0; q < kPer; q++) {
           const int i = q * kTcBlock + static_cast<int>(threadIdx.x);
-          if (i < k) list[vb + run + i] = buf[q];
+          if (i < k) row_put(list, vb, run + i, buf[q], nar);
         }
       }
       __syncthreads();
@@ -320,39 +455,33 @@ __global__ __launch_bounds__(kTcBlock) void tc_pack_huge(const int* __restrict__
   }
 }
 
-// list[k] with a 32-bit byte offset from a wave-uniform base (k < 2^30):
-// the load takes the base from scalar registers.
-IG_DEV int load_at(const int* list, int k)
+// row[k] with a 32-bit byte offset from a wave-uniform base (k < 2^30): the
+// load takes the base from scalar registers.
+IG_DEV int load_at(const int* row, int k)
 {
-  return *reinterpret_cast<const int*>(reinterpret_cast<const char*>(list) + static_cast<unsigned>(k) * 4u);
+  return *reinterpret_cast<const int*>(reinterpret_cast<const char*>(row) + static_cast<unsigned>(k) * 4u);
+}
+IG_DEV int load_at(const unsigned short* row, int k)
+{
+  return *reinterpret_cast<const unsigned short*>(reinterpret_cast<const char*>(row) + static_cast<unsigned>(k) * 2u);
 }
 
-// No vertex id and not the empty-slot marker: never found in a table.
+// Not a key and not the empty-slot marker: never found in a table.
 constexpr int kNoEntry = -2;
-
-IG_DEV bool search(const int* list, int n, int key)
-{
-  int lo = 0, hi = n;
-  while (lo < hi) {
-    const int mid = (lo + hi) >> 1;
-    if (list[mid] < key) lo = mid + 1;
-    else hi = mid;
-  }
-  return lo < n && list[lo] == key;
-}
 
 // Hash tables in LDS: buckets of 4 slots (16 bytes, read with one
 // ds_read_b128), a power of two >= 8d slots (load <= 1/8) when that fits in
 // `cap` slots, else >= 4d (callers keep 4d within cap), and at least 64.
-// The home bucket of w is bits [24 - log2(buckets), 24) of (w mod 2^24) *
-// 0x9E3779: Fibonacci hashing in a 24-bit word (0x9E3779 / 2^24 ~ 0.618, so
-// consecutive ids land ~0.618 of the table apart) with a full-rate 24-bit
-// multiply.  Ids above 2^24 share home buckets, which costs probes, never
-// results.  Empty slots hold -1.  An insertion takes the first empty slot of
-// its bucket, in slot order, and moves on to the next bucket (cyclically)
-// only when all four are taken, so the taken slots of a bucket are always a
-// prefix: a bucket whose last slot is empty ends every probe sequence through
-// it.  At load 1/8 a bucket is full with probability ~0.2%.
+// The home bucket of key w is bits [24 - log2(buckets), 24) of
+// (w mod 2^24) * 0x9E3779: Fibonacci hashing in a 24-bit word
+// (0x9E3779 / 2^24 ~ 0.618, so consecutive keys land ~0.618 of the table
+// apart) with a full-rate 24-bit multiply.  Keys above 2^24 share home
+// buckets, which costs probes, never results.  Empty slots hold -1.  An
+// insertion takes the first empty slot of its bucket, in slot order, and
+// moves on to the next bucket (cyclically) only when all four are taken, so
+// the taken slots of a bucket are always a prefix: a bucket whose last slot
+// is empty ends every probe sequence through it.  At load 1/8 a bucket is
+// full with probability ~0.2%.
 struct TcTable {
   int size;       // slots
   unsigned mask;  // buckets - 1
@@ -415,22 +544,33 @@ IG_DEV void wave_lds_sync()
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// The task of vertex x: N+(x) at olist[ob, ob + d), its in-list at
-// ilist[ib, ib + di).
+// The task of vertex x: N+(x) at row ob of olist (16-bit keys when narrow),
+// its in-list at ilist[ib, ib + di).
 struct TcTask {
   int ob, d, ib, di;
+  bool narrow;
 };
 
-// Wave-collective: counts the entries w[0..kN) (kNoEntry: none) found in
-// N+(x).  Hashed: the home buckets of every window at once, and the few lanes
-// whose bucket is full and holds no match go on through the next ones; hits
-// are added to the wave's `wcnt`.  Otherwise binary search of nx[0, d), hits
-// added to the lane's `cnt`.
-template <bool kHashed, int kN>
-IG_DEV void probe_windows(const int* w, int lane, const int* tab, const TcTable& t, const int* nx, int d,
-                          unsigned& cnt, unsigned long long& wcnt)
+// Strided over the calling lanes (i0, step): inserts N+(x)[lo, hi) into tab.
+IG_DEV void table_fill(int* tab, const TcTable& t, const int* olist, const TcTask& x, int lo, int hi, int i0,
+                       int step)
 {
-  if constexpr (kHashed) {
+  for (int i = lo + i0; i < hi; i += step) table_insert(tab, t, row_get(olist, x.ob, i, x.narrow));
+}
+
+// Membership tests of a task's N+(x) in LDS, as policies of the walks below.
+// Wave-collective: counts the keys w[0..kN) (kNoEntry: none) found, adding
+// the hits to the wave's `wcnt`.
+//
+// HashProbe: the hash table (any keys): the home buckets of every window at
+// once, and the few lanes whose bucket is full and holds no match go on
+// through the next ones.
+struct HashProbe {
+  const int* tab;
+  TcTable t;
+  template <int kN>
+  IG_DEV void windows(const int* w, int lane, unsigned long long& wcnt) const
+  {
     unsigned h[kN];
     int4 sl[kN];
 #pragma unroll
@@ -441,8 +581,8 @@ IG_DEV void probe_windows(const int* w, int lane, const int* tab, const TcTable&
     // hit masks straight from one compare per slot, on the scalar unit
 #pragma unroll
     for (int q = 0; q < kN; q++) {
-      unsigned long long hm = __ballot(sl[q].x == w[q]) | __ballot(sl[q].y == w[q]) |
-                              __ballot(sl[q].z == w[q]) | __ballot(sl[q].w == w[q]);
+      unsigned long long hm = __ballot(sl[q].x == w[q]) | __ballot(sl[q].y == w[q]) | __ballot(sl[q].z == w[q]) |
+                              __ballot(sl[q].w == w[q]);
       const unsigned long long go = __ballot(sl[q].w != -1) & ~hm;  // full bucket, no match
       if (go != 0) {
         const bool mine = ((go >> lane) & 1ull) != 0;
@@ -450,20 +590,35 @@ IG_DEV void probe_windows(const int* w, int lane, const int* tab, const TcTable&
       }
       wcnt += static_cast<unsigned long long>(__popcll(hm));
     }
-  } else {
-#pragma unroll
-    for (int q = 0; q < kN; q++)
-      if (w[q] >= 0) cnt += search(nx, d, w[q]) ? 1u : 0u;
   }
-}
+};
+
+// BitProbe: a bitmap of hub keys (kHubKeys bits), for a hub's task: its
+// N+(x) holds hubs only, so any other key misses at once.  One 4-byte LDS
+// read and a bit test per key, against a 16-byte bucket read, a hash and
+// four compares.
+struct BitProbe {
+  const unsigned* bits;
+  template <int kN>
+  IG_DEV void windows(const int* w, int, unsigned long long& wcnt) const
+  {
+    unsigned v[kN];
+#pragma unroll
+    for (int q = 0; q < kN; q++) v[q] = bits[min(static_cast<unsigned>(w[q]), kHubKeys - 1u) >> 5];
+#pragma unroll
+    for (int q = 0; q < kN; q++) {
+      const unsigned k = static_cast<unsigned>(w[q]);
+      wcnt += static_cast<unsigned long long>(__popcll(__ballot(k < kHubKeys && ((v[q] >> (k & 31u)) & 1u) != 0u)));
+    }
+  }
+};
 
 // Wave-collective: kN windows of 64 entries of row[0, ol) from `base`
 // (windows past base + 64*kN are not touched; within them, every lane loads -
 // past the end: the last entry again, a cache hit - so that the loads issue
 // back to back, and kNoEntry marks the lanes past the end), then counted.
-template <bool kHashed, int kN>
-IG_DEV void walk_step(const int* row, int base, int ol, int lane, const int* tab, const TcTable& t, const int* nx,
-                      int d, unsigned& cnt, unsigned long long& wcnt)
+template <typename T, int kN, typename P>
+IG_DEV void walk_step(const T* row, int base, int ol, int lane, const P& pr, unsigned long long& wcnt)
 {
   static_assert(kN >= 1 && kN <= kWalk, "window count");
   int w[kN];
@@ -473,111 +628,133 @@ IG_DEV void walk_step(const int* row, int base, int ol, int lane, const int* tab
     const int v = load_at(row, min(k, ol - 1));
     w[q] = k < ol ? v : kNoEntry;
   }
-  probe_windows<kHashed, kN>(w, lane, tab, t, nx, d, cnt, wcnt);
+  pr.template windows<kN>(w, lane, wcnt);
+}
+
+// Wave-collective: the whole of row[0, ol): steps of kWalk windows while more
+// than kWalk - 1 windows are left (the last of them may be partial), then one
+// step of just the 0 to kWalk - 1 windows left.
+template <typename T, typename P>
+IG_DEV void walk_row(const T* row, int ol, int lane, const P& pr, unsigned long long& wcnt)
+{
+  static_assert(kWalk == 4, "the switch below covers 1 to kWalk - 1 windows");
+  int base = 0;
+  for (; ol - base > (kWalk - 1) * kWave; base += kWalk * kWave)
+    walk_step<T, kWalk>(row, base, ol, lane, pr, wcnt);
+  switch ((ol - base + kWave - 1) / kWave) {
+    case 1: walk_step<T, 1>(row, base, ol, lane, pr, wcnt); break;
+    case 2: walk_step<T, 2>(row, base, ol, lane, pr, wcnt); break;
+    case 3: walk_step<T, 3>(row, base, ol, lane, pr, wcnt); break;
+    default: break;
+  }
 }
 
 // Wave-collective: kN windows of 64 pairs of the in-wave balancer from
-// `base` (< total): pair k of the lanes' concatenated short lists is
-// olist[shift[o] + k] for its owner lane o (incl: the inclusive prefix of the
-// list lengths).  Owners first, then every lane loads (past the end: window
+// `base` (< total): pair k of the lanes' concatenated short rows is
+// list[shift[o] + k] for its owner lane o (incl: the inclusive prefix of the
+// row lengths).  Owners first, then every lane loads (past the end: window
 // 0's first pair again) so that the loads issue back to back, then counted.
-template <bool kHashed, int kN>
-IG_DEV void balanced_step(const int* __restrict__ olist, int incl, int shift, int total, int base, int lane,
-                          const int* tab, const TcTable& t, const int* nx, int d, unsigned& cnt,
-                          unsigned long long& wcnt)
+template <typename T, int kN, typename P>
+IG_DEV void balanced_step(const T* __restrict__ list, int incl, unsigned shift, int total, int base, int lane,
+                          const P& pr, unsigned long long& wcnt)
 {
   static_assert(kN >= 1 && kN <= kWalk, "window count");
-  int at[kN];
+  unsigned at[kN];
 #pragma unroll
   for (int q = 0; q < kN; q++) {
     const int k = base + q * kWave + lane;
     const int o = min(wave_owner(incl, k), kWave - 1);
-    at[q] = wave_shfl(shift, o) + k;
+    at[q] = wave_shfl(shift, o) + static_cast<unsigned>(k);
   }
-  const int fb = wave_bcast(at[0], 0);  // pair `base`, which exists
+  const unsigned fb = static_cast<unsigned>(wave_bcast(static_cast<int>(at[0]), 0));  // pair `base`, which exists
   int w[kN];
 #pragma unroll
   for (int q = 0; q < kN; q++) {
     const bool ok = base + q * kWave + lane < total;
-    const int v = olist[ok ? at[q] : fb];
+    const int v = list[ok ? at[q] : fb];
     w[q] = ok ? v : kNoEntry;
   }
-  probe_windows<kHashed, kN>(w, lane, tab, t, nx, d, cnt, wcnt);
+  pr.template windows<kN>(w, lane, wcnt);
 }
 
-// Wave-collective: over items[ub0 + 64k + lane] (k = 0, 1, ... stepping by
-// ustep), counts the w in N+(item) found in N+(x) - through `tab`, or by
-// binary search of the sorted nx[0, d) when tab is null - for the items
-// whose list has at most `limit` entries.  Lists of kLongList or more
-// entries are walked by the whole wave, one at a time, kWalk windows of 64 per
-// step so that their loads overlap; shorter ones share windows of 64 pairs
-// through the in-wave balancer, also kWalk windows per step.
-template <bool kHashed>
-IG_DEV unsigned long long count_pairs(const int2* __restrict__ orng, const int* __restrict__ olist,
-                                      const int* __restrict__ items, int n, int limit, int ub0, int ustep,
-                                      const int* tab, const TcTable& t, const int* nx, int d)
+// Wave-collective: each lane's short row of slen entries from list[start]
+// (in entries of T), all through the in-wave balancer: kWalk windows of
+// pairs per step, then one step of just the windows left.
+template <typename T, typename P>
+IG_DEV void balanced_rows(const T* __restrict__ list, unsigned start, int slen, int lane, const P& pr,
+                          unsigned long long& wcnt)
+{
+  const int incl = wave_inclusive_scan(slen);
+  const int total = wave_bcast(incl, kWave - 1);
+  if (total == 0) return;
+  const unsigned shift = start - static_cast<unsigned>(incl - slen);  // pair k is list[shift[o] + k]
+  int base = 0;
+  for (; total - base > (kWalk - 1) * kWave; base += kWalk * kWave)
+    balanced_step<T, kWalk>(list, incl, shift, total, base, lane, pr, wcnt);
+  switch ((total - base + kWave - 1) / kWave) {
+    case 1: balanced_step<T, 1>(list, incl, shift, total, base, lane, pr, wcnt); break;
+    case 2: balanced_step<T, 2>(list, incl, shift, total, base, lane, pr, wcnt); break;
+    case 3: balanced_step<T, 3>(list, incl, shift, total, base, lane, pr, wcnt); break;
+    default: break;
+  }
+}
+
+// Wave-collective: over the item keys items[ub0 + 64k + lane] (k = 0, 1, ...
+// stepping by ustep; 16-bit keys when items_narrow), counts the keys in
+// N+(item) found in the table, for the items whose list has at most `limit`
+// entries.  Lists of kLongList or more entries are walked by the whole wave,
+// one at a time, kWalk windows of 64 per step so that their loads overlap;
+// shorter ones share windows of 64 pairs through the in-wave balancer, also
+// kWalk windows per step, the 16-bit rows and the 32-bit rows in turn.
+// Returns the count in lane 0.
+template <typename P>
+IG_DEV unsigned long long count_pairs(const int2* __restrict__ orngk, const int* __restrict__ olist,
+                                      const void* items, bool items_narrow, int n, int limit, int ub0, int ustep,
+                                      const P& pr)
 {
   const int lane = lane_id();
-  // a lane probes at most n * (limit / 64 + 1) entries here: 32 bits suffice
-  unsigned cnt = 0;
-  unsigned long long wcnt = 0;  // the long-list hits of the whole wave
+  const unsigned short* olist16 = reinterpret_cast<const unsigned short*>(olist);
+  unsigned long long wcnt = 0;  // hits of the whole wave
   for (int ub = ub0; ub < n; ub += ustep) {
     const int i = ub + lane;
-    const int2 r = i < n ? orng[items[i]] : make_int2(0, 0);
-    const int b = r.x;
-    const int l = r.y;
-    const int len = l <= limit ? l : 0;
+    int key = -1;
+    if (i < n)
+      key = items_narrow ? static_cast<int>(static_cast<const unsigned short*>(items)[i])
+                         : static_cast<const int*>(items)[i];
+    const int2 r = key >= 0 ? orngk[key] : make_int2(0, 0);
+    const bool nar = narrow_key(key);
+    const int len = r.y <= limit ? r.y : 0;
     unsigned long long todo = __ballot(len >= kLongList);
     while (todo != 0) {
       const int o = __ffsll(static_cast<long long>(todo)) - 1;
       todo &= todo - 1;
       const int ol = wave_bcast(len, o);
-      const int* row = olist + wave_bcast(b, o);
-      // steps of kWalk windows while more than kWalk - 1 windows are left
-      // (the last of them may be partial), then one step of just the 0 to
-      // kWalk - 1 windows left
-      static_assert(kWalk == 4, "the switch below covers 1 to kWalk - 1 windows");
-      int base = 0;
-      for (; ol - base > (kWalk - 1) * kWave; base += kWalk * kWave)
-        walk_step<kHashed, kWalk>(row, base, ol, lane, tab, t, nx, d, cnt, wcnt);
-      switch ((ol - base + kWave - 1) / kWave) {
-        case 1: walk_step<kHashed, 1>(row, base, ol, lane, tab, t, nx, d, cnt, wcnt); break;
-        case 2: walk_step<kHashed, 2>(row, base, ol, lane, tab, t, nx, d, cnt, wcnt); break;
-        case 3: walk_step<kHashed, 3>(row, base, ol, lane, tab, t, nx, d, cnt, wcnt); break;
-        default: break;
-      }
+      const int b = wave_bcast(r.x, o);
+      if (wave_bcast(nar ? 1 : 0, o) != 0) walk_row(reinterpret_cast<const unsigned short*>(olist + b), ol, lane, pr, wcnt);
+      else walk_row(olist + b, ol, lane, pr, wcnt);
     }
-    const int slen = len < kLongList ? len : 0;
-    const int incl = wave_inclusive_scan(slen);
-    const int total = wave_bcast(incl, kWave - 1);
-    const int shift = b - (incl - slen);  // pair k is olist[shift[o] + k] for its owner lane o
-    // kWalk windows of pairs per step, then one step of just the windows left
-    int base = 0;
-    for (; total - base > (kWalk - 1) * kWave; base += kWalk * kWave)
-      balanced_step<kHashed, kWalk>(olist, incl, shift, total, base, lane, tab, t, nx, d, cnt, wcnt);
-    switch ((total - base + kWave - 1) / kWave) {
-      case 1: balanced_step<kHashed, 1>(olist, incl, shift, total, base, lane, tab, t, nx, d, cnt, wcnt); break;
-      case 2: balanced_step<kHashed, 2>(olist, incl, shift, total, base, lane, tab, t, nx, d, cnt, wcnt); break;
-      case 3: balanced_step<kHashed, 3>(olist, incl, shift, total, base, lane, tab, t, nx, d, cnt, wcnt); break;
-      default: break;
-    }
+    const bool sh = len < kLongList;
+    balanced_rows(olist16, 2u * static_cast<unsigned>(r.x), sh && nar ? len : 0, lane, pr, wcnt);
+    balanced_rows(olist, static_cast<unsigned>(r.x), sh && !nar ? len : 0, lane, pr, wcnt);
   }
-  return cnt + (lane == 0 ? wcnt : 0ull);
+  return lane == 0 ? wcnt : 0ull;
 }
 
 // Items [lo, hi) of the task's item sequence (N+(x), then N-(x)).
-template <bool kHashed>
-IG_DEV unsigned long long task_pairs(const int2* __restrict__ orng, const int* __restrict__ olist,
+template <typename P>
+IG_DEV unsigned long long task_pairs(const int2* __restrict__ orngk, const int* __restrict__ olist,
                                      const int* __restrict__ ilist, const TcTask& x, int lo, int hi, int ub0,
-                                     int ustep, const int* tab, const TcTable& t)
+                                     int ustep, const P& pr)
 {
-  const int* nx = olist + x.ob;
   unsigned long long cnt = 0;
   const int a1 = min(hi, x.d);
-  if (lo < a1) cnt += count_pairs<kHashed>(orng, olist, nx + lo, a1 - lo, x.d, ub0, ustep, tab, t, nx, x.d);
+  if (lo < a1) {
+    const void* items = x.narrow ? static_cast<const void*>(reinterpret_cast<const unsigned short*>(olist + x.ob) + lo)
+                                 : static_cast<const void*>(olist + x.ob + lo);
+    cnt += count_pairs(orngk, olist, items, x.narrow, a1 - lo, x.d, ub0, ustep, pr);
+  }
   const int b0 = max(lo, x.d) - x.d, b1 = hi - x.d;
-  if (b0 < b1)
-    cnt += count_pairs<kHashed>(orng, olist, ilist + x.ib + b0, b1 - b0, x.d, ub0, ustep, tab, t, nx, x.d);
+  if (b0 < b1) cnt += count_pairs(orngk, olist, ilist + x.ib + b0, false, b1 - b0, x.d, ub0, ustep, pr);
   return cnt;
 }
 
@@ -588,9 +765,10 @@ IG_DEV unsigned long long task_pairs(const int2* __restrict__ orng, const int* _
 // balancer.  A counted item's list is no longer than its task's N+(x) (the
 // shorter-list rule), so at most kTinyList entries: no long-list walk.  One
 // task at a time would pay its own chain of round trips for a handful of
-// probes.
-IG_DEV unsigned long long tiny_tasks(unsigned long long mask, const int2 rl, int dl, int il,
-                                     const int2* __restrict__ orng, const int* __restrict__ olist,
+// probes.  Rows of either width mix here, so every entry is read with its
+// own row's width.
+IG_DEV unsigned long long tiny_tasks(unsigned long long mask, const int2 rl, int dl, int il, int nl,
+                                     const int2* __restrict__ orngk, const int* __restrict__ olist,
                                      const int* __restrict__ ilist, int* tab)
 {
   const int lane = lane_id();
@@ -610,6 +788,7 @@ IG_DEV unsigned long long tiny_tasks(unsigned long long mask, const int2 rl, int
   const int tob = wave_shfl(rl.x, has ? src : 0);  // in-lists share the out-lists' positions
   const int td0 = wave_shfl(dl, has ? src : 0);
   const int ti0 = wave_shfl(il, has ? src : 0);
+  const int tnar = wave_shfl(nl, has ? src : 0);
   const int td = has ? td0 : 0;
   const int tn = has ? td0 + ti0 : 0;  // items: N+(x), then N-(x)
   const TcTable t = table_for(kTinyList, kTinySlots);  // 64 slots for every task
@@ -619,12 +798,13 @@ IG_DEV unsigned long long tiny_tasks(unsigned long long mask, const int2 rl, int
   {
     const int incl = wave_inclusive_scan(td);
     const int total = wave_bcast(incl, kWave - 1);
-    const int shift = tob - (incl - td);  // entry k of the concatenation is olist[shift[j] + k]
     for (int base = 0; base < total; base += kWave) {
       const int k = base + lane;
       const int j = min(wave_owner(incl, k), kWave - 1);
-      const int at = wave_shfl(shift, j) + k;
-      if (k < total) table_insert(tab + j * stride, t, olist[at]);
+      const int i = k - wave_shfl(incl - td, j);
+      const int jb = wave_shfl(tob, j);
+      const bool jn = wave_shfl(tnar, j) != 0;
+      if (k < total) table_insert(tab + j * stride, t, row_get(olist, jb, i, jn));
     }
   }
   wave_lds_sync();
@@ -637,22 +817,26 @@ IG_DEV unsigned long long tiny_tasks(unsigned long long mask, const int2 rl, int
     const int i = k - wave_shfl(incl_n - tn, j);
     const int jd = wave_shfl(td, j);
     const int job = wave_shfl(tob, j);
-    int b = 0, len = 0;
+    const bool jn = wave_shfl(tnar, j) != 0;
+    int b = 0, len = 0, nk = 0;
     if (k < total_n) {
-      const int u = i < jd ? olist[job + i] : ilist[job + (i - jd)];
-      const int2 r = orng[u];
+      const int u = i < jd ? row_get(olist, job, i, jn) : ilist[job + (i - jd)];
+      const int2 r = orngk[u];
       b = r.x;
       len = r.y <= jd ? r.y : 0;  // the shorter-list rule of count_pairs
+      nk = narrow_key(u) ? 1 : 0;
     }
     const int incl = wave_inclusive_scan(len);
     const int total = wave_bcast(incl, kWave - 1);
-    const int shift = b - (incl - len);
+    const int excl = incl - len;
     for (int pbase = 0; pbase < total; pbase += kWave) {
       const int kk = pbase + lane;
       const int o = min(wave_owner(incl, kk), kWave - 1);
-      const int at = wave_shfl(shift, o) + kk;
+      const int ob = wave_shfl(b, o);
+      const int oi = kk - wave_shfl(excl, o);
+      const bool on = wave_shfl(nk, o) != 0;
       const int tj = wave_shfl(j, o);
-      if (kk < total) cnt += table_find(tab + tj * stride, t, olist[at]) ? 1ull : 0ull;
+      if (kk < total) cnt += table_find(tab + tj * stride, t, row_get(olist, ob, oi, on)) ? 1ull : 0ull;
     }
   }
   wave_lds_sync();
@@ -673,7 +857,8 @@ IG_DEV void append_parts(int parts, int x, unsigned long long* queue, unsigned* 
         (static_cast<unsigned long long>(x) << 32) | static_cast<unsigned>(q);
 }
 
-__global__ __launch_bounds__(kTcBlock) void tc_count(int nodes, int grab, const int2* __restrict__ orng,
+__global__ __launch_bounds__(kTcBlock) IG_SGPR_CAP void tc_count(int nodes, int grab, const int2* __restrict__ dk,
+                                                     const int2* __restrict__ orng, const int2* __restrict__ orngk,
                                                      const int* __restrict__ olist, const int* __restrict__ ilen,
                                                      const int* __restrict__ ilist, unsigned long long* mid,
                                                      unsigned long long* big, TcControl* ctl)
@@ -696,6 +881,7 @@ __global__ __launch_bounds__(kTcBlock) void tc_count(int nodes, int grab, const 
     const int vl = static_cast<int>(v0) + lane;
     const bool mine = lane < grab && vl < nodes;
     const int2 rl = mine ? orng[vl] : make_int2(0, 0);
+    const int nl = mine && narrow_key(dk[vl].y) ? 1 : 0;
     const int dl = rl.y;
     const int il = mine ? ilen[vl] : 0;
     const bool here = dl >= 2 && dl <= kWaveList && (dl + il) * dl <= kWaveWork;
@@ -713,19 +899,19 @@ __global__ __launch_bounds__(kTcBlock) void tc_count(int nodes, int grab, const 
         grp |= low;
         tiny &= tiny - 1;
       }
-      cnt += tiny_tasks(grp, rl, dl, il, orng, olist, ilist, tab);
+      cnt += tiny_tasks(grp, rl, dl, il, nl, orngk, olist, ilist, tab);
     }
     while (todo != 0) {
       const int j = __ffsll(static_cast<long long>(todo)) - 1;
       todo &= todo - 1;
       const int ob = wave_shfl(rl.x, j);  // in-lists share the out-lists' positions
-      const TcTask x{ob, wave_shfl(dl, j), ob, wave_shfl(il, j)};
+      const TcTask x{ob, wave_shfl(dl, j), ob, wave_shfl(il, j), wave_shfl(nl, j) != 0};
       const TcTable t = table_for(x.d, kWaveSlots);
       for (int i = lane; i < t.size; i += kWave) tab[i] = -1;
       wave_lds_sync();
-      for (int i = lane; i < x.d; i += kWave) table_insert(tab, t, olist[x.ob + i]);
+      table_fill(tab, t, olist, x, 0, x.d, lane, kWave);
       wave_lds_sync();
-      cnt += task_pairs<true>(orng, olist, ilist, x, 0, x.d + x.di, 0, kWave, tab, t);
+      cnt += task_pairs(orngk, olist, ilist, x, 0, x.d + x.di, 0, kWave, HashProbe{tab, t});
       wave_lds_sync();
     }
   }
@@ -733,10 +919,13 @@ __global__ __launch_bounds__(kTcBlock) void tc_count(int nodes, int grab, const 
   if (lane == 0 && cnt != 0) atomic_add_relaxed(&ctl->count, cnt);
 }
 
-// One workgroup per part from `parts` (grabbed through *next); lists up to
-// lds_list entries (<= kSlots / 4) are hashed into LDS.
+// One workgroup per part from `parts` (grabbed through *next).  N+(x) is
+// hashed into LDS lds_list (<= kSlots / 4) entries at a time - one piece
+// unless the list is longer - and the part's items are probed against every
+// piece.
 template <int kBlock, int kSlots>
-__global__ __launch_bounds__(kBlock) void tc_count_group(const int2* __restrict__ orng,
+__global__ __launch_bounds__(kBlock) IG_SGPR_CAP void tc_count_group(const int2* __restrict__ dk, const int2* __restrict__ orng,
+                                                         const int2* __restrict__ orngk,
                                                          const int* __restrict__ olist,
                                                          const int* __restrict__ ilen,
                                                          const int* __restrict__ ilist,
@@ -759,19 +948,35 @@ __global__ __launch_bounds__(kBlock) void tc_count_group(const int2* __restrict_
     const int v = static_cast<int>(p >> 32);
     const int lo = static_cast<int>(p & 0xffffffffull) * kPartItems;
     const int2 r = orng[v];
-    const TcTask x{r.x, r.y, r.x, ilen[v]};
-    const bool hashed = x.d <= lds_list;
-    const TcTable t = table_for(hashed ? x.d : 1, kSlots);
-    if (hashed) {
-      for (int i = threadIdx.x; i < t.size; i += kBlock) s_tab[i] = -1;
-      __syncthreads();
-      for (int i = threadIdx.x; i < x.d; i += kBlock) table_insert(s_tab, t, olist[x.ob + i]);
-      __syncthreads();
-    }
+    const TcTask x{r.x, r.y, r.x, ilen[v], narrow_key(dk[v].y)};
     const int hi = min(lo + kPartItems, x.d + x.di);
-    if (hashed) cnt += task_pairs<true>(orng, olist, ilist, x, lo, hi, wid * kWave, kWaves * kWave, s_tab, t);
-    else cnt += task_pairs<false>(orng, olist, ilist, x, lo, hi, wid * kWave, kWaves * kWave, nullptr, t);
-    __syncthreads();
+    if (x.narrow) {
+      // a hub's N+(x) as a bitmap of hub keys (kHubKeys bits)
+      static_assert(kSlots * 32 >= kHubKeys, "the bitmap fits the table's LDS");
+      unsigned* bits = reinterpret_cast<unsigned*>(s_tab);
+      for (int i = threadIdx.x; i < kHubKeys / 32; i += kBlock) bits[i] = 0u;
+      __syncthreads();
+      for (int i = threadIdx.x; i < x.d; i += kBlock) {
+        const unsigned k = static_cast<unsigned>(row_get(olist, x.ob, i, true));
+        atomicOr(&bits[k >> 5], 1u << (k & 31u));
+      }
+      __syncthreads();
+      cnt += task_pairs(orngk, olist, ilist, x, lo, hi, wid * kWave, kWaves * kWave, BitProbe{bits});
+      __syncthreads();
+    } else {
+      // N+(x) hashed lds_list entries at a time: one piece unless the list
+      // is longer; every piece is probed with all of the part's items
+      for (int c0 = 0; c0 < x.d; c0 += lds_list) {
+        const int c1 = min(x.d, c0 + lds_list);
+        const TcTable t = table_for(c1 - c0, kSlots);
+        for (int i = threadIdx.x; i < t.size; i += kBlock) s_tab[i] = -1;
+        __syncthreads();
+        table_fill(s_tab, t, olist, x, c0, c1, threadIdx.x, kBlock);
+        __syncthreads();
+        cnt += task_pairs(orngk, olist, ilist, x, lo, hi, wid * kWave, kWaves * kWave, HashProbe{s_tab, t});
+        __syncthreads();
+      }
+    }
   }
   cnt = wave_sum(cnt);
   if (lane_id() == 0 && cnt != 0) atomic_add_relaxed(&ctl->count, cnt);
@@ -801,19 +1006,26 @@ size_t hcnt_words(int edges)
 
 size_t tc_workspace_bytes(int nodes, int edges)
 {
-  const size_t v = static_cast<size_t>(nodes);
-  return Carver::round(sizeof(TcControl)) + 2 * Carver::round(v * sizeof(int2)) + Carver::round(v * 4) +
-         3 * Carver::round((v + 1) * 4) + 2 * Carver::round(list_words(edges) * 4) +
-         2 * Carver::round(part_words(nodes, edges) * 8) + Carver::round(hcnt_words(edges) * 4) + 4096;
+  // the takes of tc_run, in its order
+  const size_t v = static_cast<size_t>(nodes), vp = v + 1;
+  return Carver::round(sizeof(TcControl)) + Carver::round(v * sizeof(TcVert)) + Carver::round(v * sizeof(int2)) +
+         Carver::round((kHubKeys + v) * sizeof(int2)) + Carver::round(v * sizeof(int2)) + Carver::round(v * 4) +
+         3 * Carver::round(vp * 4) + 2 * Carver::round(part_words(nodes, edges) * 8) +
+         2 * Carver::round(list_words(edges) * 4) + Carver::round(hcnt_words(edges) * 4) +
+         Carver::round((kHubKeys + 1) * 4) + Carver::round(v * 4) + Carver::round(vp * 4) +
+         Carver::round(scan_temp_bytes(nodes)) + 4096;
 }
 
 int tc_run(indigo_ctx* ctx, int nodes, int edges, const int* nindex, const int* nlist, int* launches)
 {
+  if (nodes > INT_MAX - kHubKeys) return set_error(INDIGO_ERR_INVALID_ARG, "tc: too many vertices for 32-bit keys");
   Carver cv(ctx->ws);
   const size_t vn = static_cast<size_t>(nodes), vp = vn + 1;
   TcControl* ctl = cv.take<TcControl>(1);
   TcVert* vert = cv.take<TcVert>(vn);
   int2* orng = cv.take<int2>(vn);
+  int2* orngk = cv.take<int2>(kHubKeys + vn);
+  int2* dk = cv.take<int2>(vn);
   int* ilen = cv.take<int>(vn);
   FrontierQueues heavy{nullptr, cv.take<int>(vp), cv.take<int>(vp), cv.take<unsigned>(vp)};
   unsigned long long* mid = cv.take<unsigned long long>(part_words(nodes, edges));
@@ -821,39 +1033,60 @@ int tc_run(indigo_ctx* ctx, int nodes, int edges, const int* nindex, const int* 
   int* olist = cv.take<int>(list_words(edges));
   int* ilist = cv.take<int>(list_words(edges));
   int* hcnt = cv.take<int>(hcnt_words(edges));
+  unsigned* hist = cv.take<unsigned>(kHubKeys + 1);
+  int* flags = cv.take<int>(vn);
+  int* hidx = cv.take<int>(vp);
+  int* stemp = cv.take<int>(scan_temp_bytes(nodes) / 4 + 1);
   IG_TRY(zero_async(ctx, ctl, sizeof(TcControl)));
   hipStream_t s = ctx->stream;
   const int gw = share_grid(ctx, grid_for((nodes + kWave - 1) / kWave, kTcBlock / kWave, 1 << 16), kTcBlock);
   const int gh = share_grid(ctx, ctx->compute_units * 8, kTcBlock);
-  const OutSel out{nindex};
-  const OutRec orec{vert, orng};
+  const int gv = share_grid(ctx, grid_for(nodes, kTcBlock, ctx->compute_units * 8), kTcBlock);
+  int n_launch = 11;
+  // keys: every vertex is a hub (key = id) when they all fit, else the
+  // highest-degree ones up to the cap, numbered in id order
+  const int hub_hint = ctx->hints[INDIGO_HINT_TC_HUBS];
+  const int hub_cap = hub_hint > 0 && hub_hint < kHubKeys ? hub_hint : kHubKeys;
+  if (nodes <= hub_cap) {
+    tc_hub_keys<<<gv, kTcBlock, 0, s>>>(nodes, nindex, ctl, nullptr, dk);
+  } else {
+    IG_TRY(zero_async(ctx, hist, (kHubKeys + 1) * sizeof(unsigned)));
+    tc_degree_hist<<<gv, kTcBlock, 0, s>>>(nodes, nindex, hist);
+    tc_hub_threshold<<<1, kThreshBlock, 0, s>>>(hist, hub_cap, ctl);
+    tc_hub_flags<<<gv, kTcBlock, 0, s>>>(nodes, nindex, ctl, flags);
+    IG_TRY(exclusive_scan(s, flags, hidx, nodes, stemp));
+    tc_hub_keys<<<gv, kTcBlock, 0, s>>>(nodes, nindex, ctl, hidx, dk);
+    n_launch += 7;
+  }
+  const OutSel out{dk};
+  const OutRec orec{dk, vert, orng, orngk};
   tc_select<true><<<gw, kTcBlock, 0, s>>>(nodes, nindex, nlist, out, orec, olist, heavy, ctl);
   tc_select_heavy<true><<<gh, kTcBlock, 0, s>>>(nindex, nlist, out, orec, olist, heavy, hcnt, ctl);
-  tc_pack_huge<<<gh, kTcBlock, 0, s>>>(nindex, olist, heavy, hcnt, orec, ctl);
+  tc_pack_huge<<<gh, kTcBlock, 0, s>>>(nindex, olist, heavy, hcnt, out, orec, ctl);
   const InSel in{vert};
   const InRec irec{ilen};
   tc_select<false><<<gw, kTcBlock, 0, s>>>(nodes, nindex, nlist, in, irec, ilist, heavy, ctl);
   tc_select_heavy<false><<<gh, kTcBlock, 0, s>>>(nindex, nlist, in, irec, ilist, heavy, hcnt, ctl);
-  tc_pack_huge<<<gh, kTcBlock, 0, s>>>(nindex, ilist, heavy, hcnt, irec, ctl);
+  tc_pack_huge<<<gh, kTcBlock, 0, s>>>(nindex, ilist, heavy, hcnt, in, irec, ctl);
   {
     // vertices per grab: 64 from 2^18 vertices on, else nodes / 4096 (at
     // least 1), so a small graph still spreads over 4096 grabs while the
     // grabs' same-address atomics stay few
     const int grab = std::max(1, std::min(kWave, nodes / 4096));
-    tc_count<<<share_grid(ctx, ctx->compute_units * 8, kTcBlock), kTcBlock, 0, s>>>(nodes, grab, orng, olist, ilen,
-                                                                                   ilist, mid, big, ctl);
+    tc_count<<<share_grid(ctx, ctx->compute_units * 8, kTcBlock), kTcBlock, 0, s>>>(nodes, grab, dk, orng, orngk, olist, ilen, ilist, mid, big,
+                                                         ctl);
   }
   const int hint = ctx->hints[INDIGO_HINT_TC_LDS_LIST];
   const int mid_list = hint > 0 && hint < kMidSlots / 4 ? hint : kMidSlots / 4;
   const int big_list = hint > 0 && hint < kBigSlots / 4 ? hint : kBigSlots / 4;
   IG_TRY(kernel_begin(ctx, "tc_count_group<512, 8192>"));
   tc_count_group<kMidBlock, kMidSlots><<<share_grid(ctx, ctx->compute_units * 4, kMidBlock), kMidBlock, 0, s>>>(
-      orng, olist, ilen, ilist, mid, &ctl->nmid, &ctl->next_mid, mid_list, ctl);
+      dk, orng, orngk, olist, ilen, ilist, mid, &ctl->nmid, &ctl->next_mid, mid_list, ctl);
   IG_TRY(kernel_end(ctx));
   tc_count_group<kBigBlock, kBigSlots><<<share_grid(ctx, ctx->compute_units, kBigBlock), kBigBlock, 0, s>>>(
-      orng, olist, ilen, ilist, big, &ctl->nbig, &ctl->next_big, big_list, ctl);
+      dk, orng, orngk, olist, ilen, ilist, big, &ctl->nbig, &ctl->next_big, big_list, ctl);
   IG_HIP_TRY(hipGetLastError());
-  if (launches) *launches = 10;
+  if (launches) *launches = n_launch;
   return INDIGO_OK;
 }
 

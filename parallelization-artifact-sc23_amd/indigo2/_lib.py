@@ -36,6 +36,7 @@ HINT_MAX_WORKGROUPS = 8
 HINT_VARIANT = 9
 HINT_BLOCK_SIZE = 10
 HINT_GRAPH = 11
+HINT_TC_HUBS = 12
 VARIANT_VERTEX = 0
 VARIANT_EDGE = 1
 

@@ -148,7 +148,8 @@ def test_tc_rmat16(ig, orc, ctx):
 def test_tc_long_oriented_lists(ig, orc, ctx, lds_list):
     # oriented lists of 257..2048 entries take the 512-thread workgroup pass,
     # longer ones the 1024-thread pass; with the hint at 600 the lists above
-    # 600 entries are searched by binary search instead of the LDS table
+    # 600 entries are hashed 600 entries at a time, every piece probed with
+    # all of a part's items (tasks of hubs use a bitmap instead)
     ctx.set_hint(ig._lib.HINT_TC_LDS_LIST, lds_list)
     try:
         assert ctx.tc(ig.DeviceGraph(clique(ig, 1100))) == 1100 * 1099 * 1098 // 6
@@ -157,6 +158,40 @@ def test_tc_long_oriented_lists(ig, orc, ctx, lds_list):
         if lds_list == 0:
             assert ctx.tc(ig.DeviceGraph(clique(ig, 2100))) == 2100 * 2099 * 2098 // 6
     finally:
+        ctx.set_hint(ig._lib.HINT_TC_LDS_LIST, 0)
+
+
+def test_tc_rmat17_hubs_and_others(ig, orc, ctx):
+    """R-MAT 17 has 131072 vertices: the 65536 of highest degree are hubs,
+    whose out-lists hold 16-bit keys and whose workgroup-pass tasks probe a
+    bitmap, and the others keep 32-bit rows and hash tables, so tasks, items
+    and probed rows of both kinds mix in every pass."""
+    g = ig.rmat(17, 16, seed=7)
+    assert ctx.tc(ig.DeviceGraph(g)) == orc.tc(g)
+
+
+@pytest.mark.parametrize("hubs", [1, 5, 300, 4000])
+def test_tc_hub_caps(ig, orc, ctx, hubs):
+    """INDIGO_HINT_TC_HUBS moves the line between hubs (16-bit rows, bitmap
+    probes) and the rest (32-bit rows, hash probes).  With a cap of 1 the
+    clique unions have no hub at all (the largest clique's vertices tie on
+    degree, and a tie is never split); caps of 5 to 4000 cut through ties and
+    degree classes of R-MAT 14 and of a dense random graph (with 600-entry
+    LDS pieces for the hashed tasks too)."""
+    sizes = [3, 4, 5, 9, 16, 17] * 4 + [33, 64, 65, 200, 300, 2049]
+    want = sum(k * (k - 1) * (k - 2) // 6 for k in sizes)
+    ctx.set_hint(ig._lib.HINT_TC_HUBS, hubs)
+    try:
+        assert ctx.tc(ig.DeviceGraph(cliques(ig, sizes, seed=5))) == want
+        g = ig.rmat(14, 16, seed=8)
+        assert ctx.tc(ig.DeviceGraph(g)) == orc.tc(g)
+        g = ig.erdos_renyi(3000, 900000, seed=12)
+        want_er = orc.tc(g)
+        assert ctx.tc(ig.DeviceGraph(g)) == want_er
+        ctx.set_hint(ig._lib.HINT_TC_LDS_LIST, 600)
+        assert ctx.tc(ig.DeviceGraph(g)) == want_er
+    finally:
+        ctx.set_hint(ig._lib.HINT_TC_HUBS, 0)
         ctx.set_hint(ig._lib.HINT_TC_LDS_LIST, 0)
 
 
