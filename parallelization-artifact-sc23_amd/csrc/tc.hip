@@ -558,25 +558,34 @@ IG_DEV void table_fill(int* tab, const TcTable& t, const int* olist, const TcTas
   for (int i = lo + i0; i < hi; i += step) table_insert(tab, t, row_get(olist, x.ob, i, x.narrow));
 }
 
+// Hits of a wave: counted on the scalar unit (ballots) by HashProbe, per
+// lane by BitProbe, and summed once per count_pairs.
+struct Hits {
+  unsigned long long wave = 0;  // wave-uniform
+  unsigned lane = 0;            // this lane's own
+};
+
 // Membership tests of a task's N+(x) in LDS, as policies of the walks below.
-// Wave-collective: counts the keys w[0..kN) (kNoEntry: none) found, adding
-// the hits to the wave's `wcnt`.
+// Wave-collective: counts the keys w[0..kN) found.  A key is kNoEntry (never
+// found) unless kSure: then every w[q] is a real entry of a 16-bit row, so a
+// hub key (< kHubKeys).
 //
 // HashProbe: the hash table (any keys): the home buckets of every window at
 // once, and the few lanes whose bucket is full and holds no match go on
-// through the next ones.
+// through the next ones.  Hits: one ballot per slot, counted per wave.
 struct HashProbe {
+  static constexpr bool kLaneHits = false;
   const int* tab;
   TcTable t;
-  template <int kN>
-  IG_DEV void windows(const int* w, int lane, unsigned long long& wcnt) const
+  template <int kN, bool kSure>
+  IG_DEV void windows(const int* w, int lane, Hits& h) const
   {
-    unsigned h[kN];
+    unsigned b[kN];
     int4 sl[kN];
 #pragma unroll
     for (int q = 0; q < kN; q++) {
-      h[q] = bucket_of(w[q], t);
-      sl[q] = read_bucket(tab, h[q]);
+      b[q] = bucket_of(w[q], t);
+      sl[q] = read_bucket(tab, b[q]);
     }
     // hit masks straight from one compare per slot, on the scalar unit
 #pragma unroll
@@ -586,65 +595,76 @@ struct HashProbe {
       const unsigned long long go = __ballot(sl[q].w != -1) & ~hm;  // full bucket, no match
       if (go != 0) {
         const bool mine = ((go >> lane) & 1ull) != 0;
-        hm |= __ballot(mine && table_find_from(tab, t, w[q], (h[q] + 1) & t.mask));
+        hm |= __ballot(mine && table_find_from(tab, t, w[q], (b[q] + 1) & t.mask));
       }
-      wcnt += static_cast<unsigned long long>(__popcll(hm));
+      h.wave += static_cast<unsigned long long>(__popcll(hm));
     }
   }
 };
 
-// BitProbe: a bitmap of hub keys (kHubKeys bits), for a hub's task: its
-// N+(x) holds hubs only, so any other key misses at once.  One 4-byte LDS
-// read and a bit test per key, against a 16-byte bucket read, a hash and
-// four compares.
+// BitProbe: a bitmap of hub keys (kHubKeys bits, then one zero word), for a
+// hub's task: its N+(x) holds hubs only.  A key k reads word
+// min(k, kHubKeys) / 32, so every other key (65536 + id, and kNoEntry as
+// unsigned) reads the zero word; the hit is bit k mod 32, added to the
+// lane's own count.  Per key: one 4-byte LDS read and four VALU
+// instructions when kSure (address shift and mask, bit extract, add), against
+// a 16-byte bucket read, a hash and four compares.
 struct BitProbe {
+  static constexpr bool kLaneHits = true;
+  static constexpr int kWords = kHubKeys / 32 + 1;
   const unsigned* bits;
-  template <int kN>
-  IG_DEV void windows(const int* w, int, unsigned long long& wcnt) const
+  template <int kN, bool kSure>
+  IG_DEV void windows(const int* w, int, Hits& h) const
   {
     unsigned v[kN];
 #pragma unroll
-    for (int q = 0; q < kN; q++) v[q] = bits[min(static_cast<unsigned>(w[q]), kHubKeys - 1u) >> 5];
-#pragma unroll
     for (int q = 0; q < kN; q++) {
       const unsigned k = static_cast<unsigned>(w[q]);
-      wcnt += static_cast<unsigned long long>(__popcll(__ballot(k < kHubKeys && ((v[q] >> (k & 31u)) & 1u) != 0u)));
+      v[q] = bits[(kSure ? k : min(k, static_cast<unsigned>(kHubKeys))) >> 5];
     }
+#pragma unroll
+    for (int q = 0; q < kN; q++)  // v_bfe_u32 takes the offset mod 32 by itself
+      h.lane += __builtin_amdgcn_ubfe(v[q], static_cast<unsigned>(w[q]), 1u);
   }
 };
 
 // Wave-collective: kN windows of 64 entries of row[0, ol) from `base`
-// (windows past base + 64*kN are not touched; within them, every lane loads -
-// past the end: the last entry again, a cache hit - so that the loads issue
-// back to back, and kNoEntry marks the lanes past the end), then counted.
-template <typename T, int kN, typename P>
-IG_DEV void walk_step(const T* row, int base, int ol, int lane, const P& pr, unsigned long long& wcnt)
+// (windows past base + 64*kN are not touched), then counted.  kFull: all kN
+// windows lie inside the row.  Otherwise every lane loads - past the end:
+// the last entry again, a cache hit - so that the loads issue back to back,
+// and kNoEntry marks the lanes past the end.
+template <typename T, int kN, bool kFull, typename P>
+IG_DEV void walk_step(const T* row, int base, int ol, int lane, const P& pr, Hits& h)
 {
   static_assert(kN >= 1 && kN <= kWalk, "window count");
   int w[kN];
+  const T* at = row + base + lane;  // kFull: one address, the windows at immediate offsets
 #pragma unroll
   for (int q = 0; q < kN; q++) {
     const int k = base + q * kWave + lane;
-    const int v = load_at(row, min(k, ol - 1));
-    w[q] = k < ol ? v : kNoEntry;
+    if (kFull) {
+      w[q] = at[q * kWave];
+    } else {
+      const int v = load_at(row, min(k, ol - 1));
+      w[q] = k < ol ? v : kNoEntry;
+    }
   }
-  pr.template windows<kN>(w, lane, wcnt);
+  pr.template windows<kN, kFull && sizeof(T) == 2>(w, lane, h);
 }
 
-// Wave-collective: the whole of row[0, ol): steps of kWalk windows while more
-// than kWalk - 1 windows are left (the last of them may be partial), then one
-// step of just the 0 to kWalk - 1 windows left.
+// Wave-collective: the whole of row[0, ol): steps of kWalk whole windows,
+// then one step of the 1 to kWalk windows left (the last of them partial).
 template <typename T, typename P>
-IG_DEV void walk_row(const T* row, int ol, int lane, const P& pr, unsigned long long& wcnt)
+IG_DEV void walk_row(const T* row, int ol, int lane, const P& pr, Hits& h)
 {
-  static_assert(kWalk == 4, "the switch below covers 1 to kWalk - 1 windows");
+  static_assert(kWalk == 4, "the switch below covers 1 to kWalk windows");
   int base = 0;
-  for (; ol - base > (kWalk - 1) * kWave; base += kWalk * kWave)
-    walk_step<T, kWalk>(row, base, ol, lane, pr, wcnt);
+  for (; ol - base >= kWalk * kWave; base += kWalk * kWave) walk_step<T, kWalk, true>(row, base, ol, lane, pr, h);
   switch ((ol - base + kWave - 1) / kWave) {
-    case 1: walk_step<T, 1>(row, base, ol, lane, pr, wcnt); break;
-    case 2: walk_step<T, 2>(row, base, ol, lane, pr, wcnt); break;
-    case 3: walk_step<T, 3>(row, base, ol, lane, pr, wcnt); break;
+    case 1: walk_step<T, 1, false>(row, base, ol, lane, pr, h); break;
+    case 2: walk_step<T, 2, false>(row, base, ol, lane, pr, h); break;
+    case 3: walk_step<T, 3, false>(row, base, ol, lane, pr, h); break;
+    case 4: walk_step<T, 4, false>(row, base, ol, lane, pr, h); break;
     default: break;
   }
 }
@@ -656,7 +676,7 @@ IG_DEV void walk_row(const T* row, int ol, int lane, const P& pr, unsigned long 
 // 0's first pair again) so that the loads issue back to back, then counted.
 template <typename T, int kN, typename P>
 IG_DEV void balanced_step(const T* __restrict__ list, int incl, unsigned shift, int total, int base, int lane,
-                          const P& pr, unsigned long long& wcnt)
+                          const P& pr, Hits& h)
 {
   static_assert(kN >= 1 && kN <= kWalk, "window count");
   unsigned at[kN];
@@ -674,15 +694,14 @@ IG_DEV void balanced_step(const T* __restrict__ list, int incl, unsigned shift, 
     const int v = list[ok ? at[q] : fb];
     w[q] = ok ? v : kNoEntry;
   }
-  pr.template windows<kN>(w, lane, wcnt);
+  pr.template windows<kN, false>(w, lane, h);
 }
 
 // Wave-collective: each lane's short row of slen entries from list[start]
 // (in entries of T), all through the in-wave balancer: kWalk windows of
 // pairs per step, then one step of just the windows left.
 template <typename T, typename P>
-IG_DEV void balanced_rows(const T* __restrict__ list, unsigned start, int slen, int lane, const P& pr,
-                          unsigned long long& wcnt)
+IG_DEV void balanced_rows(const T* __restrict__ list, unsigned start, int slen, int lane, const P& pr, Hits& h)
 {
   const int incl = wave_inclusive_scan(slen);
   const int total = wave_bcast(incl, kWave - 1);
@@ -690,11 +709,11 @@ IG_DEV void balanced_rows(const T* __restrict__ list, unsigned start, int slen, 
   const unsigned shift = start - static_cast<unsigned>(incl - slen);  // pair k is list[shift[o] + k]
   int base = 0;
   for (; total - base > (kWalk - 1) * kWave; base += kWalk * kWave)
-    balanced_step<T, kWalk>(list, incl, shift, total, base, lane, pr, wcnt);
+    balanced_step<T, kWalk>(list, incl, shift, total, base, lane, pr, h);
   switch ((total - base + kWave - 1) / kWave) {
-    case 1: balanced_step<T, 1>(list, incl, shift, total, base, lane, pr, wcnt); break;
-    case 2: balanced_step<T, 2>(list, incl, shift, total, base, lane, pr, wcnt); break;
-    case 3: balanced_step<T, 3>(list, incl, shift, total, base, lane, pr, wcnt); break;
+    case 1: balanced_step<T, 1>(list, incl, shift, total, base, lane, pr, h); break;
+    case 2: balanced_step<T, 2>(list, incl, shift, total, base, lane, pr, h); break;
+    case 3: balanced_step<T, 3>(list, incl, shift, total, base, lane, pr, h); break;
     default: break;
   }
 }
@@ -714,7 +733,7 @@ IG_DEV unsigned long long count_pairs(const int2* __restrict__ orngk, const int*
 {
   const int lane = lane_id();
   const unsigned short* olist16 = reinterpret_cast<const unsigned short*>(olist);
-  unsigned long long wcnt = 0;  // hits of the whole wave
+  Hits h;
   for (int ub = ub0; ub < n; ub += ustep) {
     const int i = ub + lane;
     int key = -1;
@@ -730,14 +749,16 @@ IG_DEV unsigned long long count_pairs(const int2* __restrict__ orngk, const int*
       todo &= todo - 1;
       const int ol = wave_bcast(len, o);
       const int b = wave_bcast(r.x, o);
-      if (wave_bcast(nar ? 1 : 0, o) != 0) walk_row(reinterpret_cast<const unsigned short*>(olist + b), ol, lane, pr, wcnt);
-      else walk_row(olist + b, ol, lane, pr, wcnt);
+      if (wave_bcast(nar ? 1 : 0, o) != 0) walk_row(reinterpret_cast<const unsigned short*>(olist + b), ol, lane, pr, h);
+      else walk_row(olist + b, ol, lane, pr, h);
     }
     const bool sh = len < kLongList;
-    balanced_rows(olist16, 2u * static_cast<unsigned>(r.x), sh && nar ? len : 0, lane, pr, wcnt);
-    balanced_rows(olist, static_cast<unsigned>(r.x), sh && !nar ? len : 0, lane, pr, wcnt);
+    balanced_rows(olist16, 2u * static_cast<unsigned>(r.x), sh && nar ? len : 0, lane, pr, h);
+    balanced_rows(olist, static_cast<unsigned>(r.x), sh && !nar ? len : 0, lane, pr, h);
   }
-  return lane == 0 ? wcnt : 0ull;
+  unsigned long long c = h.wave;
+  if (P::kLaneHits) c += wave_sum(h.lane);  // at most a part's probes (< 2^32)
+  return lane == 0 ? c : 0ull;
 }
 
 // Items [lo, hi) of the task's item sequence (N+(x), then N-(x)).
@@ -951,10 +972,11 @@ __global__ __launch_bounds__(kBlock) IG_SGPR_CAP void tc_count_group(const int2*
     const TcTask x{r.x, r.y, r.x, ilen[v], narrow_key(dk[v].y)};
     const int hi = min(lo + kPartItems, x.d + x.di);
     if (x.narrow) {
-      // a hub's N+(x) as a bitmap of hub keys (kHubKeys bits)
-      static_assert(kSlots * 32 >= kHubKeys, "the bitmap fits the table's LDS");
+      // a hub's N+(x) as a bitmap of hub keys (kHubKeys bits, then a zero
+      // word)
+      static_assert(kSlots >= BitProbe::kWords, "the bitmap fits the table's LDS");
       unsigned* bits = reinterpret_cast<unsigned*>(s_tab);
-      for (int i = threadIdx.x; i < kHubKeys / 32; i += kBlock) bits[i] = 0u;
+      for (int i = threadIdx.x; i < BitProbe::kWords; i += kBlock) bits[i] = 0u;
       __syncthreads();
       for (int i = threadIdx.x; i < x.d; i += kBlock) {
         const unsigned k = static_cast<unsigned>(row_get(olist, x.ob, i, true));
