@@ -40,11 +40,13 @@
 //                  in-wave balancer, longer ones binned into mid (one wave
 //                  each) and huge (kHugeChunk-entry items, one workgroup
 //                  each, compacted per item); also flags unsorted lists and
-//                  self loops;  records (deg, |N+|, key) per vertex;
+//                  self loops; records (|N+|, key) per vertex and, per CSR
+//                  entry, one byte: does the neighbour outrank the vertex;
 //   2. pack       : one workgroup per huge list closes the gaps between its
 //                  items;
-//   3-4.           the same for the in-lists N-(x), reading each
-//                  neighbour's (deg, |N+|, key) in one 16-byte load;
+//   3-4.           the same for the in-lists N-(x): an entry whose byte is
+//                  set is out at once, the others read the neighbour's
+//                  (|N+|, key) in one 8-byte load;
 //   5. count, wave pass: a wave grabs up to 64 consecutive vertices at a time
 //      from a global counter (a static stride would be badly unbalanced: on
 //      R-MAT graphs a vertex's degree follows the bit pattern of its id); a
@@ -126,38 +128,39 @@ IG_DEV void row_put(int* list, int b, int k, int key, bool narrow)
   else list[b + k] = key;
 }
 
-// Orientation state of a vertex, written by the out-selection: its degree,
-// |N+(v)| and key.  The in-selection reads it for every neighbour in one
-// 16-byte load.
-struct alignas(16) TcVert {
-  int deg, odeg, key, pad;
-};
-
 // Out-list of v: the keys of the neighbours that outrank v; 16-bit when v is
-// a hub.
+// a hub.  Each CSR entry e of v's row also gets its outcome in up[e], so the
+// in-selection needs no rank test.
 struct OutSel {
   const int2* __restrict__ dk;  // (degree, key) per vertex
+  unsigned char* up;            // per CSR entry: 1 when the neighbour outranks the row's vertex
   IG_DEV int own_odeg(int) const { return 0; }
   IG_DEV bool narrow(int v) const { return narrow_key(dk[v].y); }
-  IG_DEV bool operator()(int v, int dv, int, int u, int& key) const
+  IG_DEV bool operator()(int v, int dv, int, int e, int u, int& key) const
   {
     const int2 r = dk[u];
     key = r.y;
-    return outranks(r.x, u, dv, v);
+    const bool k = outranks(r.x, u, dv, v);
+    up[e] = k ? 1 : 0;
+    return k;
   }
 };
 
 // In-list of v: the keys of the neighbours u that v outranks (u -> v) whose
-// out-list is shorter than v's; always 32-bit.
+// out-list is shorter than v's; always 32-bit.  The out-selection's up[e]
+// rules out the neighbours that outrank v (about half the entries) with a
+// 1-byte coalesced load; only the others read their (|N+(u)|, key).
 struct InSel {
-  const TcVert* __restrict__ vert;
-  IG_DEV int own_odeg(int v) const { return vert[v].odeg; }
+  const unsigned char* __restrict__ up;
+  const int2* __restrict__ ok;  // (|N+(v)|, key) per vertex
+  IG_DEV int own_odeg(int v) const { return ok[v].x; }
   IG_DEV bool narrow(int) const { return false; }
-  IG_DEV bool operator()(int v, int dv, int odv, int u, int& key) const
+  IG_DEV bool operator()(int, int, int odv, int e, int u, int& key) const
   {
-    const TcVert k = vert[u];
-    key = k.key;
-    return !outranks(k.deg, u, dv, v) && k.odeg < odv;
+    if (up[e] != 0) return false;
+    const int2 r = ok[u];
+    key = r.y;
+    return r.x < odv;
   }
 };
 
@@ -165,13 +168,13 @@ struct InSel {
 // CSR position (list[nindex[v] + k]), so no count pass or scan is needed.
 struct OutRec {
   const int2* __restrict__ dk;
-  TcVert* vert;
+  int2* ok;     // (|N+(v)|, key) by vertex id
   int2* orng;   // (start, |N+(v)|) by vertex id
   int2* orngk;  // the same by key
-  IG_DEV void operator()(int v, int dv, int beg, int kept) const
+  IG_DEV void operator()(int v, int, int beg, int kept) const
   {
     const int key = dk[v].y;
-    vert[v] = TcVert{dv, kept, key, 0};
+    ok[v] = make_int2(kept, key);
     orng[v] = make_int2(beg, kept);
     orngk[key] = make_int2(beg, kept);
   }
@@ -304,7 +307,7 @@ __global__ __launch_bounds__(kTcBlock) IG_SGPR_CAP void tc_select(int nodes, con
         const int e = obeg + (k - oexcl);
         const int u = nlist[e];
         if (kOut) bad |= (u == ov) || (e > obeg && nlist[e - 1] >= u);
-        keep = sel(ov, odeg, oodv, u, key);
+        keep = sel(ov, odeg, oodv, e, u, key);
       }
       const unsigned long long m = __ballot(keep);
       // rank among the kept entries of the same owner in this step
@@ -361,7 +364,7 @@ __global__ __launch_bounds__(kTcBlock) IG_SGPR_CAP void tc_select_heavy(const in
       if (e < end) {
         const int u = nlist[e];
         if (kOut) bad |= (u == v) || (e > vb && nlist[e - 1] >= u);
-        keep = sel(v, dv, odv, u, key);
+        keep = sel(v, dv, odv, e, u, key);
       }
       const unsigned long long m = __ballot(keep);
       if (lane == 0) s_wcnt[wid] = __popcll(m);
@@ -399,7 +402,7 @@ __global__ __launch_bounds__(kTcBlock) IG_SGPR_CAP void tc_select_heavy(const in
 #pragma unroll
       for (int q = 0; q < kUnroll; q++) {
         key[q] = 0;
-        keep[q] = u[q] >= 0 && sel(v, dv, odv, u[q], key[q]);
+        keep[q] = u[q] >= 0 && sel(v, dv, odv, base + q * kWave + lane, u[q], key[q]);
       }
 #pragma unroll
       for (int q = 0; q < kUnroll; q++) {
@@ -1030,12 +1033,12 @@ size_t tc_workspace_bytes(int nodes, int edges)
 {
   // the takes of tc_run, in its order
   const size_t v = static_cast<size_t>(nodes), vp = v + 1;
-  return Carver::round(sizeof(TcControl)) + Carver::round(v * sizeof(TcVert)) + Carver::round(v * sizeof(int2)) +
+  return Carver::round(sizeof(TcControl)) + Carver::round(v * sizeof(int2)) + Carver::round(v * sizeof(int2)) +
          Carver::round((kHubKeys + v) * sizeof(int2)) + Carver::round(v * sizeof(int2)) + Carver::round(v * 4) +
          3 * Carver::round(vp * 4) + 2 * Carver::round(part_words(nodes, edges) * 8) +
          2 * Carver::round(list_words(edges) * 4) + Carver::round(hcnt_words(edges) * 4) +
          Carver::round((kHubKeys + 1) * 4) + Carver::round(v * 4) + Carver::round(vp * 4) +
-         Carver::round(scan_temp_bytes(nodes)) + 4096;
+         Carver::round(scan_temp_bytes(nodes)) + Carver::round(static_cast<size_t>(edges)) + 4096;
 }
 
 int tc_run(indigo_ctx* ctx, int nodes, int edges, const int* nindex, const int* nlist, int* launches)
@@ -1044,7 +1047,7 @@ int tc_run(indigo_ctx* ctx, int nodes, int edges, const int* nindex, const int* 
   Carver cv(ctx->ws);
   const size_t vn = static_cast<size_t>(nodes), vp = vn + 1;
   TcControl* ctl = cv.take<TcControl>(1);
-  TcVert* vert = cv.take<TcVert>(vn);
+  int2* ok = cv.take<int2>(vn);
   int2* orng = cv.take<int2>(vn);
   int2* orngk = cv.take<int2>(kHubKeys + vn);
   int2* dk = cv.take<int2>(vn);
@@ -1059,6 +1062,7 @@ int tc_run(indigo_ctx* ctx, int nodes, int edges, const int* nindex, const int* 
   int* flags = cv.take<int>(vn);
   int* hidx = cv.take<int>(vp);
   int* stemp = cv.take<int>(scan_temp_bytes(nodes) / 4 + 1);
+  unsigned char* up = cv.take<unsigned char>(static_cast<size_t>(edges));
   IG_TRY(zero_async(ctx, ctl, sizeof(TcControl)));
   hipStream_t s = ctx->stream;
   const int gw = share_grid(ctx, grid_for((nodes + kWave - 1) / kWave, kTcBlock / kWave, 1 << 16), kTcBlock);
@@ -1080,12 +1084,12 @@ int tc_run(indigo_ctx* ctx, int nodes, int edges, const int* nindex, const int* 
     tc_hub_keys<<<gv, kTcBlock, 0, s>>>(nodes, nindex, ctl, hidx, dk);
     n_launch += 7;
   }
-  const OutSel out{dk};
-  const OutRec orec{dk, vert, orng, orngk};
+  const OutSel out{dk, up};
+  const OutRec orec{dk, ok, orng, orngk};
   tc_select<true><<<gw, kTcBlock, 0, s>>>(nodes, nindex, nlist, out, orec, olist, heavy, ctl);
   tc_select_heavy<true><<<gh, kTcBlock, 0, s>>>(nindex, nlist, out, orec, olist, heavy, hcnt, ctl);
   tc_pack_huge<<<gh, kTcBlock, 0, s>>>(nindex, olist, heavy, hcnt, out, orec, ctl);
-  const InSel in{vert};
+  const InSel in{up, ok};
   const InRec irec{ilen};
   tc_select<false><<<gw, kTcBlock, 0, s>>>(nodes, nindex, nlist, in, irec, ilist, heavy, ctl);
   tc_select_heavy<false><<<gh, kTcBlock, 0, s>>>(nindex, nlist, in, irec, ilist, heavy, hcnt, ctl);
