@@ -458,17 +458,6 @@ __global__ __launch_bounds__(kTcBlock) void tc_pack_huge(const int* __restrict__
   }
 }
 
-// row[k] with a 32-bit byte offset from a wave-uniform base (k < 2^30): the
-// load takes the base from scalar registers.
-IG_DEV int load_at(const int* row, int k)
-{
-  return *reinterpret_cast<const int*>(reinterpret_cast<const char*>(row) + static_cast<unsigned>(k) * 4u);
-}
-IG_DEV int load_at(const unsigned short* row, int k)
-{
-  return *reinterpret_cast<const unsigned short*>(reinterpret_cast<const char*>(row) + static_cast<unsigned>(k) * 2u);
-}
-
 // Not a key and not the empty-slot marker: never found in a table.
 constexpr int kNoEntry = -2;
 
@@ -570,8 +559,9 @@ struct Hits {
 
 // Membership tests of a task's N+(x) in LDS, as policies of the walks below.
 // Wave-collective: counts the keys w[0..kN) found.  A key is kNoEntry (never
-// found) unless kSure: then every w[q] is a real entry of a 16-bit row, so a
-// hub key (< kHubKeys).
+// found) unless kSure: then every w[q] was read from a 16-bit row, so it is
+// below kHubKeys, and the lanes with ok[q] false (past the row's end, holding
+// whatever follows it) must not count.
 //
 // HashProbe: the hash table (any keys): the home buckets of every window at
 // once, and the few lanes whose bucket is full and holds no match go on
@@ -581,8 +571,9 @@ struct HashProbe {
   const int* tab;
   TcTable t;
   template <int kN, bool kSure>
-  IG_DEV void windows(const int* w, int lane, Hits& h) const
+  IG_DEV void windows(const int* w, const bool*, int lane, Hits& h) const
   {
+    static_assert(!kSure, "16-bit rows with unmasked lanes go to BitProbe only");
     unsigned b[kN];
     int4 sl[kN];
 #pragma unroll
@@ -617,7 +608,7 @@ struct BitProbe {
   static constexpr int kWords = kHubKeys / 32 + 1;
   const unsigned* bits;
   template <int kN, bool kSure>
-  IG_DEV void windows(const int* w, int, Hits& h) const
+  IG_DEV void windows(const int* w, const bool* ok, int, Hits& h) const
   {
     unsigned v[kN];
 #pragma unroll
@@ -626,33 +617,36 @@ struct BitProbe {
       v[q] = bits[(kSure ? k : min(k, static_cast<unsigned>(kHubKeys))) >> 5];
     }
 #pragma unroll
-    for (int q = 0; q < kN; q++)  // v_bfe_u32 takes the offset mod 32 by itself
-      h.lane += __builtin_amdgcn_ubfe(v[q], static_cast<unsigned>(w[q]), 1u);
+    for (int q = 0; q < kN; q++) {  // v_bfe_u32 takes the offset mod 32 by itself
+      const unsigned hit = __builtin_amdgcn_ubfe(v[q], static_cast<unsigned>(w[q]), 1u);
+      h.lane += !kSure || ok[q] ? hit : 0u;
+    }
   }
 };
 
 // Wave-collective: kN windows of 64 entries of row[0, ol) from `base`
-// (windows past base + 64*kN are not touched), then counted.  kFull: all kN
-// windows lie inside the row.  Otherwise every lane loads - past the end:
-// the last entry again, a cache hit - so that the loads issue back to back,
-// and kNoEntry marks the lanes past the end.
+// (windows past base + 64*kN are not touched), then counted.  Every lane
+// loads, from one address with the windows at immediate offsets, so the
+// loads issue back to back; unless kFull (all kN windows inside the row)
+// the lanes past the row's end read what follows it, which the list arrays'
+// padding keeps inside their allocation (kListPad).  Those lanes are masked:
+// 16-bit rows under BitProbe keep the value (a valid bitmap index) and mask
+// the lane's count, the others take kNoEntry.
 template <typename T, int kN, bool kFull, typename P>
 IG_DEV void walk_step(const T* row, int base, int ol, int lane, const P& pr, Hits& h)
 {
   static_assert(kN >= 1 && kN <= kWalk, "window count");
+  constexpr bool kSure = sizeof(T) == 2 && P::kLaneHits;
   int w[kN];
-  const T* at = row + base + lane;  // kFull: one address, the windows at immediate offsets
+  bool ok[kN];
+  const T* at = row + base + lane;
 #pragma unroll
   for (int q = 0; q < kN; q++) {
-    const int k = base + q * kWave + lane;
-    if (kFull) {
-      w[q] = at[q * kWave];
-    } else {
-      const int v = load_at(row, min(k, ol - 1));
-      w[q] = k < ol ? v : kNoEntry;
-    }
+    const int v = at[q * kWave];
+    ok[q] = kFull || base + q * kWave + lane < ol;
+    w[q] = kFull || kSure || ok[q] ? v : kNoEntry;
   }
-  pr.template windows<kN, kFull && sizeof(T) == 2>(w, lane, h);
+  pr.template windows<kN, kSure>(w, ok, lane, h);
 }
 
 // Wave-collective: the whole of row[0, ol): steps of kWalk whole windows,
@@ -697,7 +691,7 @@ IG_DEV void balanced_step(const T* __restrict__ list, int incl, unsigned shift, 
     const int v = list[ok ? at[q] : fb];
     w[q] = ok ? v : kNoEntry;
   }
-  pr.template windows<kN, false>(w, lane, h);
+  pr.template windows<kN, false>(w, nullptr, lane, h);
 }
 
 // Wave-collective: each lane's short row of slen entries from list[start]
@@ -1013,7 +1007,10 @@ int grid_for(long long items, int per_block, int cap)
   return static_cast<int>(g < 1 ? 1 : (g > cap ? cap : g));
 }
 
-size_t list_words(int edges) { return static_cast<size_t>(edges) + 16; }
+// The walks read up to kWalk * 64 - 1 entries past a row's end (see
+// walk_step): at most that many 32-bit words past the last row.
+constexpr size_t kListPad = kWalk * kWave + 16;
+size_t list_words(int edges) { return static_cast<size_t>(edges) + kListPad; }
 
 // Parts: at most one per vertex plus one per kPartItems list entries.
 size_t part_words(int nodes, int edges)
