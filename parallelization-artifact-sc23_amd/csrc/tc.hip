@@ -606,7 +606,9 @@ struct HashProbe {
 struct BitProbe {
   static constexpr bool kLaneHits = true;
   static constexpr int kWords = kHubKeys / 32 + 1;
+  static constexpr int kHistAt = 4096;  // the waves' owner histograms follow the bitmap in LDS
   const unsigned* bits;
+  int* hist;  // this wave's kWalk * 64 ints (16-byte aligned), for balanced_step
   template <int kN, bool kSure>
   IG_DEV void windows(const int* w, const bool* ok, int, Hits& h) const
   {
@@ -668,20 +670,45 @@ IG_DEV void walk_row(const T* row, int ol, int lane, const P& pr, Hits& h)
 
 // Wave-collective: kN windows of 64 pairs of the in-wave balancer from
 // `base` (< total): pair k of the lanes' concatenated short rows is
-// list[shift[o] + k] for its owner lane o (incl: the inclusive prefix of the
-// row lengths).  Owners first, then every lane loads (past the end: window
-// 0's first pair again) so that the loads issue back to back, then counted.
+// list[shift[o] + k] for its owner lane o = #{lanes i : incl_i <= k} (incl:
+// the inclusive prefix of the row lengths).  Owners first, then every lane
+// loads (past the end: window 0's first pair again) so that the loads issue
+// back to back, then counted.
+//
+// Owners: with an LDS histogram (policies that carry one) the rows ending
+// inside the step's windows add 1 at their end position, and one wave scan
+// per window gives every lane its owner (about 10 VALU instructions a
+// window); else a binary search over the lanes' incl through 6 shuffles
+// (about 45 a window).
 template <typename T, int kN, typename P>
 IG_DEV void balanced_step(const T* __restrict__ list, int incl, unsigned shift, int total, int base, int lane,
                           const P& pr, Hits& h)
 {
   static_assert(kN >= 1 && kN <= kWalk, "window count");
   unsigned at[kN];
+  if constexpr (P::kLaneHits) {
+    int* hs = pr.hist;
+    reinterpret_cast<int4*>(hs)[lane] = make_int4(0, 0, 0, 0);  // kWalk * 64 ints
+    wave_lds_sync();
+    const int qe = incl - base;  // this lane's row ends before pair base + qe
+    if (qe >= 1 && qe < kN * kWave) atomicAdd(&hs[qe], 1);
+    wave_lds_sync();
+    int carry = __popcll(__ballot(qe <= 0));  // rows ended by pair `base`
 #pragma unroll
-  for (int q = 0; q < kN; q++) {
-    const int k = base + q * kWave + lane;
-    const int o = min(wave_owner(incl, k), kWave - 1);
-    at[q] = wave_shfl(shift, o) + static_cast<unsigned>(k);
+    for (int q = 0; q < kN; q++) {
+      const int sc = wave_inclusive_scan(hs[q * kWave + lane]);
+      const int o = min(carry + sc, kWave - 1);
+      carry += wave_bcast(sc, kWave - 1);
+      at[q] = wave_shfl(shift, o) + static_cast<unsigned>(base + q * kWave + lane);
+    }
+    wave_lds_sync();  // the reads above before the next step's clear
+  } else {
+#pragma unroll
+    for (int q = 0; q < kN; q++) {
+      const int k = base + q * kWave + lane;
+      const int o = min(wave_owner(incl, k), kWave - 1);
+      at[q] = wave_shfl(shift, o) + static_cast<unsigned>(k);
+    }
   }
   const unsigned fb = static_cast<unsigned>(wave_bcast(static_cast<int>(at[0]), 0));  // pair `base`, which exists
   int w[kN];
@@ -971,7 +998,8 @@ __global__ __launch_bounds__(kBlock) IG_SGPR_CAP void tc_count_group(const int2*
     if (x.narrow) {
       // a hub's N+(x) as a bitmap of hub keys (kHubKeys bits, then a zero
       // word)
-      static_assert(kSlots >= BitProbe::kWords, "the bitmap fits the table's LDS");
+      static_assert(BitProbe::kWords <= BitProbe::kHistAt && BitProbe::kHistAt + kWaves * kWalk * kWave <= kSlots,
+                    "the bitmap and the owner histograms fit the table's LDS");
       unsigned* bits = reinterpret_cast<unsigned*>(s_tab);
       for (int i = threadIdx.x; i < BitProbe::kWords; i += kBlock) bits[i] = 0u;
       __syncthreads();
@@ -980,7 +1008,8 @@ __global__ __launch_bounds__(kBlock) IG_SGPR_CAP void tc_count_group(const int2*
         atomicOr(&bits[k >> 5], 1u << (k & 31u));
       }
       __syncthreads();
-      cnt += task_pairs(orngk, olist, ilist, x, lo, hi, wid * kWave, kWaves * kWave, BitProbe{bits});
+      cnt += task_pairs(orngk, olist, ilist, x, lo, hi, wid * kWave, kWaves * kWave,
+                        BitProbe{bits, s_tab + BitProbe::kHistAt + wid * kWalk * kWave});
       __syncthreads();
     } else {
       // N+(x) hashed lds_list entries at a time: one piece unless the list
