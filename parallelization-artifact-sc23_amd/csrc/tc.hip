@@ -90,7 +90,8 @@ constexpr int kWaveWork = 1 << 15;  // probe bound of a wave-pass task
 constexpr int kPartItems = 1024;    // items per workgroup-pass part
 constexpr int kUnroll = 4;          // windows of 64 per step over a long list
 constexpr int kLongList = 96;       // lists walked by the whole wave (64, 160, 256: slower; 128: the same)
-constexpr int kWalk = 4;            // windows of 64 in flight per step of that walk (2 and 8: slower)
+constexpr int kWalk = 4;            // windows of 64 in flight per step of that walk (2 and 8: slower);
+                                    // bitmap tasks, whose windows need fewer registers, take 2 * kWalk
 constexpr int kTinyList = 16;       // wave-pass tasks this small are counted kTinyGroup at a time
 constexpr int kTinyGroup = 16;      // tables of kTinySlots side by side in the wave's region
 constexpr int kTinySlots = 64;
@@ -608,7 +609,7 @@ struct BitProbe {
   static constexpr int kWords = kHubKeys / 32 + 1;
   static constexpr int kHistAt = 4096;  // the waves' owner histograms follow the bitmap in LDS
   const unsigned* bits;
-  int* hist;  // this wave's kWalk * 64 ints (16-byte aligned), for balanced_step
+  int* hist;  // this wave's 2 * kWalk * 64 ints (16-byte aligned), for balanced_step
   template <int kN, bool kSure>
   IG_DEV void windows(const int* w, const bool* ok, int, Hits& h) const
   {
@@ -637,7 +638,7 @@ struct BitProbe {
 template <typename T, int kN, bool kFull, typename P>
 IG_DEV void walk_step(const T* row, int base, int ol, int lane, const P& pr, Hits& h)
 {
-  static_assert(kN >= 1 && kN <= kWalk, "window count");
+  static_assert(kN >= 1 && kN <= 2 * kWalk, "window count");
   constexpr bool kSure = sizeof(T) == 2 && P::kLaneHits;
   int w[kN];
   bool ok[kN];
@@ -651,13 +652,18 @@ IG_DEV void walk_step(const T* row, int base, int ol, int lane, const P& pr, Hit
   pr.template windows<kN, kSure>(w, ok, lane, h);
 }
 
-// Wave-collective: the whole of row[0, ol): steps of kWalk whole windows,
-// then one step of the 1 to kWalk windows left (the last of them partial).
+// Wave-collective: the whole of row[0, ol): steps of kWalk whole windows
+// (2 * kWalk under lane-counted probes, while that many are left), then one
+// step of the 1 to kWalk windows left (the last of them partial).
 template <typename T, typename P>
 IG_DEV void walk_row(const T* row, int ol, int lane, const P& pr, Hits& h)
 {
   static_assert(kWalk == 4, "the switch below covers 1 to kWalk windows");
   int base = 0;
+  if constexpr (P::kLaneHits) {
+    for (; ol - base >= 2 * kWalk * kWave; base += 2 * kWalk * kWave)
+      walk_step<T, 2 * kWalk, true>(row, base, ol, lane, pr, h);
+  }
   for (; ol - base >= kWalk * kWave; base += kWalk * kWave) walk_step<T, kWalk, true>(row, base, ol, lane, pr, h);
   switch ((ol - base + kWave - 1) / kWave) {
     case 1: walk_step<T, 1, false>(row, base, ol, lane, pr, h); break;
@@ -684,11 +690,12 @@ template <typename T, int kN, typename P>
 IG_DEV void balanced_step(const T* __restrict__ list, int incl, unsigned shift, int total, int base, int lane,
                           const P& pr, Hits& h)
 {
-  static_assert(kN >= 1 && kN <= kWalk, "window count");
+  static_assert(kN >= 1 && kN <= 2 * kWalk, "window count");
   unsigned at[kN];
   if constexpr (P::kLaneHits) {
     int* hs = pr.hist;
-    reinterpret_cast<int4*>(hs)[lane] = make_int4(0, 0, 0, 0);  // kWalk * 64 ints
+    reinterpret_cast<int4*>(hs)[lane] = make_int4(0, 0, 0, 0);  // kWalk * 64 ints, twice for 2 * kWalk windows
+    if (kN > kWalk) reinterpret_cast<int4*>(hs)[kWave + lane] = make_int4(0, 0, 0, 0);
     wave_lds_sync();
     const int qe = incl - base;  // this lane's row ends before pair base + qe
     if (qe >= 1 && qe < kN * kWave) atomicAdd(&hs[qe], 1);
@@ -723,7 +730,8 @@ IG_DEV void balanced_step(const T* __restrict__ list, int incl, unsigned shift, 
 
 // Wave-collective: each lane's short row of slen entries from list[start]
 // (in entries of T), all through the in-wave balancer: kWalk windows of
-// pairs per step, then one step of just the windows left.
+// pairs per step (2 * kWalk under lane-counted probes, while that many are
+// left), then one step of just the windows left.
 template <typename T, typename P>
 IG_DEV void balanced_rows(const T* __restrict__ list, unsigned start, int slen, int lane, const P& pr, Hits& h)
 {
@@ -732,6 +740,10 @@ IG_DEV void balanced_rows(const T* __restrict__ list, unsigned start, int slen, 
   if (total == 0) return;
   const unsigned shift = start - static_cast<unsigned>(incl - slen);  // pair k is list[shift[o] + k]
   int base = 0;
+  if constexpr (P::kLaneHits) {
+    for (; total - base >= 2 * kWalk * kWave; base += 2 * kWalk * kWave)
+      balanced_step<T, 2 * kWalk>(list, incl, shift, total, base, lane, pr, h);
+  }
   for (; total - base > (kWalk - 1) * kWave; base += kWalk * kWave)
     balanced_step<T, kWalk>(list, incl, shift, total, base, lane, pr, h);
   switch ((total - base + kWave - 1) / kWave) {
@@ -998,7 +1010,7 @@ __global__ __launch_bounds__(kBlock) IG_SGPR_CAP void tc_count_group(const int2*
     if (x.narrow) {
       // a hub's N+(x) as a bitmap of hub keys (kHubKeys bits, then a zero
       // word)
-      static_assert(BitProbe::kWords <= BitProbe::kHistAt && BitProbe::kHistAt + kWaves * kWalk * kWave <= kSlots,
+      static_assert(BitProbe::kWords <= BitProbe::kHistAt && BitProbe::kHistAt + kWaves * 2 * kWalk * kWave <= kSlots,
                     "the bitmap and the owner histograms fit the table's LDS");
       unsigned* bits = reinterpret_cast<unsigned*>(s_tab);
       for (int i = threadIdx.x; i < BitProbe::kWords; i += kBlock) bits[i] = 0u;
@@ -1009,7 +1021,7 @@ __global__ __launch_bounds__(kBlock) IG_SGPR_CAP void tc_count_group(const int2*
       }
       __syncthreads();
       cnt += task_pairs(orngk, olist, ilist, x, lo, hi, wid * kWave, kWaves * kWave,
-                        BitProbe{bits, s_tab + BitProbe::kHistAt + wid * kWalk * kWave});
+                        BitProbe{bits, s_tab + BitProbe::kHistAt + wid * 2 * kWalk * kWave});
       __syncthreads();
     } else {
       // N+(x) hashed lds_list entries at a time: one piece unless the list
