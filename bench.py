@@ -45,8 +45,8 @@ def tc_count_model(g) -> dict:
     its out-list plus its in-list.  Tasks with 2 <= |N+(x)| <= 256 and
     (|N+| + |N-|) * |N+| <= 32768 run in the wave pass, the rest with
     |N+(x)| <= 2048 in this kernel, in parts of 1024 items.  Hubs are the at
-    most 65536 vertices of largest degree (all vertices when they fit, else
-    min(degree, 65536) >= the smallest bin whose suffix holds at most 65536
+    most 65535 vertices of largest degree (all vertices when they fit, else
+    min(degree, 65536) >= the smallest bin whose suffix holds at most 65535
     vertices); a hub's out-list holds hubs only and is stored as 16-bit keys.
     Algorithmic bytes per launch: per probe the element of the iterated list
     (2 from a hub's list, else 4), 12 per item (its id and offset pair), per
@@ -54,12 +54,12 @@ def tc_count_model(g) -> dict:
     (task header)."""
     v = g.nodes
     deg = g.degree.astype(np.int64)
-    if v <= 65536:
+    if v <= 65535:
         hub = np.ones(v, dtype=bool)
     else:
         dcl = np.minimum(deg, 65536)
         suffix = np.cumsum(np.bincount(dcl, minlength=65537)[::-1])[::-1]
-        ok = np.flatnonzero(suffix <= 65536)
+        ok = np.flatnonzero(suffix <= 65535)
         hub = dcl >= (ok[0] if ok.size else 65537)
     src = np.repeat(np.arange(v, dtype=np.int64), deg)
     dst = g.nlist.astype(np.int64)

@@ -199,7 +199,7 @@ int indigo_ctx_set_hint(indigo_ctx* ctx, int key, int value);
 #define INDIGO_HINT_GRAPH 11        /* 1: CC and TC run their kernel sequence as a HIP graph, captured on the
                                         first call with the same graph, output and workspace and replayed after
                                         (one launch instead of 5-10); such calls report no separate main kernel */
-#define INDIGO_HINT_TC_HUBS 12       /* cap on TC's hub vertices, whose out-lists hold 16-bit keys (0: 65536) */
+#define INDIGO_HINT_TC_HUBS 12       /* cap on TC's hub vertices, whose out-lists hold 16-bit keys (0: 65535) */
 #define INDIGO_VARIANT_VERTEX 0      /* vertex-centric, data-driven (degree-binned worklists); the default */
 #define INDIGO_VARIANT_EDGE 1        /* edge-centric, topology-driven: one lane per CSR entry over an edge-source
                                         array built on the device (the suite's E_Topo_Push family) */

@@ -19,7 +19,7 @@
 // so every oriented edge is taken exactly once, by its endpoint with the
 // longer list (2.4x fewer probes than always iterating N+(u) on R-MAT).
 //
-// Keys.  The hubs are the at most 65536 vertices of largest degree (a
+// Keys.  The hubs are the at most 65535 vertices of largest degree (a
 // degree threshold: every vertex above it is a hub), and they are numbered
 // 0.. in id order; every other vertex v is named by the key 65536 + v.  The
 // hubs are closed upwards in the (degree, id) order, so the out-list of a hub
@@ -34,7 +34,7 @@
 // builds it - no count pass, no scan:
 //   0. keys       : degree histogram, hub threshold, hub numbering (a scan
 //                  of the hub flags) and (degree, key) per vertex; on a
-//                  graph of at most 65536 vertices every vertex is a hub and
+//                  graph of at most 65535 vertices every vertex is a hub and
 //                  its key is its id, in one pass;
 //   1. out-select : N+ lists, order-preserving; lists < kMidDeg by the
 //                  in-wave balancer, longer ones binned into mid (one wave
@@ -99,6 +99,8 @@ constexpr int kWaveRegion = kWaveSlots;  // a wave's LDS words
 static_assert(4 * kTinyList <= kTinySlots && kTinyGroup * kTinySlots <= kWaveRegion && kTinyList < kLongList,
               "tiny-task tables: 64 slots each, all in the wave's region, no long lists");
 constexpr int kHubKeys = 1 << 16;  // keys below this number hubs (16-bit rows); others are 65536 + id
+constexpr int kMaxHubs = kHubKeys - 1;  // hub numbers 0..65534: 0xffff marks "no hub" in the 4-byte records
+constexpr int kRankDeg = (1 << 15) - 1;  // degrees at or above this compare exactly (from nindex)
 constexpr int kHistLds = 2048;     // degree bins a workgroup counts in LDS
 constexpr int kThreshBlock = 1024;
 static_assert(kHubKeys % kThreshBlock == 0, "the threshold pass gives every thread a whole run of bins");
@@ -129,19 +131,36 @@ IG_DEV void row_put(int* list, int b, int k, int key, bool narrow)
   else list[b + k] = key;
 }
 
+// Key of vertex u from the low half of its 4-byte record: its hub number, or
+// 0xffff for "not a hub".
+IG_DEV int key_of(unsigned rec, int u)
+{
+  const int h = static_cast<int>(rec & 0xffffu);
+  return h != 0xffff ? h : kHubKeys + u;
+}
+
 // Out-list of v: the keys of the neighbours that outrank v; 16-bit when v is
-// a hub.  Each CSR entry e of v's row also gets its outcome in up[e], so the
-// in-selection needs no rank test.
+// a hub.  A neighbour's rank and key come from one 4-byte record, rk[u] =
+// min(degree, kRankDeg) << 16 | hub number (0xffff: none), so the gathers
+// read a 16 MB array at scale 22 instead of 32 MB; two clamped degrees
+// compare exactly from nindex.  Each CSR entry e of v's row also gets its
+// outcome in up[e], so the in-selection needs no rank test.
 struct OutSel {
   const int2* __restrict__ dk;  // (degree, key) per vertex
+  const unsigned* __restrict__ rk;
+  const int* __restrict__ nindex;
   unsigned char* up;            // per CSR entry: 1 when the neighbour outranks the row's vertex
   IG_DEV int own_odeg(int) const { return 0; }
   IG_DEV bool narrow(int v) const { return narrow_key(dk[v].y); }
   IG_DEV bool operator()(int v, int dv, int, int e, int u, int& key) const
   {
-    const int2 r = dk[u];
-    key = r.y;
-    const bool k = outranks(r.x, u, dv, v);
+    const unsigned r = rk[u];
+    key = key_of(r, u);
+    const int cu = static_cast<int>(r >> 16), cv = min(dv, kRankDeg);
+    bool k;
+    if (cu != cv) k = cu > cv;
+    else if (cu < kRankDeg) k = u > v;  // equal degrees
+    else k = outranks(nindex[u + 1] - nindex[u], u, dv, v);
     up[e] = k ? 1 : 0;
     return k;
   }
@@ -150,18 +169,19 @@ struct OutSel {
 // In-list of v: the keys of the neighbours u that v outranks (u -> v) whose
 // out-list is shorter than v's; always 32-bit.  The out-selection's up[e]
 // rules out the neighbours that outrank v (about half the entries) with a
-// 1-byte coalesced load; only the others read their (|N+(u)|, key).
+// 1-byte coalesced load; only the others read their 4-byte record
+// ok[u] = |N+(u)| << 16 | hub number (|N+| <= sqrt(E) < 2^16).
 struct InSel {
   const unsigned char* __restrict__ up;
-  const int2* __restrict__ ok;  // (|N+(v)|, key) per vertex
-  IG_DEV int own_odeg(int v) const { return ok[v].x; }
+  const unsigned* __restrict__ ok;
+  IG_DEV int own_odeg(int v) const { return static_cast<int>(ok[v] >> 16); }
   IG_DEV bool narrow(int) const { return false; }
   IG_DEV bool operator()(int, int, int odv, int e, int u, int& key) const
   {
     if (up[e] != 0) return false;
-    const int2 r = ok[u];
-    key = r.y;
-    return r.x < odv;
+    const unsigned r = ok[u];
+    key = key_of(r, u);
+    return static_cast<int>(r >> 16) < odv;
   }
 };
 
@@ -169,13 +189,13 @@ struct InSel {
 // CSR position (list[nindex[v] + k]), so no count pass or scan is needed.
 struct OutRec {
   const int2* __restrict__ dk;
-  int2* ok;     // (|N+(v)|, key) by vertex id
-  int2* orng;   // (start, |N+(v)|) by vertex id
-  int2* orngk;  // the same by key
+  unsigned* ok;  // |N+(v)| << 16 | hub number (0xffff: none) by vertex id
+  int2* orng;    // (start, |N+(v)|) by vertex id
+  int2* orngk;   // the same by key
   IG_DEV void operator()(int v, int, int beg, int kept) const
   {
     const int key = dk[v].y;
-    ok[v] = make_int2(kept, key);
+    ok[v] = static_cast<unsigned>(kept) << 16 | static_cast<unsigned>(narrow_key(key) ? key : 0xffff);
     orng[v] = make_int2(beg, kept);
     orngk[key] = make_int2(beg, kept);
   }
@@ -250,13 +270,14 @@ __global__ __launch_bounds__(kTcBlock) void tc_hub_flags(int nodes, const int* _
 // null when every vertex is a hub (then a hub's number is its id).
 __global__ __launch_bounds__(kTcBlock) void tc_hub_keys(int nodes, const int* __restrict__ nindex,
                                                         const TcControl* ctl, const int* __restrict__ hidx,
-                                                        int2* dk)
+                                                        int2* dk, unsigned* rk)
 {
   const int hb = hidx ? ctl->hub_bin : 0;
   for (int v = blockIdx.x * kTcBlock + threadIdx.x; v < nodes; v += gridDim.x * kTcBlock) {
     const int d = nindex[v + 1] - nindex[v];
     const int key = !hidx ? v : (min(d, kHubKeys) >= hb ? hidx[v] : kHubKeys + v);
     dk[v] = make_int2(d, key);
+    rk[v] = static_cast<unsigned>(min(d, kRankDeg)) << 16 | static_cast<unsigned>(narrow_key(key) ? key : 0xffff);
   }
 }
 
@@ -1071,7 +1092,7 @@ size_t tc_workspace_bytes(int nodes, int edges)
 {
   // the takes of tc_run, in its order
   const size_t v = static_cast<size_t>(nodes), vp = v + 1;
-  return Carver::round(sizeof(TcControl)) + Carver::round(v * sizeof(int2)) + Carver::round(v * sizeof(int2)) +
+  return Carver::round(sizeof(TcControl)) + 2 * Carver::round(v * 4) + Carver::round(v * sizeof(int2)) +
          Carver::round((kHubKeys + v) * sizeof(int2)) + Carver::round(v * sizeof(int2)) + Carver::round(v * 4) +
          3 * Carver::round(vp * 4) + 2 * Carver::round(part_words(nodes, edges) * 8) +
          2 * Carver::round(list_words(edges) * 4) + Carver::round(hcnt_words(edges) * 4) +
@@ -1085,7 +1106,8 @@ int tc_run(indigo_ctx* ctx, int nodes, int edges, const int* nindex, const int* 
   Carver cv(ctx->ws);
   const size_t vn = static_cast<size_t>(nodes), vp = vn + 1;
   TcControl* ctl = cv.take<TcControl>(1);
-  int2* ok = cv.take<int2>(vn);
+  unsigned* ok = cv.take<unsigned>(vn);
+  unsigned* rk = cv.take<unsigned>(vn);
   int2* orng = cv.take<int2>(vn);
   int2* orngk = cv.take<int2>(kHubKeys + vn);
   int2* dk = cv.take<int2>(vn);
@@ -1110,19 +1132,19 @@ int tc_run(indigo_ctx* ctx, int nodes, int edges, const int* nindex, const int* 
   // keys: every vertex is a hub (key = id) when they all fit, else the
   // highest-degree ones up to the cap, numbered in id order
   const int hub_hint = ctx->hints[INDIGO_HINT_TC_HUBS];
-  const int hub_cap = hub_hint > 0 && hub_hint < kHubKeys ? hub_hint : kHubKeys;
+  const int hub_cap = hub_hint > 0 && hub_hint < kMaxHubs ? hub_hint : kMaxHubs;
   if (nodes <= hub_cap) {
-    tc_hub_keys<<<gv, kTcBlock, 0, s>>>(nodes, nindex, ctl, nullptr, dk);
+    tc_hub_keys<<<gv, kTcBlock, 0, s>>>(nodes, nindex, ctl, nullptr, dk, rk);
   } else {
     IG_TRY(zero_async(ctx, hist, (kHubKeys + 1) * sizeof(unsigned)));
     tc_degree_hist<<<gv, kTcBlock, 0, s>>>(nodes, nindex, hist);
     tc_hub_threshold<<<1, kThreshBlock, 0, s>>>(hist, hub_cap, ctl);
     tc_hub_flags<<<gv, kTcBlock, 0, s>>>(nodes, nindex, ctl, flags);
     IG_TRY(exclusive_scan(s, flags, hidx, nodes, stemp));
-    tc_hub_keys<<<gv, kTcBlock, 0, s>>>(nodes, nindex, ctl, hidx, dk);
+    tc_hub_keys<<<gv, kTcBlock, 0, s>>>(nodes, nindex, ctl, hidx, dk, rk);
     n_launch += 7;
   }
-  const OutSel out{dk, up};
+  const OutSel out{dk, rk, nindex, up};
   const OutRec orec{dk, ok, orng, orngk};
   tc_select<true><<<gw, kTcBlock, 0, s>>>(nodes, nindex, nlist, out, orec, olist, heavy, ctl);
   tc_select_heavy<true><<<gh, kTcBlock, 0, s>>>(nindex, nlist, out, orec, olist, heavy, hcnt, ctl);

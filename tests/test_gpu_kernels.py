@@ -162,7 +162,7 @@ def test_tc_long_oriented_lists(ig, orc, ctx, lds_list):
 
 
 def test_tc_rmat17_hubs_and_others(ig, orc, ctx):
-    """R-MAT 17 has 131072 vertices: the 65536 of highest degree are hubs,
+    """R-MAT 17 has 131072 vertices: up to 65535 of highest degree are hubs,
     whose out-lists hold 16-bit keys and whose workgroup-pass tasks probe a
     bitmap, and the others keep 32-bit rows and hash tables, so tasks, items
     and probed rows of both kinds mix in every pass."""
@@ -193,6 +193,21 @@ def test_tc_hub_caps(ig, orc, ctx, hubs):
     finally:
         ctx.set_hint(ig._lib.HINT_TC_HUBS, 0)
         ctx.set_hint(ig._lib.HINT_TC_LDS_LIST, 0)
+
+
+@pytest.mark.parametrize("extra_b", [0, 7])
+def test_tc_two_big_hubs(ig, ctx, extra_b):
+    """Two adjacent hubs with 40000+ leaves each, 5000 of them shared: 5000
+    triangles (hub, hub, shared leaf).  Degrees at or above 32767 are ranked
+    from nindex rather than the clamped 4-byte record; with extra_b 0 the two
+    degrees tie (the orientation falls back to ids), with 7 they differ."""
+    n_a, n_b, shared = 40000, 40000 + extra_b, 5000
+    leaves_a = np.arange(2, 2 + n_a)
+    leaves_b = np.arange(2 + n_a - shared, 2 + n_a - shared + n_b)
+    src = np.concatenate([[0], np.zeros(n_a, np.int64), np.ones(n_b, np.int64)])
+    dst = np.concatenate([[1], leaves_a, leaves_b])
+    g = ig.from_edges(int(dst.max()) + 1, src, dst)
+    assert ctx.tc(ig.DeviceGraph(g)) == shared
 
 
 def test_tc_rejects_unsorted(ig, ctx):
