@@ -105,14 +105,20 @@ constexpr int kHistLds = 2048;     // degree bins a workgroup counts in LDS
 constexpr int kThreshBlock = 1024;
 static_assert(kHubKeys % kThreshBlock == 0, "the threshold pass gives every thread a whole run of bins");
 
+// The wave pass's vertex grabs go through kGrabStripes counters, one per
+// 128-byte line: same-address atomics serialise (about 10 ns each), and one
+// counter for all 65536 grabs of R-MAT 22 had set the pass's length.
+constexpr int kGrabStripes = 16;
+
 struct TcControl {
   FrontierCounters heavy;  // lists of >= kMidDeg entries
   unsigned long long count;
   unsigned nmid, nbig;     // tasks deferred to the workgroup passes
-  unsigned next, next_mid, next_big;
+  unsigned next_mid, next_big;
   int error;
   int hub_bin;             // hubs: min(degree, kHubKeys) >= hub_bin
-  int pad[23];
+  int pad[24];
+  unsigned grab[kGrabStripes][32];  // stripe s: grabs of chunks s, s + kGrabStripes, ... (first word used)
 };
 
 IG_DEV bool outranks(int du, int u, int dv, int v) { return du > dv || (du == dv && u > v); }
@@ -945,15 +951,33 @@ __global__ __launch_bounds__(kTcBlock) IG_SGPR_CAP void tc_count(int nodes, int 
   const int lane = lane_id();
   int* tab = s_tab[threadIdx.x / kWave];
   unsigned long long cnt = 0;
+  // chunks of `grab` (at most 64) consecutive vertices, one per lane: 64 on
+  // large graphs (8 grabs per wave on average at R-MAT 22: coarser grabs
+  // leave a tail), fewer on small ones, whose few grabs would otherwise
+  // leave one wave with a long serial run of tasks.  Chunk c is grabbed
+  // through stripe c mod kGrabStripes; a wave starts at its own stripe and
+  // moves on when it runs dry, and leaves after finding every stripe dry in
+  // a row (a dry stripe stays dry).
+  const unsigned nch = static_cast<unsigned>((nodes + grab - 1) / grab);
+  int stripe = static_cast<int>((blockIdx.x * (kTcBlock / kWave) + threadIdx.x / kWave) % kGrabStripes);
+  int dry = 0;
   for (;;) {
-    // `grab` (at most 64) consecutive vertices per grab, one per lane: 64 on
-    // large graphs (8 grabs per wave on average at R-MAT 22: coarser grabs
-    // leave a tail), fewer on small ones, whose few grabs would otherwise
-    // leave one wave with a long serial run of tasks
-    unsigned v0 = 0;
-    if (lane == 0) v0 = atomic_add_relaxed(&ctl->next, static_cast<unsigned>(grab));
-    v0 = wave_shfl(v0, 0);
-    if (v0 >= static_cast<unsigned>(nodes)) break;
+    unsigned c = 0;
+    if (lane == 0) {
+      // a dry stripe is seen with a load, without another atomic
+      unsigned* g = &ctl->grab[stripe][0];
+      const unsigned k = ld_relaxed(g) * kGrabStripes + static_cast<unsigned>(stripe) < nch ? atomic_add_relaxed(g, 1u)
+                                                                                             : ld_relaxed(g);
+      c = k * kGrabStripes + static_cast<unsigned>(stripe);
+    }
+    c = wave_shfl(c, 0);
+    if (c >= nch) {
+      if (++dry == kGrabStripes) break;
+      stripe = (stripe + 1) % kGrabStripes;
+      continue;
+    }
+    dry = 0;
+    const unsigned v0 = c * static_cast<unsigned>(grab);
     // heavy tasks go to the workgroup passes in parts (|N+(x)| < 2: no
     // triangle can be found through N+(x))
     const int vl = static_cast<int>(v0) + lane;
