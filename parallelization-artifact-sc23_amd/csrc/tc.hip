@@ -237,27 +237,44 @@ __global__ __launch_bounds__(kThreshBlock) void tc_hub_threshold(const unsigned*
                                                                  TcControl* ctl)
 {
   constexpr int kPer = kHubKeys / kThreshBlock;
+  static_assert(kPer % 4 == 0, "runs load as uint4");
   constexpr int kWaves = kThreshBlock / kWave;
   __shared__ int s_wave[kWaves];
   __shared__ int s_best;
   const int t = threadIdx.x, lane = lane_id(), wid = t / kWave;
   const int lo = t * kPer;
-  const int hi = t == kThreshBlock - 1 ? kHubKeys + 1 : lo + kPer;  // the top bin joins the last run
-  int s = 0;
-  for (int b = lo; b < hi; b++) s += static_cast<int>(hist[b]);
+  // the run in registers, loaded at once (one dependent load per bin had
+  // made this kernel 60 us); the top bin joins the last run
+  int h[kPer];
+#pragma unroll
+  for (int q = 0; q < kPer / 4; q++) {
+    const uint4 w = reinterpret_cast<const uint4*>(hist + lo)[q];
+    h[4 * q] = static_cast<int>(w.x);
+    h[4 * q + 1] = static_cast<int>(w.y);
+    h[4 * q + 2] = static_cast<int>(w.z);
+    h[4 * q + 3] = static_cast<int>(w.w);
+  }
+  const int top = t == kThreshBlock - 1 ? static_cast<int>(hist[kHubKeys]) : 0;
+  int s = top;
+#pragma unroll
+  for (int q = 0; q < kPer; q++) s += h[q];
   // suffix sums over the lanes: an inclusive scan of the lanes in reverse
   const int inc = wave_inclusive_scan(wave_shfl(s, kWave - 1 - lane));
   const int suf = wave_shfl(inc, kWave - 1 - lane);  // s of this lane and the lanes above
   if (lane == 0) s_wave[wid] = suf;
   if (t == 0) s_best = kHubKeys + 1;
   __syncthreads();
-  int acc = suf - s;  // vertices in bins >= hi
+  int acc = suf - s;  // vertices in bins above the run
   for (int w = wid + 1; w < kWaves; w++) acc += s_wave[w];
   int best = kHubKeys + 1;
-  for (int b = hi - 1; b >= lo; b--) {
-    acc += static_cast<int>(hist[b]);
-    if (acc > cap) break;
-    best = b;
+  acc += top;
+  if (t == kThreshBlock - 1 && acc <= cap) best = kHubKeys;
+  if (acc <= cap) {
+#pragma unroll
+    for (int q = kPer - 1; q >= 0; q--) {
+      acc += h[q];
+      if (acc <= cap) best = lo + q;
+    }
   }
   if (best <= kHubKeys) atomicMin(&s_best, best);
   __syncthreads();
