@@ -1178,7 +1178,10 @@ int tc_run(indigo_ctx* ctx, int nodes, int edges, const int* nindex, const int* 
     tc_hub_keys<<<gv, kTcBlock, 0, s>>>(nodes, nindex, ctl, nullptr, dk, rk);
   } else {
     IG_TRY(zero_async(ctx, hist, (kHubKeys + 1) * sizeof(unsigned)));
-    tc_degree_hist<<<gv, kTcBlock, 0, s>>>(nodes, nindex, hist);
+    // one workgroup per CU: each flushes its busy low bins with global
+    // atomics on the same addresses as every other workgroup's
+    tc_degree_hist<<<std::min(gv, share_grid(ctx, ctx->compute_units, kTcBlock)), kTcBlock, 0, s>>>(nodes, nindex,
+                                                                                                  hist);
     tc_hub_threshold<<<1, kThreshBlock, 0, s>>>(hist, hub_cap, ctl);
     tc_hub_flags<<<gv, kTcBlock, 0, s>>>(nodes, nindex, ctl, flags);
     IG_TRY(exclusive_scan(s, flags, hidx, nodes, stemp));
