@@ -25,7 +25,9 @@
 // hubs are closed upwards in the (degree, id) order, so the out-list of a hub
 // holds hubs only and is stored as 16-bit keys; on R-MAT graphs three
 // quarters of all probes read such lists, which halves their bytes.  Lists
-// hold keys, the LDS tables hold keys, and orngk[key] locates a list.
+// hold keys, the LDS tables hold keys, and orngk[key] locates a list.  A
+// hub's task probes a bitmap of its N+(x) (kHubKeys bits) instead of a hash
+// table: one LDS word and a bit extract per key, counted per lane.
 //
 // Passes (plain launches on the context stream).  Both oriented lists of a
 // vertex are subsets of its adjacency list, so each is stored at the
@@ -33,33 +35,36 @@
 // entries of the same bytes for a hub's out-list) and one pass per list
 // builds it - no count pass, no scan:
 //   0. keys       : degree histogram, hub threshold, hub numbering (a scan
-//                  of the hub flags) and (degree, key) per vertex; on a
-//                  graph of at most 65535 vertices every vertex is a hub and
-//                  its key is its id, in one pass;
+//                  of the hub flags) and (degree, key) per vertex, with the
+//                  4-byte (clamped degree, hub number) record the
+//                  out-selection gathers; on a graph of at most 65535
+//                  vertices every vertex is a hub and its key is its id, in
+//                  one pass;
 //   1. out-select : N+ lists, order-preserving; lists < kMidDeg by the
 //                  in-wave balancer, longer ones binned into mid (one wave
 //                  each) and huge (kHugeChunk-entry items, one workgroup
 //                  each, compacted per item); also flags unsorted lists and
-//                  self loops; records (|N+|, key) per vertex and, per CSR
-//                  entry, one byte: does the neighbour outrank the vertex;
+//                  self loops; records (|N+|, hub number) per vertex and,
+//                  per CSR entry, one byte: does the neighbour outrank the
+//                  vertex;
 //   2. pack       : one workgroup per huge list closes the gaps between its
 //                  items;
 //   3-4.           the same for the in-lists N-(x): an entry whose byte is
 //                  set is out at once, the others read the neighbour's
-//                  (|N+|, key) in one 8-byte load;
+//                  (|N+|, hub number) in one 4-byte load;
 //   5. count, wave pass: a wave grabs up to 64 consecutive vertices at a time
-//      from a global counter (a static stride would be badly unbalanced: on
-//      R-MAT graphs a vertex's degree follows the bit pattern of its id); a
+//      through striped counters (a static stride would be badly unbalanced:
+//      on R-MAT graphs a vertex's degree follows the bit pattern of its id); a
 //      task with 2 <= |N+(x)| <= kWaveList and at most kWaveWork probes
 //      ((|N+(x)| + |N-(x)|) * |N+(x)| bounds them) builds an open-addressing
 //      table of N+(x) in LDS and walks the (list, element) pairs 64 per step
 //      with the in-wave balancer, one probe each.  Every other task is cut
 //      into parts of kPartItems items (out-list, then in-list), queued for
 //      pass 6;
-//   6. count, workgroup passes: a workgroup grabs a part, hashes N+(x) and
-//      spreads the part's items over its waves (512 threads and a 32 KiB
-//      table up to kMidList entries, 1024 threads and a 128 KiB table
-//      beyond).  Parts bound the work of one grab, so a few heavy vertices
+//   6. count, workgroup passes: a workgroup grabs a part, hashes N+(x) (or,
+//      for a hub, sets its bitmap) and spreads the part's items over its
+//      waves (512 threads and a 32 KiB table up to kMidList entries, 1024
+//      threads and a 128 KiB table beyond).  Parts bound the work of one grab, so a few heavy vertices
 //      cannot form a tail.  A list too long for the table (m > 2^25 edges, or
 //      a hint) is hashed a table's worth at a time, and the part's items are
 //      probed against each piece in turn (N+(x) is the disjoint union of the
