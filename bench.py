@@ -44,7 +44,7 @@ def tc_count_model(g) -> dict:
     longer out-list, iterating the shorter one (ties: v); the task of x is
     its out-list plus its in-list.  Tasks with 2 <= |N+(x)| <= 256 and
     (|N+| + |N-|) * |N+| <= 32768 run in the wave pass, the rest with
-    |N+(x)| <= 2048 in this kernel, in parts of 1024 items.  Hubs are the at
+    |N+(x)| <= 2048 in this kernel, in parts of 2048 items.  Hubs are the at
     most 65535 vertices of largest degree (all vertices when they fit, else
     min(degree, 65536) >= the smallest bin whose suffix holds at most 65535
     vertices); a hub's out-list holds hubs only and is stored as 16-bit keys.
@@ -77,7 +77,7 @@ def tc_count_model(g) -> dict:
     items = oc + inl
     wave = (oc >= 2) & (oc <= 256) & (items * oc <= 32768)
     mid = (oc >= 2) & ~wave & (oc <= 2048)
-    parts = (items + 1023) // 1024
+    parts = (items + 2047) // 2048
     m = {"probes": float(probes[mid].sum()), "items": int(items[mid].sum()), "parts": int(parts[mid].sum()),
          "table_entries": int((oc * parts)[mid].sum()), "probes_all_passes": float(probes.sum()),
          "hub_tasks": int((mid & hub).sum()), "tasks": int(mid.sum())}

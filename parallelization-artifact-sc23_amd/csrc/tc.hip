@@ -92,7 +92,7 @@ constexpr int kBigSlots = 32768;
 static_assert(kMidBlock == 512 && kMidSlots == 8192, "the main-kernel name below spells these out");
 static_assert(kHugeChunk % kTcBlock == 0, "tc_pack_huge moves whole items");
 constexpr int kWaveWork = 1 << 15;  // probe bound of a wave-pass task
-constexpr int kPartItems = 1024;    // items per workgroup-pass part
+constexpr int kPartItems = 2048;    // items per workgroup-pass part (1024 and 4096: slower)
 constexpr int kUnroll = 4;          // windows of 64 per step over a long list
 constexpr int kLongList = 96;       // lists walked by the whole wave (64, 160, 256: slower; 128: the same)
 constexpr int kWalk = 4;            // windows of 64 in flight per step of that walk (2 and 8: slower);
