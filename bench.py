@@ -43,7 +43,8 @@ def tc_count_model(g) -> dict:
     (degree, id); each oriented edge v -> u is taken by the endpoint with the
     longer out-list, iterating the shorter one (ties: v); the task of x is
     its out-list plus its in-list.  Tasks with 2 <= |N+(x)| <= 256 and
-    (|N+| + |N-|) * |N+| <= 32768 run in the wave pass, the rest with
+    (|N+| + |N-|) * |N+| <= W run in the wave pass (W = 2^13 below 2^24 CSR
+    entries, else 2^14, as tc.hip's wave_work_for), the rest with
     |N+(x)| <= 2048 in this kernel, in parts of 2048 items.  Hubs are the at
     most 65535 vertices of largest degree (all vertices when they fit, else
     min(degree, 65536) >= the smallest bin whose suffix holds at most 65535
@@ -75,7 +76,7 @@ def tc_count_model(g) -> dict:
     probes = np.bincount(owner, weights=oc[walked], minlength=v)
     pbytes = np.bincount(owner, weights=oc[walked] * np.where(hub[walked], 2.0, 4.0), minlength=v)
     items = oc + inl
-    wave = (oc >= 2) & (oc <= 256) & (items * oc <= 32768)
+    wave = (oc >= 2) & (oc <= 256) & (items * oc <= (1 << 13 if g.edges < (1 << 24) else 1 << 14))
     mid = (oc >= 2) & ~wave & (oc <= 2048)
     parts = (items + 2047) // 2048
     m = {"probes": float(probes[mid].sum()), "items": int(items[mid].sum()), "parts": int(parts[mid].sum()),

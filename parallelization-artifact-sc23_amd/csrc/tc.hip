@@ -55,7 +55,7 @@
 //   5. count, wave pass: a wave grabs up to 64 consecutive vertices at a time
 //      through striped counters (a static stride would be badly unbalanced:
 //      on R-MAT graphs a vertex's degree follows the bit pattern of its id); a
-//      task with 2 <= |N+(x)| <= kWaveList and at most kWaveWork probes
+//      task with 2 <= |N+(x)| <= kWaveList and at most wave_work probes
 //      ((|N+(x)| + |N-(x)|) * |N+(x)| bounds them) builds an open-addressing
 //      table of N+(x) in LDS and walks the (list, element) pairs 64 per step
 //      with the in-wave balancer, one probe each.  Every other task is cut
@@ -91,7 +91,13 @@ constexpr int kBigBlock = 1024;
 constexpr int kBigSlots = 32768;
 static_assert(kMidBlock == 512 && kMidSlots == 8192, "the main-kernel name below spells these out");
 static_assert(kHugeChunk % kTcBlock == 0, "tc_pack_huge moves whole items");
-constexpr int kWaveWork = 1 << 15;  // probe bound of a wave-pass task
+// Probe bound of a wave-pass task: 2^13 below 2^24 CSR entries, else 2^14.
+// The workgroup passes take the rest; on small graphs their parallelism
+// beats the wave pass's per-task round trips (R-MAT 14-18: 0.38 -> 0.25,
+// 0.71 -> 0.41 and 1.34 -> 0.89 ms against 2^15), while from R-MAT 20 on
+// 2^13 puts too many hash tasks there (scale 22: 10.6 -> 11.1 ms) and 2^14
+// ties with 2^15.
+int wave_work_for(int edges) { return edges < (1 << 24) ? 1 << 13 : 1 << 14; }
 constexpr int kPartItems = 2048;    // items per workgroup-pass part (1024 and 4096: slower)
 constexpr int kUnroll = 4;          // windows of 64 per step over a long list
 constexpr int kLongList = 96;       // lists walked by the whole wave (64, 160, 256: slower; 128: the same)
@@ -963,7 +969,8 @@ IG_DEV void append_parts(int parts, int x, unsigned long long* queue, unsigned* 
         (static_cast<unsigned long long>(x) << 32) | static_cast<unsigned>(q);
 }
 
-__global__ __launch_bounds__(kTcBlock) IG_SGPR_CAP void tc_count(int nodes, int grab, const int2* __restrict__ dk,
+__global__ __launch_bounds__(kTcBlock) IG_SGPR_CAP void tc_count(int nodes, int grab, int wave_work,
+                                                     const int2* __restrict__ dk,
                                                      const int2* __restrict__ orng, const int2* __restrict__ orngk,
                                                      const int* __restrict__ olist, const int* __restrict__ ilen,
                                                      const int* __restrict__ ilist, unsigned long long* mid,
@@ -1008,7 +1015,7 @@ __global__ __launch_bounds__(kTcBlock) IG_SGPR_CAP void tc_count(int nodes, int 
     const int nl = mine && narrow_key(dk[vl].y) ? 1 : 0;
     const int dl = rl.y;
     const int il = mine ? ilen[vl] : 0;
-    const bool here = dl >= 2 && dl <= kWaveList && (dl + il) * dl <= kWaveWork;
+    const bool here = dl >= 2 && dl <= kWaveList && (dl + il) * dl <= wave_work;
     const int parts = dl >= 2 && !here ? (dl + il + kPartItems - 1) / kPartItems : 0;
     append_parts(dl <= kMidList ? parts : 0, vl, mid, &ctl->nmid);
     append_parts(dl > kMidList ? parts : 0, vl, big, &ctl->nbig);
@@ -1208,8 +1215,8 @@ int tc_run(indigo_ctx* ctx, int nodes, int edges, const int* nindex, const int* 
     // least 1), so a small graph still spreads over 4096 grabs while the
     // grabs' same-address atomics stay few
     const int grab = std::max(1, std::min(kWave, nodes / 4096));
-    tc_count<<<share_grid(ctx, ctx->compute_units * 8, kTcBlock), kTcBlock, 0, s>>>(nodes, grab, dk, orng, orngk, olist, ilen, ilist, mid, big,
-                                                         ctl);
+    tc_count<<<share_grid(ctx, ctx->compute_units * 8, kTcBlock), kTcBlock, 0, s>>>(
+        nodes, grab, wave_work_for(edges), dk, orng, orngk, olist, ilen, ilist, mid, big, ctl);
   }
   const int hint = ctx->hints[INDIGO_HINT_TC_LDS_LIST];
   const int mid_list = hint > 0 && hint < kMidSlots / 4 ? hint : kMidSlots / 4;

@@ -84,10 +84,11 @@ def test_tc_count_model_on_cliques():
     """The TC byte model against closed forms.  On a k-clique (all degrees
     equal, so the orientation is by id and |N+(i)| = k - 1 - i) every probe
     hits, so the probes of all passes are C(k, 3); a task has no in-list, and
-    it goes to the workgroup pass exactly when |N+| > 181 (wave tasks need
-    |N+|^2 <= 32768), so on a 300-clique that pass owns the tasks with
-    |N+| in 182..299, whose probes sum to C(300, 3) - C(182, 3).  Every vertex
-    of a graph this small is a hub: 2 bytes per probe and per table entry."""
+    on a graph this small (under 2^24 CSR entries) it goes to the workgroup
+    pass exactly when |N+| > 90 (wave tasks need |N+|^2 <= 2^13), so on a
+    300-clique that pass owns the tasks with |N+| in 91..299, whose probes
+    sum to C(300, 3) - C(91, 3).  Every vertex of a graph this small is a
+    hub: 2 bytes per probe and per table entry."""
     sys.path.insert(0, os.path.join(REPO, "parallelization-artifact-sc23_amd"))
     sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
     import bench
@@ -97,8 +98,8 @@ def test_tc_count_model_on_cliques():
     sizes = [3, 5, 40, 300]
     m = bench.tc_count_model(cliques(ig, sizes))
     assert m["probes_all_passes"] == sum(_comb3(k) for k in sizes)
-    mids = list(range(182, 300))  # |N+| of the workgroup-pass tasks
+    mids = list(range(91, 300))  # |N+| of the workgroup-pass tasks
     assert m["tasks"] == m["hub_tasks"] == len(mids)
-    assert m["probes"] == _comb3(300) - _comb3(182)
+    assert m["probes"] == _comb3(300) - _comb3(91)
     assert m["items"] == sum(mids) and m["parts"] == len(mids) and m["table_entries"] == sum(mids)
     assert m["bytes"] == 2 * m["probes"] + 12 * m["items"] + 2 * m["table_entries"] + 16 * m["parts"]
