@@ -813,10 +813,10 @@ IG_DEV void balanced_rows(const T* __restrict__ list, unsigned start, int slen, 
 // stepping by ustep; 16-bit keys when items_narrow), counts the keys in
 // N+(item) found in the table, for the items whose list has at most `limit`
 // entries.  Lists of kLongList or more entries are walked by the whole wave,
-// one at a time, kWalk windows of 64 per step so that their loads overlap;
-// shorter ones share windows of 64 pairs through the in-wave balancer, also
-// kWalk windows per step, the 16-bit rows and the 32-bit rows in turn.
-// Returns the count in lane 0.
+// one at a time, kWalk (bitmap probes: 2 * kWalk) windows of 64 per step so
+// that their loads overlap; shorter ones share windows of 64 pairs through
+// the in-wave balancer, as many windows per step, the 16-bit rows and the
+// 32-bit rows in turn.  Returns the count in lane 0.
 template <typename P>
 IG_DEV unsigned long long count_pairs(const int2* __restrict__ orngk, const int* __restrict__ olist,
                                       const void* items, bool items_narrow, int n, int limit, int ub0, int ustep,
@@ -1050,7 +1050,8 @@ __global__ __launch_bounds__(kTcBlock) IG_SGPR_CAP void tc_count(int nodes, int 
   if (lane == 0 && cnt != 0) atomic_add_relaxed(&ctl->count, cnt);
 }
 
-// One workgroup per part from `parts` (grabbed through *next).  N+(x) is
+// One workgroup per part from `parts` (grabbed through *next).  A hub's
+// N+(x) becomes a bitmap of hub keys in LDS (BitProbe); any other N+(x) is
 // hashed into LDS lds_list (<= kSlots / 4) entries at a time - one piece
 // unless the list is longer - and the part's items are probed against every
 // piece.
