@@ -89,6 +89,30 @@ struct SsspArgs {
 
 IG_DEV int degree_of(const int* __restrict__ nindex, int u) { return nindex[u + 1] - nindex[u]; }
 
+// Mid lists of a near iteration: one wave each, as frontier.hpp's
+// expand_mid, or, while there are few of them, k waves each (k up to 16,
+// while 2 * k waves per list still fit the grid), wave j of a list taking
+// its windows j, j + k, ...  Each window is a chain of device-scope round
+// trips (the vertex's distance, the neighbour's check, the atomicMin), so a
+// 2000-entry list walked by one wave set a small graph's iteration length.
+template <typename F>
+IG_DEV void expand_mid_spread(const FrontierQueues& in, unsigned nm, const int* __restrict__ nindex, F&& f)
+{
+  const int lane = lane_id();
+  const unsigned wpb = blockDim.x / kWave;
+  const unsigned nwaves = gridDim.x * wpb;
+  unsigned k = 1;
+  while (k < 16 && static_cast<unsigned long long>(nm) * k * 2 <= nwaves) k *= 2;
+  for (unsigned i = blockIdx.x * wpb + threadIdx.x / kWave; i < nm * k; i += nwaves) {
+    const int v = ld_relaxed(&in.mq[i / k]);
+    const int beg = nindex[v], end = nindex[v + 1];
+    for (int base = beg + static_cast<int>(i % k) * kWave; base < end; base += static_cast<int>(k) * kWave) {
+      const int e = base + lane;
+      f(v, e, e < end);
+    }
+  }
+}
+
 // Relaxes one CSR entry e of v (distance dv) when `active`: an improvement
 // below the split T joins the next near frontier (deduplicated per iteration
 // by stamp), one at or above T is parked in the far pile (deduplicated per
@@ -563,7 +587,7 @@ __global__ __launch_bounds__(kSsspBlock) __attribute__((amdgpu_waves_per_eu(6, 8
         relax(a, dv, e, active, T, it, epoch, far, far_cnt, s_emit, out, oc);
       };
       expand_huge(in, nh, nch, a.nindex, visit);
-      expand_mid(in, nm, a.nindex, visit);
+      expand_mid_spread(in, nm, a.nindex, visit);
       expand_small(in.q, n, a.nindex, visit);
       se_flush(s_emit, out, oc, far, far_cnt, far_min);
     }
