@@ -54,6 +54,30 @@ def test_sssp_rmat18(ig, orc, ctx):
     assert np.array_equal(ctx.sssp(ig.DeviceGraph(g), 0), orc.sssp(g, 0))
 
 
+def test_sssp_few_mid_lists(ig, orc, ctx):
+    """A source joined to 20 sub-hubs of 3000 leaves each (lists of 64 to
+    4095 entries: mid lists), with weighted cross edges among the leaves:
+    the second near iteration holds only 20 mid lists, which SSSP spreads
+    over up to 16 waves each.  Distances and BFS levels against the oracle."""
+    rng = np.random.default_rng(11)
+    hubs, leaves = 20, 3000
+    n = 1 + hubs + hubs * leaves
+    src = [np.zeros(hubs, np.int64)]
+    dst = [np.arange(1, 1 + hubs)]
+    for h in range(hubs):
+        src.append(np.full(leaves, 1 + h))
+        dst.append(1 + hubs + h * leaves + np.arange(leaves))
+    cross = rng.integers(1 + hubs, n, size=(2, 20000))
+    src.append(cross[0])
+    dst.append(cross[1])
+    s_, d_ = np.concatenate(src), np.concatenate(dst)
+    keep = s_ != d_
+    g = ig.from_edges(n, s_[keep].astype(np.int32), d_[keep].astype(np.int32), wseed=5, wmin=1, wmax=1000)
+    dg = ig.DeviceGraph(g)
+    assert np.array_equal(ctx.sssp(dg, 0), orc.sssp(g, 0))
+    assert np.array_equal(ctx.bfs(dg, 0), orc.bfs(g, 0))
+
+
 def test_cc_catalog(ig, orc, ctx):
     for g in catalog(ig):
         assert np.array_equal(ctx.cc(ig.DeviceGraph(g)), orc.cc(g)), g.name
