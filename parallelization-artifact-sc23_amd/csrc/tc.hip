@@ -810,7 +810,9 @@ IG_DEV void balanced_rows(const T* __restrict__ list, unsigned start, int slen, 
 }
 
 // Wave-collective: over the item keys items[ub0 + 64k + lane] (k = 0, 1, ...
-// stepping by ustep; 16-bit keys when items_narrow), counts the keys in
+// stepping by ustep) of the sequence items_a[0, n_a) (16-bit keys when
+// a_narrow), then items_b[0, n_b), so that one batch of 64 can span both
+// (a task's N+(x) and N-(x) in one chain of round trips), counts the keys in
 // N+(item) found in the table, for the items whose list has at most `limit`
 // entries.  Lists of kLongList or more entries are walked by the whole wave,
 // one at a time, kWalk (bitmap probes: 2 * kWalk) windows of 64 per step so
@@ -819,18 +821,21 @@ IG_DEV void balanced_rows(const T* __restrict__ list, unsigned start, int slen, 
 // 32-bit rows in turn.  Returns the count in lane 0.
 template <typename P>
 IG_DEV unsigned long long count_pairs(const int2* __restrict__ orngk, const int* __restrict__ olist,
-                                      const void* items, bool items_narrow, int n, int limit, int ub0, int ustep,
-                                      const P& pr)
+                                      const void* items_a, bool a_narrow, int n_a, const int* __restrict__ items_b,
+                                      int n_b, int limit, int ub0, int ustep, const P& pr)
 {
   const int lane = lane_id();
   const unsigned short* olist16 = reinterpret_cast<const unsigned short*>(olist);
+  const int n = n_a + n_b;
   Hits h;
   for (int ub = ub0; ub < n; ub += ustep) {
     const int i = ub + lane;
     int key = -1;
-    if (i < n)
-      key = items_narrow ? static_cast<int>(static_cast<const unsigned short*>(items)[i])
-                         : static_cast<const int*>(items)[i];
+    if (i < n_a)
+      key = a_narrow ? static_cast<int>(static_cast<const unsigned short*>(items_a)[i])
+                     : static_cast<const int*>(items_a)[i];
+    else if (i < n)
+      key = items_b[i - n_a];
     const int2 r = key >= 0 ? orngk[key] : make_int2(0, 0);
     const bool nar = narrow_key(key);
     const int len = r.y <= limit ? r.y : 0;
@@ -858,16 +863,11 @@ IG_DEV unsigned long long task_pairs(const int2* __restrict__ orngk, const int* 
                                      const int* __restrict__ ilist, const TcTask& x, int lo, int hi, int ub0,
                                      int ustep, const P& pr)
 {
-  unsigned long long cnt = 0;
-  const int a1 = min(hi, x.d);
-  if (lo < a1) {
-    const void* items = x.narrow ? static_cast<const void*>(reinterpret_cast<const unsigned short*>(olist + x.ob) + lo)
-                                 : static_cast<const void*>(olist + x.ob + lo);
-    cnt += count_pairs(orngk, olist, items, x.narrow, a1 - lo, x.d, ub0, ustep, pr);
-  }
-  const int b0 = max(lo, x.d) - x.d, b1 = hi - x.d;
-  if (b0 < b1) cnt += count_pairs(orngk, olist, ilist + x.ib + b0, false, b1 - b0, x.d, ub0, ustep, pr);
-  return cnt;
+  const int a0 = min(lo, x.d), a1 = min(hi, x.d);
+  const int b0 = max(lo, x.d) - x.d, b1 = max(hi, x.d) - x.d;
+  const void* items_a = x.narrow ? static_cast<const void*>(reinterpret_cast<const unsigned short*>(olist + x.ob) + a0)
+                                 : static_cast<const void*>(olist + x.ob + a0);
+  return count_pairs(orngk, olist, items_a, x.narrow, a1 - a0, ilist + x.ib + b0, b1 - b0, x.d, ub0, ustep, pr);
 }
 
 // Wave-collective: the tiny tasks (|N+(x)| <= kTinyList) of the lanes in
