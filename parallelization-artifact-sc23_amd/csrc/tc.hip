@@ -100,7 +100,7 @@ static_assert(kHugeChunk % kTcBlock == 0, "tc_pack_huge moves whole items");
 int wave_work_for(int edges) { return edges < (1 << 24) ? 1 << 13 : 1 << 14; }
 constexpr int kPartItems = 2048;    // items per workgroup-pass part (1024 and 4096: slower)
 constexpr int kUnroll = 4;          // windows of 64 per step over a long list
-constexpr int kLongList = 96;       // lists walked by the whole wave (64, 160, 256: slower; 128: the same)
+constexpr int kLongList = 128;      // lists walked by the whole wave (64, 96: slower; 160, 192: the same)
 constexpr int kWalk = 4;            // windows of 64 in flight per step of that walk (2 and 8: slower);
                                     // bitmap tasks, whose windows need fewer registers, take 2 * kWalk
 constexpr int kTinyList = 16;       // wave-pass tasks this small are counted kTinyGroup at a time
